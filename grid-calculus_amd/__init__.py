@@ -1,0 +1,384 @@
+"""Python binding of libgridcalc_hip.so, the MI355X engine behind gridcalc's periodic-grid
+operators, reductions and explicit time stepping.
+
+This module is plumbing for the test-suite, the benchmark and scripted use: it calls the
+C ABI declared in include/gridcalc_hip.h through ctypes and moves numpy arrays to and from
+device buffers allocated by the library.  It has no CPU fallback: if the shared library is
+missing it raises, and every computation runs in the HIP kernels.
+
+Arrays follow gridcalc's layout: i-fastest (site (i, j, k) at i + Nx*(j + Ny*k)), i.e.
+C-ordered numpy arrays of shape (Nz, Ny, Nx) for scalar fields, (Nz, Ny, Nx, 3) for Vec3
+fields and (Nz, Ny, Nx, 9) for column-major Mat3 fields.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libgridcalc_hip.so")
+REPO_ROOT = os.path.dirname(PKG_DIR)
+
+# status codes (gc_status)
+OK, ERR_INVALID, ERR_EXTENT, ERR_CFL, ERR_NONPOS_D, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED = range(8)
+F64, F32 = 0, 1
+PAIRWISE, KAHAN = 0, 1
+EULER, RK4 = 0, 1
+
+_NP_DTYPE = {F64: np.float64, F32: np.float32}
+
+from . import inputs  # noqa: E402  (seeded synthetic inputs)
+
+
+class GridcalcError(RuntimeError):
+    """A gc_status other than GC_OK; ``code`` holds the status, the message is the
+    library's gc_last_error() text (which mirrors the reference's exception text)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class Grid(C.Structure):
+    """gc_grid: extents and cell size (core::Grid, core/Grid.hpp:28-148)."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double)]
+
+    @staticmethod
+    def make(nx, ny, nz, h=(1.0, 1.0, 1.0)):
+        if not isinstance(h, (tuple, list)):
+            h = (h, h, h)
+        return Grid(int(nx), int(ny), int(nz), float(h[0]), float(h[1]), float(h[2]))
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def size(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def cell_volume(self):
+        return self.hx * self.hy * self.hz
+
+    def __repr__(self):
+        return f"Grid({self.nx}, {self.ny}, {self.nz}, h=({self.hx}, {self.hy}, {self.hz}))"
+
+
+class CHParams(C.Structure):
+    _fields_ = [("M", C.c_double), ("kappa", C.c_double)]
+
+
+def build(force=False):
+    """Compiles libgridcalc_hip.so in-tree for gfx950 (hipcc via make)."""
+    args = ["make", "-s", "-C", PKG_DIR, "-j8"]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+
+
+_lib = None
+
+
+def lib():
+    """Loads libgridcalc_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, gp, sz, i64p, dp = C.c_void_p, C.POINTER(Grid), C.c_size_t, C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    st = C.c_int
+    sig = {
+        "gc_last_error": ([], C.c_char_p),
+        "gc_version": ([], C.c_int),
+        "gc_device_count": ([C.POINTER(C.c_int)], st),
+        "gc_context_create": ([C.POINTER(vp), vp], st),
+        "gc_context_destroy": ([vp], st),
+        "gc_context_sync": ([vp], st),
+        "gc_context_stream": ([vp], vp),
+        "gc_context_scratch_bytes": ([vp], sz),
+        "gc_malloc": ([C.POINTER(vp), sz], st),
+        "gc_free": ([vp], st),
+        "gc_memcpy_htod": ([vp, vp, vp, sz], st),
+        "gc_memcpy_dtoh": ([vp, vp, vp, sz], st),
+        "gc_memcpy_dtod": ([vp, vp, vp, sz], st),
+        "gc_timer_start": ([vp], st),
+        "gc_timer_stop": ([vp, C.POINTER(C.c_float)], st),
+        "gc_laplacian": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_gradient": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_gradient_vec": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_divergence": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_divergence_mat": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_biharmonic": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_cell_laplacian": ([vp, gp, C.c_int, vp, vp, vp, i64p], st),
+        "gc_first_nonpositive": ([vp, gp, C.c_int, vp, i64p], st),
+        "gc_max_face_d": ([vp, gp, C.c_int, vp, dp], st),
+        "gc_integrate": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
+        "gc_ch_free_energy": ([vp, gp, C.c_int, C.c_double, C.c_int, vp, dp], st),
+        "gc_gradient_energy": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
+        "gc_axpy": ([vp, C.c_int, sz, vp, C.c_double, vp, vp], st),
+        "gc_euler_update": ([vp, C.c_int, sz, vp, C.c_double, vp], st),
+        "gc_rk4_combine": ([vp, C.c_int, sz, vp, C.c_double, vp, vp, vp, vp], st),
+        "gc_ch_rhs": ([vp, gp, C.c_int, C.POINTER(CHParams), vp, vp], st),
+        "gc_ch_integrate": ([vp, gp, C.c_int, C.POINTER(CHParams), vp, C.c_double, C.c_int, C.c_int], st),
+        "gc_diffuse_const": ([vp, gp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_int], st),
+        "gc_diffuse_field": ([vp, gp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int, i64p], st),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+#: every entry point include/gridcalc_hip.h declares (checked by the CPU test-suite)
+def exported_symbols():
+    lib()
+    return sorted(n for n in _SIGNATURES_DOC)
+
+
+_SIGNATURES_DOC = (
+    "gc_last_error gc_version gc_device_count gc_context_create gc_context_destroy "
+    "gc_context_sync gc_context_stream gc_context_scratch_bytes gc_malloc gc_free "
+    "gc_memcpy_htod gc_memcpy_dtoh gc_memcpy_dtod gc_timer_start gc_timer_stop gc_laplacian "
+    "gc_gradient gc_gradient_vec gc_divergence gc_divergence_mat gc_biharmonic "
+    "gc_cell_laplacian gc_first_nonpositive gc_max_face_d gc_integrate gc_ch_free_energy "
+    "gc_gradient_energy gc_axpy gc_euler_update gc_rk4_combine gc_ch_rhs gc_ch_integrate "
+    "gc_diffuse_const gc_diffuse_field").split()
+
+
+def _check(rc):
+    if rc != OK:
+        msg = lib().gc_last_error().decode("utf-8", "replace")
+        raise GridcalcError(rc, msg)
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().gc_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """A gc_context: one HIP stream plus the scratch memory the fused kernels use."""
+
+    def __init__(self, stream=None):
+        self._h = C.c_void_p()
+        _check(lib().gc_context_create(C.byref(self._h), stream))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self):
+        _check(lib().gc_context_sync(self._h))
+
+    def scratch_bytes(self):
+        return lib().gc_context_scratch_bytes(self._h)
+
+    def timer_start(self):
+        _check(lib().gc_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(lib().gc_timer_stop(self._h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self._h:
+            lib().gc_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DeviceArray:
+    """A device buffer owned by Python (gc_malloc / gc_free) with a numpy shape/dtype."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._p = C.c_void_p()
+        _check(lib().gc_malloc(C.byref(self._p), self.nbytes))
+
+    @property
+    def ptr(self):
+        return self._p
+
+    @property
+    def gc_dtype(self):
+        return F64 if self.dtype == np.float64 else F32
+
+    @staticmethod
+    def from_numpy(ctx, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype if dtype is not None else a.dtype)
+        d = DeviceArray(a.shape, a.dtype)
+        _check(lib().gc_memcpy_htod(ctx.handle, d.ptr, a.ctypes.data_as(C.c_void_p), d.nbytes))
+        return d
+
+    def upload(self, ctx, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().gc_memcpy_htod(ctx.handle, self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def numpy(self, ctx):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().gc_memcpy_dtoh(ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, ctx, other):
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().gc_memcpy_dtod(ctx.handle, self.ptr, other.ptr, self.nbytes))
+
+    def free(self):
+        if self._p:
+            lib().gc_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dt(a):
+    return a.gc_dtype
+
+
+# ---- operators ------------------------------------------------------------------------
+
+def laplacian(ctx, g, f, order=2, out=None):
+    out = out or DeviceArray(g.shape, f.dtype)
+    _check(lib().gc_laplacian(ctx.handle, C.byref(g), _dt(f), order, f.ptr, out.ptr))
+    return out
+
+
+def gradient(ctx, g, f, order=2, out=None):
+    out = out or DeviceArray(g.shape + (3,), f.dtype)
+    _check(lib().gc_gradient(ctx.handle, C.byref(g), _dt(f), order, f.ptr, out.ptr))
+    return out
+
+
+def gradient_vec(ctx, g, v, order=2, out=None):
+    out = out or DeviceArray(g.shape + (9,), v.dtype)
+    _check(lib().gc_gradient_vec(ctx.handle, C.byref(g), _dt(v), order, v.ptr, out.ptr))
+    return out
+
+
+def divergence(ctx, g, v, order=2, out=None):
+    out = out or DeviceArray(g.shape, v.dtype)
+    _check(lib().gc_divergence(ctx.handle, C.byref(g), _dt(v), order, v.ptr, out.ptr))
+    return out
+
+
+def divergence_mat(ctx, g, m, order=2, out=None):
+    out = out or DeviceArray(g.shape + (3,), m.dtype)
+    _check(lib().gc_divergence_mat(ctx.handle, C.byref(g), _dt(m), order, m.ptr, out.ptr))
+    return out
+
+
+def biharmonic(ctx, g, f, order=2, out=None):
+    out = out or DeviceArray(g.shape, f.dtype)
+    _check(lib().gc_biharmonic(ctx.handle, C.byref(g), _dt(f), order, f.ptr, out.ptr))
+    return out
+
+
+def cell_laplacian(ctx, g, psi, D, out=None):
+    out = out or DeviceArray(g.shape, psi.dtype)
+    bad = C.c_int64(-1)
+    _check(lib().gc_cell_laplacian(ctx.handle, C.byref(g), _dt(psi), psi.ptr, D.ptr, out.ptr, C.byref(bad)))
+    return out
+
+
+def first_nonpositive(ctx, g, D):
+    bad = C.c_int64(-1)
+    _check(lib().gc_first_nonpositive(ctx.handle, C.byref(g), _dt(D), D.ptr, C.byref(bad)))
+    return bad.value
+
+
+def max_face_d(ctx, g, D):
+    r = C.c_double()
+    _check(lib().gc_max_face_d(ctx.handle, C.byref(g), _dt(D), D.ptr, C.byref(r)))
+    return r.value
+
+
+# ---- reductions -------------------------------------------------------------------------
+
+def integrate(ctx, g, f, method=PAIRWISE):
+    r = C.c_double()
+    _check(lib().gc_integrate(ctx.handle, C.byref(g), _dt(f), method, f.ptr, C.byref(r)))
+    return r.value
+
+
+def ch_free_energy(ctx, g, psi, kappa, method=PAIRWISE):
+    r = C.c_double()
+    _check(lib().gc_ch_free_energy(ctx.handle, C.byref(g), _dt(psi), kappa, method, psi.ptr, C.byref(r)))
+    return r.value
+
+
+def gradient_energy(ctx, g, psi, method=PAIRWISE):
+    r = C.c_double()
+    _check(lib().gc_gradient_energy(ctx.handle, C.byref(g), _dt(psi), method, psi.ptr, C.byref(r)))
+    return r.value
+
+
+# ---- time stepping ------------------------------------------------------------------------
+
+def axpy(ctx, a, scale, b, out):
+    n = int(np.prod(a.shape))
+    _check(lib().gc_axpy(ctx.handle, _dt(a), n, a.ptr, scale, b.ptr, out.ptr))
+
+
+def euler_update(ctx, psi, dt, r):
+    n = int(np.prod(psi.shape))
+    _check(lib().gc_euler_update(ctx.handle, _dt(psi), n, psi.ptr, dt, r.ptr))
+
+
+def rk4_combine(ctx, psi, dt, k1, k2, k3, k4):
+    n = int(np.prod(psi.shape))
+    _check(lib().gc_rk4_combine(ctx.handle, _dt(psi), n, psi.ptr, dt, k1.ptr, k2.ptr, k3.ptr, k4.ptr))
+
+
+def ch_rhs(ctx, g, psi, M=1.0, kappa=1.0, out=None):
+    out = out or DeviceArray(g.shape, psi.dtype)
+    P = CHParams(M, kappa)
+    _check(lib().gc_ch_rhs(ctx.handle, C.byref(g), _dt(psi), C.byref(P), psi.ptr, out.ptr))
+    return out
+
+
+def ch_integrate(ctx, g, psi, M, kappa, dt, n_steps, integrator=RK4):
+    """In place: psi advances n_steps of the fused Cahn-Hilliard integrator."""
+    P = CHParams(M, kappa)
+    _check(lib().gc_ch_integrate(ctx.handle, C.byref(g), _dt(psi), C.byref(P), psi.ptr, dt,
+                                 n_steps, integrator))
+
+
+def diffuse_const(ctx, g, psi, D, dt, n_steps, integrator=EULER):
+    _check(lib().gc_diffuse_const(ctx.handle, C.byref(g), _dt(psi), psi.ptr, D, dt, n_steps, integrator))
+
+
+def diffuse_field(ctx, g, psi, D, dt, n_steps, integrator=EULER):
+    bad = C.c_int64(-1)
+    _check(lib().gc_diffuse_field(ctx.handle, C.byref(g), _dt(psi), psi.ptr, D.ptr, dt, n_steps,
+                                  integrator, C.byref(bad)))

@@ -1,0 +1,448 @@
+// gc_rk4.hip -- explicit time stepping: the pointwise integrator steps of the generic
+// right-hand-side path and the fused stage pipeline for the Cahn-Hilliard, constant-D
+// diffusion and heterogeneous-D (FVM) right-hand sides.
+//
+// Stage pipeline (solve/RK4.hpp:122-153).  The reference computes per step
+//   k1 = f(p); k2 = f(p + dt/2 k1); k3 = f(p + dt/2 k2); k4 = f(p + dt k3)
+//   p += dt/6 (k1 + k4) + dt/3 (k2 + k3)
+// Here each stage is ONE kernel that evaluates the right-hand side at a site and
+// applies that stage's update in the same pass:
+//   S1: y=p  -> a = k1,          yA = p + dt/2 k1
+//   S2: y=yA -> b = k2,          yB = p + dt/2 k2
+//   S3: y=yB -> b = b + k3,      yA = p + dt k3
+//   S4: y=yA -> p = p + (dt/6 (a + k4) + dt/3 b)
+// which performs exactly the reference's roundings (k2 + k3 is formed once, as the
+// reference forms p2 + p3).  HBM traffic per step: 3 + 4 + 5 + 5 = 17 words per site.
+#include "gc_common.hpp"
+
+namespace gc {
+
+gc_status launch_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                           const void* in, void* out);
+gc_status first_nonpositive(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* D,
+                            int64_t* first_bad);
+gc_status max_face_d(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* D, double* dmax);
+
+namespace {
+
+constexpr int TX = 64, TY = 4;
+
+// ---- pointwise steps --------------------------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(256) void k_axpy(size_t n, const T* __restrict__ a, T scale,
+                                              const T* __restrict__ b, T* __restrict__ out) {
+  const size_t G = (size_t)gridDim.x * blockDim.x;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += G)
+    out[c] = a[c] + scale * b[c];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_euler_update(size_t n, T* __restrict__ p, T dt,
+                                                      const T* __restrict__ r) {
+  const size_t G = (size_t)gridDim.x * blockDim.x;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += G)
+    p[c] += dt * r[c];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_rk4_combine(size_t n, T* __restrict__ p, T sixth,
+                                                     T third, const T* __restrict__ k1,
+                                                     const T* __restrict__ k2,
+                                                     const T* __restrict__ k3,
+                                                     const T* __restrict__ k4) {
+  const size_t G = (size_t)gridDim.x * blockDim.x;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += G)
+    p[c] += sixth * (k1[c] + k4[c]) + third * (k2[c] + k3[c]);
+}
+
+inline unsigned pw_blocks(size_t n) {
+  const size_t want = (n + 255) / 256;
+  return (unsigned)(want < 8192 ? (want == 0 ? 1 : want) : 8192);
+}
+
+// ---- right-hand sides ---------------------------------------------------------------------
+
+struct Site {
+  int i, j, k;
+};
+
+__device__ __forceinline__ size_t lin(const Dims& d, int i, int j, int k) {
+  return (size_t)i + (size_t)d.nx * (size_t)j + (size_t)d.plane * (size_t)k;
+}
+
+// Order-2 Laplacian of y at (i,j,k) (diff/Laplacian.hpp:60-70).
+template <class T>
+__device__ __forceinline__ T lap2_at(const T* __restrict__ y, const Dims& d, int i, int j, int k,
+                                     T ihx2, T ihy2, T ihz2) {
+  const int im = wrap(i - 1, d.nx), ip = wrap(i + 1, d.nx);
+  const int jm = wrap(j - 1, d.ny), jp = wrap(j + 1, d.ny);
+  const int km = wrap(k - 1, d.nz), kp = wrap(k + 1, d.nz);
+  const T c = y[lin(d, i, j, k)];
+  T dxx = 0, dyy = 0, dzz = 0;
+  dxx += T(1) * y[lin(d, im, j, k)];
+  dyy += T(1) * y[lin(d, i, jm, k)];
+  dzz += T(1) * y[lin(d, i, j, km)];
+  dxx += T(-2) * c;
+  dyy += T(-2) * c;
+  dzz += T(-2) * c;
+  dxx += T(1) * y[lin(d, ip, j, k)];
+  dyy += T(1) * y[lin(d, i, jp, k)];
+  dzz += T(1) * y[lin(d, i, j, kp)];
+  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+}
+
+// Cahn-Hilliard: M * lap(mu), mu = -y + y^3 - kappa*lap(y)
+// (examples/common/cahn_hilliard.hpp:41-57).
+template <class T>
+struct CHRhs {
+  T M, kappa, ihx2, ihy2, ihz2;
+  __device__ __forceinline__ T mu(const T* __restrict__ y, const Dims& d, int i, int j,
+                                  int k) const {
+    const T l = lap2_at(y, d, i, j, k, ihx2, ihy2, ihz2);
+    const T p = y[lin(d, i, j, k)];
+    return -p + p * p * p - kappa * l;
+  }
+  __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
+                                          int k) const {
+    const T mc = mu(y, d, i, j, k);
+    const T mxm = mu(y, d, wrap(i - 1, d.nx), j, k), mxp = mu(y, d, wrap(i + 1, d.nx), j, k);
+    const T mym = mu(y, d, i, wrap(j - 1, d.ny), k), myp = mu(y, d, i, wrap(j + 1, d.ny), k);
+    const T mzm = mu(y, d, i, j, wrap(k - 1, d.nz)), mzp = mu(y, d, i, j, wrap(k + 1, d.nz));
+    T dxx = 0, dyy = 0, dzz = 0;
+    dxx += T(1) * mxm;
+    dyy += T(1) * mym;
+    dzz += T(1) * mzm;
+    dxx += T(-2) * mc;
+    dyy += T(-2) * mc;
+    dzz += T(-2) * mc;
+    dxx += T(1) * mxp;
+    dyy += T(1) * myp;
+    dzz += T(1) * mzp;
+    const T lap = dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+    return lap * M;
+  }
+};
+
+// solve/Diffusion.hpp:108 -- laplacian<2>
+template <class T>
+struct LapRhs {
+  T ihx2, ihy2, ihz2;
+  __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
+                                          int k) const {
+    return lap2_at(y, d, i, j, k, ihx2, ihy2, ihz2);
+  }
+};
+
+// solve/Diffusion.hpp:212 -- fvm::cellLaplacian (fvm/CellLaplacian.hpp:119-138)
+template <class T>
+struct FvmRhs {
+  const T* __restrict__ D;
+  T ihx2, ihy2, ihz2;
+  __device__ __forceinline__ static T hm(T a, T b) { return T(2) * a * b / (a + b); }
+  __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
+                                          int k) const {
+    const size_t c = lin(d, i, j, k);
+    const size_t xp = lin(d, wrap(i + 1, d.nx), j, k), xm = lin(d, wrap(i - 1, d.nx), j, k);
+    const size_t yp = lin(d, i, wrap(j + 1, d.ny), k), ym = lin(d, i, wrap(j - 1, d.ny), k);
+    const size_t zp = lin(d, i, j, wrap(k + 1, d.nz)), zm = lin(d, i, j, wrap(k - 1, d.nz));
+    const T pc = y[c], Dc = D[c];
+    const T dxp = hm(Dc, D[xp]), dxm = hm(D[xm], Dc);
+    const T dyp = hm(Dc, D[yp]), dym = hm(D[ym], Dc);
+    const T dzp = hm(Dc, D[zp]), dzm = hm(D[zm], Dc);
+    const T rx = (dxp * (y[xp] - pc) - dxm * (pc - y[xm])) * ihx2;
+    const T ry = (dyp * (y[yp] - pc) - dym * (pc - y[ym])) * ihy2;
+    const T rz = (dzp * (y[zp] - pc) - dzm * (pc - y[zm])) * ihz2;
+    return rx + ry + rz;
+  }
+};
+
+// ---- fused stage kernel ---------------------------------------------------------------------
+
+enum Stage : int { kRhsOnly = 0, kS1 = 1, kS2 = 2, kS3 = 3, kS4 = 4, kEuler = 5 };
+
+template <class T>
+struct StageCoef {
+  T half_dt, dt, sixth_dt, third_dt;
+};
+
+template <class T, class Rhs>
+__global__ __launch_bounds__(256) void k_stage(Rhs rhs, int stage, const T* __restrict__ y,
+                                               T* psi, T* __restrict__ ynext, T* __restrict__ a,
+                                               T* __restrict__ b, Dims d, int nbx, int nby,
+                                               StageCoef<T> co) {
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = (int)(bid % (unsigned)nbx);
+  const unsigned r = bid / (unsigned)nbx;
+  const int by = (int)(r % (unsigned)nby);
+  const int k = (int)(r / (unsigned)nby);
+  const int i = bx * TX + (int)(threadIdx.x % TX);
+  const int j = by * TY + (int)(threadIdx.x / TX);
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = lin(d, i, j, k);
+  const T kv = rhs(y, d, i, j, k);
+  switch (stage) {
+    case kRhsOnly:
+      ynext[c] = kv;
+      break;
+    case kS1:
+      a[c] = kv;
+      ynext[c] = psi[c] + co.half_dt * kv;
+      break;
+    case kS2:
+      b[c] = kv;
+      ynext[c] = psi[c] + co.half_dt * kv;
+      break;
+    case kS3:
+      b[c] = b[c] + kv;
+      ynext[c] = psi[c] + co.dt * kv;
+      break;
+    case kS4:
+      psi[c] = psi[c] + (co.sixth_dt * (a[c] + kv) + co.third_dt * b[c]);
+      break;
+    default:  // explicit Euler: ynext = y + dt*f(y)
+      ynext[c] = y[c] + co.dt * kv;
+      break;
+  }
+}
+
+template <class T, class Rhs>
+gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int stage, const T* y,
+                       T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  const Dims d = dims_of(g);
+  const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;
+  const dim3 grid((unsigned)(nbx * nby * d.nz));
+  hipLaunchKernelGGL((k_stage<T, Rhs>), grid, dim3(TX * TY), 0, ctx->stream, rhs, stage, y, psi,
+                     ynext, a, b, d, nbx, nby, co);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// Runs n_steps of RK4 or explicit Euler with the fused stage kernels.  psi in place.
+template <class T, class Rhs>
+gc_status run_steps(gc_context* ctx, const gc_grid* g, const Rhs& rhs, T* psi, double dt,
+                    int n_steps, int integrator) {
+  if (n_steps == 0) return GC_OK;
+  const size_t n = sites(g);
+  StageCoef<T> co;
+  // solve/RK4.hpp:123-125 (in double, then rounded to T)
+  co.half_dt = T(0.5 * dt);
+  co.dt = T(dt);
+  co.sixth_dt = T(dt / 6.0);
+  co.third_dt = T(dt / 3.0);
+  void* scr = nullptr;
+  if (integrator == GC_RK4) {
+    GC_TRY(scratch(ctx, 4 * n * sizeof(T), &scr));
+    T* yA = (T*)scr;
+    T* yB = yA + n;
+    T* a = yB + n;
+    T* b = a + n;
+    for (int s = 0; s < n_steps; ++s) {
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS1, psi, psi, yA, a, b, co));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS2, yA, psi, yB, a, b, co));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS3, yB, psi, yA, a, b, co));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS4, yA, psi, nullptr, a, b, co));
+    }
+    return GC_OK;
+  }
+  GC_TRY(scratch(ctx, n * sizeof(T), &scr));
+  T* cur = psi;
+  T* other = (T*)scr;
+  for (int s = 0; s < n_steps; ++s) {
+    GC_TRY(launch_stage<T>(ctx, g, rhs, kEuler, cur, nullptr, other, nullptr, nullptr, co));
+    T* t = cur;
+    cur = other;
+    other = t;
+  }
+  if (cur != psi) GC_HIP(hipMemcpyAsync(psi, cur, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  return GC_OK;
+}
+
+gc_status check_steps(const char* who, double dt, int n_steps) {
+  if (n_steps < 0)
+    return set_error(GC_ERR_INVALID, "%s: n_steps must be >= 0; got %d", who, n_steps);
+  if (dt < 0.0) return set_error(GC_ERR_INVALID, "%s: dt must be >= 0; got %f", who, dt);
+  return GC_OK;
+}
+
+const char* integ_name(int integrator) {
+  return integrator == GC_RK4 ? "solve::integrate(... RK4)" : "solve::integrate(... ExplicitEuler)";
+}
+
+gc_status base_checks(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* psi,
+                      int integrator) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_grid(g));
+  GC_TRY(check_dtype(dt));
+  if (!psi) return set_error(GC_ERR_INVALID, "gridcalc: null field pointer");
+  if (integrator != GC_RK4 && integrator != GC_EULER)
+    return set_error(GC_ERR_INVALID, "gridcalc: unknown integrator %d", integrator);
+  return GC_OK;
+}
+
+template <class T>
+CHRhs<T> make_ch(const gc_grid* g, const gc_ch_params* p) {
+  return CHRhs<T>{T(p->M), T(p->kappa), T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)),
+                  T(1.0 / (g->hz * g->hz))};
+}
+template <class T>
+LapRhs<T> make_lap(const gc_grid* g) {
+  return LapRhs<T>{T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)), T(1.0 / (g->hz * g->hz))};
+}
+template <class T>
+FvmRhs<T> make_fvm(const gc_grid* g, const void* D) {
+  return FvmRhs<T>{(const T*)D, T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)),
+                   T(1.0 / (g->hz * g->hz))};
+}
+
+// solve/Diffusion.hpp:45-59
+gc_status check_cfl(const gc_grid* g, double D, double dt, int integrator) {
+  const double limit = integrator == GC_RK4 ? 0.6963 : 0.5;
+  const double inv_sum = 1.0 / (g->hx * g->hx) + 1.0 / (g->hy * g->hy) + 1.0 / (g->hz * g->hz);
+  const double cfl = D * dt * inv_sum;
+  if (cfl > limit)
+    return set_error(GC_ERR_CFL,
+                     "solve::diffuse: explicit-diffusion CFL violated for %s: D=%f, dt=%f, "
+                     "h=(%f, %f, %f); D*dt*(1/hx^2 + 1/hy^2 + 1/hz^2) = %f > %f",
+                     integrator == GC_RK4 ? "RK4" : "ExplicitEuler", D, dt, g->hx, g->hy, g->hz,
+                     cfl, limit);
+  return GC_OK;
+}
+
+}  // namespace
+}  // namespace gc
+
+using namespace gc;
+
+extern "C" {
+
+gc_status gc_axpy(gc_context* ctx, gc_dtype dt, size_t n, const void* a, double scale,
+                  const void* b, void* out) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_dtype(dt));
+  if (n == 0) return GC_OK;
+  if (!a || !b || !out) return set_error(GC_ERR_INVALID, "gridcalc: null field pointer");
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_axpy<double>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (const double*)a, scale, (const double*)b, (double*)out);
+  else
+    hipLaunchKernelGGL((k_axpy<float>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (const float*)a, (float)scale, (const float*)b, (float*)out);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status gc_euler_update(gc_context* ctx, gc_dtype dt, size_t n, void* psi, double step,
+                          const void* r) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_dtype(dt));
+  if (n == 0) return GC_OK;
+  if (!psi || !r) return set_error(GC_ERR_INVALID, "gridcalc: null field pointer");
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_euler_update<double>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (double*)psi, step, (const double*)r);
+  else
+    hipLaunchKernelGGL((k_euler_update<float>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (float*)psi, (float)step, (const float*)r);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status gc_rk4_combine(gc_context* ctx, gc_dtype dt, size_t n, void* psi, double step,
+                         const void* k1, const void* k2, const void* k3, const void* k4) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_dtype(dt));
+  if (n == 0) return GC_OK;
+  if (!psi || !k1 || !k2 || !k3 || !k4)
+    return set_error(GC_ERR_INVALID, "gridcalc: null field pointer");
+  const double sixth = step / 6.0, third = step / 3.0;
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_rk4_combine<double>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (double*)psi, sixth, third, (const double*)k1, (const double*)k2,
+                       (const double*)k3, (const double*)k4);
+  else
+    hipLaunchKernelGGL((k_rk4_combine<float>), dim3(pw_blocks(n)), dim3(256), 0, ctx->stream, n,
+                       (float*)psi, (float)sixth, (float)third, (const float*)k1,
+                       (const float*)k2, (const float*)k3, (const float*)k4);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status gc_ch_rhs(gc_context* ctx, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
+                    const void* psi, void* out) {
+  GC_TRY(base_checks(ctx, g, dt, psi, GC_RK4));
+  if (!p || !out) return set_error(GC_ERR_INVALID, "gridcalc: null argument");
+  if (out == psi) return set_error(GC_ERR_INVALID, "gridcalc: output must not alias the input");
+  GC_TRY(check_extent(g, 1));
+  if (dt == GC_F64)
+    return launch_stage<double>(ctx, g, make_ch<double>(g, p), kRhsOnly, (const double*)psi,
+                                nullptr, (double*)out, nullptr, nullptr, StageCoef<double>{});
+  return launch_stage<float>(ctx, g, make_ch<float>(g, p), kRhsOnly, (const float*)psi, nullptr,
+                             (float*)out, nullptr, nullptr, StageCoef<float>{});
+}
+
+gc_status gc_ch_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
+                          void* psi, double step, int n_steps, int integrator) {
+  GC_TRY(base_checks(ctx, g, dt, psi, integrator));
+  if (!p) return set_error(GC_ERR_INVALID, "gridcalc: null parameters");
+  GC_TRY(check_steps(integ_name(integrator), step, n_steps));
+  if (n_steps == 0) return GC_OK;
+  GC_TRY(check_extent(g, 1));
+  if (dt == GC_F64)
+    return run_steps<double>(ctx, g, make_ch<double>(g, p), (double*)psi, step, n_steps, integrator);
+  return run_steps<float>(ctx, g, make_ch<float>(g, p), (float*)psi, step, n_steps, integrator);
+}
+
+gc_status gc_diffuse_const(gc_context* ctx, const gc_grid* g, gc_dtype dt, void* psi, double D,
+                           double step, int n_steps, int integrator) {
+  GC_TRY(base_checks(ctx, g, dt, psi, integrator));
+  // solve/Diffusion.hpp:95-105
+  if (D < 0.0) return set_error(GC_ERR_INVALID, "solve::diffuse: D must be >= 0; got %f", D);
+  if (step < 0.0) return set_error(GC_ERR_INVALID, "solve::diffuse: dt must be >= 0; got %f", step);
+  if (n_steps < 0)
+    return set_error(GC_ERR_INVALID, "solve::diffuse: n_steps must be >= 0; got %d", n_steps);
+  GC_TRY(check_cfl(g, D, step, integrator));
+  if (n_steps == 0) return GC_OK;
+  GC_TRY(check_extent(g, 1));
+  // D folded into the step (Diffusion.hpp:106-111)
+  const double eff = D * step;
+  if (dt == GC_F64)
+    return run_steps<double>(ctx, g, make_lap<double>(g), (double*)psi, eff, n_steps, integrator);
+  return run_steps<float>(ctx, g, make_lap<float>(g), (float*)psi, eff, n_steps, integrator);
+}
+
+gc_status gc_diffuse_field(gc_context* ctx, const gc_grid* g, gc_dtype dt, void* psi,
+                           const void* D, double step, int n_steps, int integrator,
+                           int64_t* first_bad) {
+  GC_TRY(base_checks(ctx, g, dt, psi, integrator));
+  if (!D) return set_error(GC_ERR_INVALID, "gridcalc: null D pointer");
+  if (D == psi) return set_error(GC_ERR_INVALID, "gridcalc: D must not alias psi");
+  if (first_bad) *first_bad = -1;
+  // solve/Diffusion.hpp:167-174
+  if (step < 0.0) return set_error(GC_ERR_INVALID, "solve::diffuse: dt must be >= 0; got %f", step);
+  if (n_steps < 0)
+    return set_error(GC_ERR_INVALID, "solve::diffuse: n_steps must be >= 0; got %d", n_steps);
+  double dmax = 0.0;
+  GC_TRY(max_face_d(ctx, g, dt, D, &dmax));
+  GC_TRY(check_cfl(g, dmax, step, integrator));
+  if (n_steps == 0) return GC_OK;
+  // the first right-hand-side evaluation runs cellLaplacian's checks
+  // (fvm/CellLaplacian.hpp:95-109); D is constant over the run, so once suffices
+  GC_TRY(check_extent(g, 1));
+  int64_t bad = -1;
+  GC_TRY(first_nonpositive(ctx, g, dt, D, &bad));
+  if (bad >= 0) {
+    if (first_bad) *first_bad = bad;
+    return set_error(GC_ERR_NONPOS_D,
+                     "fvm::cellLaplacian: D must be strictly positive at every cell (the "
+                     "harmonic-mean face-D averaging is undefined otherwise); first violation "
+                     "at flat index %lld",
+                     (long long)bad);
+  }
+  if (dt == GC_F64)
+    return run_steps<double>(ctx, g, make_fvm<double>(g, D), (double*)psi, step, n_steps,
+                             integrator);
+  return run_steps<float>(ctx, g, make_fvm<float>(g, D), (float*)psi, step, n_steps, integrator);
+}
+
+}  // extern "C"

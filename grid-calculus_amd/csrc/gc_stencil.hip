@@ -1,0 +1,447 @@
+// gc_stencil.hip -- the stencil engine: Laplacian, gradient, divergence, biharmonic
+// and the finite-volume cell Laplacian on periodic grids.
+//
+// Every operator keeps the reference's arithmetic: per-axis partial sums accumulated
+// in stencil-offset order starting from 0, then the scaled axis terms summed left to
+// right; this file is compiled with -ffp-contract=off, so double-precision results
+// equal the CPU path bit for bit.
+//
+// Layout: one thread per site.  A 64 x 4 thread block covers a 64 x 4 tile of one
+// xy-plane; block ids are remapped so each XCD owns a contiguous range of tiles
+// (neighbouring rows and planes meet in the same L2).  Neighbours come through the
+// L1/L2 caches: the operators move 2-4 words per site and are bounded by HBM.
+#include "gc_common.hpp"
+
+namespace gc {
+namespace {
+
+constexpr int TX = 64, TY = 4;
+
+struct Tile {
+  int i, j, k;
+  bool valid;
+};
+
+// Maps the (remapped) linear block id to a site.  Grid is 1-D: nbx*nby*nz blocks.
+__device__ __forceinline__ Tile tile_site(const Dims& d, int nbx, int nby) {
+  const unsigned nb = gridDim.x;
+  const unsigned b = xcd_remap(blockIdx.x, nb);
+  const int bx = (int)(b % (unsigned)nbx);
+  const unsigned r = b / (unsigned)nbx;
+  const int by = (int)(r % (unsigned)nby);
+  const int bz = (int)(r / (unsigned)nby);
+  Tile t;
+  t.i = bx * TX + (int)(threadIdx.x % TX);
+  t.j = by * TY + (int)(threadIdx.x / TX);
+  t.k = bz;
+  t.valid = t.i < d.nx && t.j < d.ny;
+  return t;
+}
+
+struct Launch {
+  dim3 grid, block;
+  int nbx, nby;
+};
+inline Launch launch_for(const Dims& d) {
+  Launch L;
+  L.nbx = (d.nx + TX - 1) / TX;
+  L.nby = (d.ny + TY - 1) / TY;
+  L.grid = dim3((unsigned)(L.nbx * L.nby * d.nz));
+  L.block = dim3(TX * TY);
+  return L;
+}
+
+// Neighbour address helpers.
+struct Nb {
+  const Dims& d;
+  int i, j, k;
+  __device__ __forceinline__ size_t at(int di, int dj, int dk) const {
+    return (size_t)wrap(i + di, d.nx) + (size_t)d.nx * (size_t)wrap(j + dj, d.ny) +
+           (size_t)d.plane * (size_t)wrap(k + dk, d.nz);
+  }
+  __device__ __forceinline__ size_t c() const {
+    return (size_t)i + (size_t)d.nx * (size_t)j + (size_t)d.plane * (size_t)k;
+  }
+};
+
+// diff/Laplacian.hpp:56-71
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_laplacian(const T* __restrict__ f, T* __restrict__ out,
+                                                   Dims d, int nbx, int nby, T ihx2, T ihy2,
+                                                   T ihz2) {
+  using S = D2<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T dxx = 0, dyy = 0, dzz = 0;
+#pragma unroll
+  for (int m = 0; m < S::N; ++m) {
+    const int o = m - S::R;
+    const T w = T(S::w(m));
+    dxx += w * f[nb.at(o, 0, 0)];
+    dyy += w * f[nb.at(0, o, 0)];
+    dzz += w * f[nb.at(0, 0, o)];
+  }
+  out[nb.c()] = dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+}
+
+// diff/Gradient.hpp:56-70 (scalar -> Vec3, AoS)
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_gradient(const T* __restrict__ f, T* __restrict__ out,
+                                                  Dims d, int nbx, int nby, T ihx, T ihy,
+                                                  T ihz) {
+  using S = D1<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+  for (int m = 0; m < S::N; ++m) {
+    const int o = m - S::R;
+    const T w = T(S::w(m));
+    dx += w * f[nb.at(o, 0, 0)];
+    dy += w * f[nb.at(0, o, 0)];
+    dz += w * f[nb.at(0, 0, o)];
+  }
+  T* o3 = out + 3 * nb.c();
+  o3[0] = dx * ihx;
+  o3[1] = dy * ihy;
+  o3[2] = dz * ihz;
+}
+
+// diff/Gradient.hpp:113-134 (Vec3 -> Mat3, column-major: M(comp, axis) at comp + 3*axis)
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_gradient_vec(const T* __restrict__ v,
+                                                      T* __restrict__ out, Dims d, int nbx,
+                                                      int nby, T ihx, T ihy, T ihz) {
+  using S = D1<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T* o9 = out + 9 * nb.c();
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+    for (int m = 0; m < S::N; ++m) {
+      const int o = m - S::R;
+      const T w = T(S::w(m));
+      dx += w * v[3 * nb.at(o, 0, 0) + comp];
+      dy += w * v[3 * nb.at(0, o, 0) + comp];
+      dz += w * v[3 * nb.at(0, 0, o) + comp];
+    }
+    o9[comp + 0] = dx * ihx;
+    o9[comp + 3] = dy * ihy;
+    o9[comp + 6] = dz * ihz;
+  }
+}
+
+// diff/Divergence.hpp:56-69 (Vec3 -> scalar)
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_divergence(const T* __restrict__ v, T* __restrict__ out,
+                                                    Dims d, int nbx, int nby, T ihx, T ihy,
+                                                    T ihz) {
+  using S = D1<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T ax = 0, ay = 0, az = 0;
+#pragma unroll
+  for (int m = 0; m < S::N; ++m) {
+    const int o = m - S::R;
+    const T w = T(S::w(m));
+    ax += w * v[3 * nb.at(o, 0, 0) + 0];
+    ay += w * v[3 * nb.at(0, o, 0) + 1];
+    az += w * v[3 * nb.at(0, 0, o) + 2];
+  }
+  out[nb.c()] = ax * ihx + ay * ihy + az * ihz;
+}
+
+// diff/Divergence.hpp:113-131 (Mat3 -> Vec3)
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_divergence_mat(const T* __restrict__ m9,
+                                                        T* __restrict__ out, Dims d, int nbx,
+                                                        int nby, T ihx, T ihy, T ihz) {
+  using S = D1<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T* o3 = out + 3 * nb.c();
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+    for (int m = 0; m < S::N; ++m) {
+      const int o = m - S::R;
+      const T w = T(S::w(m));
+      dx += w * m9[9 * nb.at(o, 0, 0) + comp + 0];
+      dy += w * m9[9 * nb.at(0, o, 0) + comp + 3];
+      dz += w * m9[9 * nb.at(0, 0, o) + comp + 6];
+    }
+    o3[comp] = dx * ihx + dy * ihy + dz * ihz;
+  }
+}
+
+// diff/Biharmonic.hpp:67-101: S4 loop, mixed loop (mb outer, ma inner), final
+// expression in the reference's order.
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_biharmonic(const T* __restrict__ f, T* __restrict__ out,
+                                                    Dims d, int nbx, int nby, T ihx2, T ihy2,
+                                                    T ihz2, T ihx4, T ihy4, T ihz4) {
+  using S2 = D2<O>;
+  using S4 = D4<O>;
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  T dx4 = 0, dy4 = 0, dz4 = 0;
+#pragma unroll
+  for (int m = 0; m < S4::N; ++m) {
+    const int o = m - S4::R;
+    const T w = T(S4::w(m));
+    dx4 += w * f[nb.at(o, 0, 0)];
+    dy4 += w * f[nb.at(0, o, 0)];
+    dz4 += w * f[nb.at(0, 0, o)];
+  }
+  T dxy = 0, dxz = 0, dyz = 0;
+#pragma unroll
+  for (int mb = 0; mb < S2::N; ++mb) {
+    const int ob = mb - S2::R;
+    const double wb = S2::w(mb);
+#pragma unroll
+    for (int ma = 0; ma < S2::N; ++ma) {
+      const int oa = ma - S2::R;
+      const T wab = T(S2::w(ma) * wb);
+      dxy += wab * f[nb.at(oa, ob, 0)];
+      dxz += wab * f[nb.at(oa, 0, ob)];
+      dyz += wab * f[nb.at(0, oa, ob)];
+    }
+  }
+  out[nb.c()] = dx4 * ihx4 + dy4 * ihy4 + dz4 * ihz4 +
+                T(2) * (dxy * ihx2 * ihy2 + dxz * ihx2 * ihz2 + dyz * ihy2 * ihz2);
+}
+
+// fvm/CellLaplacian.hpp:48-50
+template <class T>
+__device__ __forceinline__ T hmean(T a, T b) {
+  return T(2) * a * b / (a + b);
+}
+
+// fvm/CellLaplacian.hpp:115-141
+template <class T>
+__global__ __launch_bounds__(256) void k_cell_laplacian(const T* __restrict__ psi,
+                                                        const T* __restrict__ D,
+                                                        T* __restrict__ out, Dims d, int nbx,
+                                                        int nby, T ihx2, T ihy2, T ihz2) {
+  const Tile t = tile_site(d, nbx, nby);
+  if (!t.valid) return;
+  const Nb nb{d, t.i, t.j, t.k};
+  const size_t c = nb.c();
+  const T pc = psi[c];
+  const T Dc = D[c];
+  const size_t xp = nb.at(1, 0, 0), xm = nb.at(-1, 0, 0);
+  const size_t yp = nb.at(0, 1, 0), ym = nb.at(0, -1, 0);
+  const size_t zp = nb.at(0, 0, 1), zm = nb.at(0, 0, -1);
+  const T dxp = hmean(Dc, D[xp]);
+  const T dxm = hmean(D[xm], Dc);
+  const T dyp = hmean(Dc, D[yp]);
+  const T dym = hmean(D[ym], Dc);
+  const T dzp = hmean(Dc, D[zp]);
+  const T dzm = hmean(D[zm], Dc);
+  const T rx = (dxp * (psi[xp] - pc) - dxm * (pc - psi[xm])) * ihx2;
+  const T ry = (dyp * (psi[yp] - pc) - dym * (pc - psi[ym])) * ihy2;
+  const T rz = (dzp * (psi[zp] - pc) - dzm * (pc - psi[zm])) * ihz2;
+  out[c] = rx + ry + rz;
+}
+
+template <class T>
+struct Inv {
+  T x, y, z;
+};
+inline Inv<double> inv_h(const gc_grid* g) { return {1.0 / g->hx, 1.0 / g->hy, 1.0 / g->hz}; }
+inline Inv<double> inv_h2(const gc_grid* g) {
+  return {1.0 / (g->hx * g->hx), 1.0 / (g->hy * g->hy), 1.0 / (g->hz * g->hz)};
+}
+
+}  // namespace
+
+// ---- typed launchers (shared with the time-stepping kernels) -----------------------
+
+template <class T>
+static gc_status run_laplacian(gc_context* ctx, const gc_grid* g, int order, const T* in, T* out) {
+  const Dims d = dims_of(g);
+  const Launch L = launch_for(d);
+  const Inv<double> ih = inv_h2(g);
+  if (order == 2)
+    hipLaunchKernelGGL((k_laplacian<T, 2>), L.grid, L.block, 0, ctx->stream, in, out, d, L.nbx,
+                       L.nby, T(ih.x), T(ih.y), T(ih.z));
+  else
+    hipLaunchKernelGGL((k_laplacian<T, 4>), L.grid, L.block, 0, ctx->stream, in, out, d, L.nbx,
+                       L.nby, T(ih.x), T(ih.y), T(ih.z));
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status launch_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                           const void* in, void* out) {
+  return dt == GC_F64 ? run_laplacian<double>(ctx, g, order, (const double*)in, (double*)out)
+                      : run_laplacian<float>(ctx, g, order, (const float*)in, (float*)out);
+}
+
+}  // namespace gc
+
+using namespace gc;
+
+namespace {
+
+gc_status common_checks(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                        const void* in, const void* out) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_grid(g));
+  GC_TRY(check_dtype(dt));
+  if (order != 2 && order != 4)
+    return set_error(GC_ERR_INVALID, "gridcalc: Order must be 2 or 4; got %d", order);
+  if (!in || !out) return set_error(GC_ERR_INVALID, "gridcalc: null field pointer");
+  if (in == out) return set_error(GC_ERR_INVALID, "gridcalc: output must not alias the input");
+  return GC_OK;
+}
+
+template <class T, int O>
+void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void* in, void* out) {
+  const Dims d = dims_of(g);
+  const Launch L = launch_for(d);
+  const Inv<double> ih = inv_h(g);
+  const T x = T(ih.x), y = T(ih.y), z = T(ih.z);
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL((k_gradient<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
+                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_gradient_vec<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
+                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_divergence<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
+                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      break;
+    default:
+      hipLaunchKernelGGL((k_divergence_mat<T, O>), L.grid, L.block, 0, ctx->stream,
+                         (const T*)in, (T*)out, d, L.nbx, L.nby, x, y, z);
+      break;
+  }
+}
+
+gc_status first_deriv(int which, gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                      const void* in, void* out) {
+  GC_TRY(common_checks(ctx, g, dt, order, in, out));
+  GC_TRY(check_extent(g, order == 2 ? D1<2>::R : D1<4>::R));
+  if (dt == GC_F64) {
+    if (order == 2) first_deriv_launch<double, 2>(which, ctx, g, in, out);
+    else first_deriv_launch<double, 4>(which, ctx, g, in, out);
+  } else {
+    if (order == 2) first_deriv_launch<float, 2>(which, ctx, g, in, out);
+    else first_deriv_launch<float, 4>(which, ctx, g, in, out);
+  }
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+template <class T, int O>
+void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
+  const Dims d = dims_of(g);
+  const Launch L = launch_for(d);
+  const Inv<double> i2 = inv_h2(g);
+  // inv_h4 = inv_h2 * inv_h2 in double (Biharmonic.hpp:62-64), then cast
+  hipLaunchKernelGGL((k_biharmonic<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
+                     (T*)out, d, L.nbx, L.nby, T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x),
+                     T(i2.y * i2.y), T(i2.z * i2.z));
+}
+
+}  // namespace
+
+namespace gc {
+gc_status launch_cell_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* psi,
+                                const void* D, void* out) {
+  const Dims d = dims_of(g);
+  const Launch L = launch_for(d);
+  const Inv<double> ih = inv_h2(g);
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_cell_laplacian<double>), L.grid, L.block, 0, ctx->stream,
+                       (const double*)psi, (const double*)D, (double*)out, d, L.nbx, L.nby, ih.x,
+                       ih.y, ih.z);
+  else
+    hipLaunchKernelGGL((k_cell_laplacian<float>), L.grid, L.block, 0, ctx->stream,
+                       (const float*)psi, (const float*)D, (float*)out, d, L.nbx, L.nby,
+                       float(ih.x), float(ih.y), float(ih.z));
+  GC_LAUNCHED();
+  return GC_OK;
+}
+// defined in gc_reduce.hip
+gc_status first_nonpositive(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* D,
+                            int64_t* first_bad);
+}  // namespace gc
+
+extern "C" {
+
+gc_status gc_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order, const void* in,
+                       void* out) {
+  GC_TRY(common_checks(ctx, g, dt, order, in, out));
+  GC_TRY(check_extent(g, order == 2 ? D2<2>::R : D2<4>::R));
+  return launch_laplacian(ctx, g, dt, order, in, out);
+}
+
+gc_status gc_gradient(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order, const void* in,
+                      void* out_vec3) {
+  return first_deriv(0, ctx, g, dt, order, in, out_vec3);
+}
+
+gc_status gc_gradient_vec(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                          const void* in_vec3, void* out_mat3) {
+  return first_deriv(1, ctx, g, dt, order, in_vec3, out_mat3);
+}
+
+gc_status gc_divergence(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                        const void* in_vec3, void* out) {
+  return first_deriv(2, ctx, g, dt, order, in_vec3, out);
+}
+
+gc_status gc_divergence_mat(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
+                            const void* in_mat3, void* out_vec3) {
+  return first_deriv(3, ctx, g, dt, order, in_mat3, out_vec3);
+}
+
+gc_status gc_biharmonic(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order, const void* in,
+                        void* out) {
+  GC_TRY(common_checks(ctx, g, dt, order, in, out));
+  // max(S4 radius, S2 radius) (Biharmonic.hpp:52-57)
+  GC_TRY(check_extent(g, order == 2 ? D4<2>::R : D4<4>::R));
+  if (dt == GC_F64) {
+    if (order == 2) bih_launch<double, 2>(ctx, g, in, out);
+    else bih_launch<double, 4>(ctx, g, in, out);
+  } else {
+    if (order == 2) bih_launch<float, 2>(ctx, g, in, out);
+    else bih_launch<float, 4>(ctx, g, in, out);
+  }
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status gc_cell_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* psi,
+                            const void* D, void* out, int64_t* first_bad) {
+  GC_TRY(common_checks(ctx, g, dt, 2, psi, out));
+  if (!D) return set_error(GC_ERR_INVALID, "gridcalc: null D pointer");
+  if (D == out) return set_error(GC_ERR_INVALID, "gridcalc: output must not alias D");
+  GC_TRY(check_extent(g, 1));
+  int64_t bad = -1;
+  GC_TRY(first_nonpositive(ctx, g, dt, D, &bad));
+  if (first_bad) *first_bad = bad;
+  if (bad >= 0)
+    return set_error(GC_ERR_NONPOS_D,
+                     "fvm::cellLaplacian: D must be strictly positive at every cell (the "
+                     "harmonic-mean face-D averaging is undefined otherwise); first violation "
+                     "at flat index %lld",
+                     (long long)bad);
+  return launch_cell_laplacian(ctx, g, dt, psi, D, out);
+}
+
+}  // extern "C"

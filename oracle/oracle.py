@@ -1,0 +1,245 @@
+"""ctypes wrapper around oracle/liboracle.so, the CPU restatement of gridcalc's
+operator and time-stepping path.
+
+Test infrastructure only: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker and by ``bench.py``'s ``cpu_baseline`` leg.  The shipped
+library (grid-calculus_amd/) never imports it.
+
+Arrays are numpy float64, i-fastest (index i + Nx*(j + Ny*k)), i.e. a C-ordered
+array of shape (Nz, Ny, Nx); vector fields are (Nz, Ny, Nx, 3) and tensor
+fields (Nz, Ny, Nx, 9) with the 3x3 matrix stored column-major (Eigen's
+default), M(r, c) at r + 3*c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+OK, EXTENT, ARG, CFL, NONPOS_D, NOMEM = range(6)
+PAIRWISE, KAHAN = 0, 1
+EULER, RK4 = 0, 1
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"oracle status {code} {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double)]
+
+    @staticmethod
+    def make(nx, ny, nz, h=(1.0, 1.0, 1.0)):
+        if not isinstance(h, (tuple, list)):
+            h = (h, h, h)
+        return Grid(nx, ny, nz, float(h[0]), float(h[1]), float(h[2]))
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def size(self):
+        return self.nx * self.ny * self.nz
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER
+        dp = P(C.c_double)
+        gp = P(Grid)
+        for name in ("or_laplacian", "or_gradient", "or_gradient_vec", "or_divergence",
+                     "or_divergence_mat", "or_biharmonic"):
+            f = getattr(L, name)
+            f.argtypes = [gp, C.c_int, dp, dp]
+            f.restype = C.c_int
+        L.or_cell_laplacian.argtypes = [gp, dp, dp, dp, P(C.c_longlong)]
+        L.or_cell_laplacian.restype = C.c_int
+        L.or_max_face_d.argtypes = [gp, dp]
+        L.or_max_face_d.restype = C.c_double
+        L.or_pairwise_sum.argtypes = [dp, C.c_size_t]
+        L.or_pairwise_sum.restype = C.c_double
+        L.or_neumaier_sum.argtypes = [dp, C.c_size_t]
+        L.or_neumaier_sum.restype = C.c_double
+        L.or_integrate.argtypes = [gp, C.c_int, dp]
+        L.or_integrate.restype = C.c_double
+        L.or_free_energy.argtypes = [gp, C.c_double, C.c_int, dp, dp]
+        L.or_free_energy.restype = C.c_int
+        L.or_gradient_energy.argtypes = [gp, C.c_int, dp, dp]
+        L.or_gradient_energy.restype = C.c_int
+        L.or_ch_rhs.argtypes = [C.c_void_p, gp, dp, dp]
+        L.or_ch_rhs.restype = C.c_int
+        L.or_ch_integrate.argtypes = [gp, C.c_double, C.c_double, dp, C.c_double, C.c_int, C.c_int]
+        L.or_ch_integrate.restype = C.c_int
+        L.or_cfl_violated.argtypes = [gp, C.c_double, C.c_double, C.c_double]
+        L.or_cfl_violated.restype = C.c_int
+        L.or_diffuse_const.argtypes = [gp, dp, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.or_diffuse_const.restype = C.c_int
+        L.or_diffuse_field.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, P(C.c_longlong)]
+        L.or_diffuse_field.restype = C.c_int
+        L.or_stencil_table.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), dp]
+        L.or_stencil_table.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.size != int(np.prod(shape)):
+        raise ValueError(f"array of size {a.size} does not match {shape}")
+    return a
+
+
+def _check(rc):
+    if rc != OK:
+        raise OracleError(rc)
+
+
+def stencil_table(kind, order):
+    n, r = C.c_int(), C.c_int()
+    offs = (C.c_int * 8)()
+    w = (C.c_double * 8)()
+    _check(lib().or_stencil_table(kind, order, C.byref(n), C.byref(r), offs, w))
+    return r.value, [offs[m] for m in range(n.value)], [w[m] for m in range(n.value)]
+
+
+def _unary(name, g, order, f, in_comp, out_comp):
+    f = _f64(f)
+    if f.size != g.size * in_comp:
+        raise ValueError("input size does not match grid")
+    shape = g.shape + ((out_comp,) if out_comp > 1 else ())
+    out = np.empty(shape, dtype=np.float64)
+    _check(getattr(lib(), name)(C.byref(g), order, _dp(f), _dp(out)))
+    return out
+
+
+def laplacian(g, f, order=2):
+    return _unary("or_laplacian", g, order, f, 1, 1)
+
+
+def gradient(g, f, order=2):
+    return _unary("or_gradient", g, order, f, 1, 3)
+
+
+def gradient_vec(g, v, order=2):
+    return _unary("or_gradient_vec", g, order, v, 3, 9)
+
+
+def divergence(g, v, order=2):
+    return _unary("or_divergence", g, order, v, 3, 1)
+
+
+def divergence_mat(g, m, order=2):
+    return _unary("or_divergence_mat", g, order, m, 9, 3)
+
+
+def biharmonic(g, f, order=2):
+    return _unary("or_biharmonic", g, order, f, 1, 1)
+
+
+def cell_laplacian(g, psi, D):
+    psi, D = _f64(psi), _f64(D)
+    out = np.empty(g.shape, dtype=np.float64)
+    bad = C.c_longlong(-1)
+    rc = lib().or_cell_laplacian(C.byref(g), _dp(psi), _dp(D), _dp(out), C.byref(bad))
+    if rc == NONPOS_D:
+        raise OracleError(rc, f"first violation at flat index {bad.value}")
+    _check(rc)
+    return out
+
+
+def max_face_d(g, D):
+    D = _f64(D)
+    return lib().or_max_face_d(C.byref(g), _dp(D))
+
+
+def pairwise_sum(x):
+    x = _f64(x).ravel()
+    return lib().or_pairwise_sum(_dp(x), x.size)
+
+
+def neumaier_sum(x):
+    x = _f64(x).ravel()
+    return lib().or_neumaier_sum(_dp(x), x.size)
+
+
+def integrate(g, x, method=PAIRWISE):
+    x = _f64(x)
+    return lib().or_integrate(C.byref(g), method, _dp(x))
+
+
+def free_energy(g, psi, kappa, method=PAIRWISE):
+    psi = _f64(psi)
+    F = C.c_double()
+    _check(lib().or_free_energy(C.byref(g), kappa, method, _dp(psi), C.byref(F)))
+    return F.value
+
+
+def gradient_energy(g, psi, method=PAIRWISE):
+    psi = _f64(psi)
+    E = C.c_double()
+    _check(lib().or_gradient_energy(C.byref(g), method, _dp(psi), C.byref(E)))
+    return E.value
+
+
+class _CHParams(C.Structure):
+    _fields_ = [("M", C.c_double), ("kappa", C.c_double)]
+
+
+def ch_rhs(g, psi, M=1.0, kappa=1.0):
+    psi = _f64(psi)
+    out = np.empty(g.shape, dtype=np.float64)
+    P = _CHParams(M, kappa)
+    _check(lib().or_ch_rhs(C.cast(C.pointer(P), C.c_void_p), C.byref(g), _dp(psi), _dp(out)))
+    return out
+
+
+def ch_integrate(g, psi, M, kappa, dt, n_steps, method=RK4):
+    """Returns a new array: psi after n_steps (psi itself is not modified)."""
+    out = _f64(psi).copy()
+    _check(lib().or_ch_integrate(C.byref(g), M, kappa, _dp(out), dt, n_steps, method))
+    return out
+
+
+def cfl_violated(g, D, dt, limit):
+    return bool(lib().or_cfl_violated(C.byref(g), D, dt, limit))
+
+
+def diffuse_const(g, psi, D, dt, n_steps, method=EULER):
+    out = _f64(psi).copy()
+    _check(lib().or_diffuse_const(C.byref(g), _dp(out), D, dt, n_steps, method))
+    return out
+
+
+def diffuse_field(g, psi, D, dt, n_steps, method=EULER):
+    out = _f64(psi).copy()
+    D = _f64(D)
+    bad = C.c_longlong(-1)
+    rc = lib().or_diffuse_field(C.byref(g), _dp(out), _dp(D), dt, n_steps, method, C.byref(bad))
+    if rc == NONPOS_D:
+        raise OracleError(rc, f"first violation at flat index {bad.value}")
+    _check(rc)
+    return out
