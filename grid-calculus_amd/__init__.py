@@ -93,6 +93,7 @@ def lib():
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
     L = C.CDLL(LIB_PATH)
     vp, gp, sz, i64p, dp = C.c_void_p, C.POINTER(Grid), C.c_size_t, C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
     st = C.c_int
     sig = {
         "gc_last_error": ([], C.c_char_p),
@@ -129,29 +130,28 @@ def lib():
         "gc_ch_integrate": ([vp, gp, C.c_int, C.POINTER(CHParams), vp, C.c_double, C.c_int, C.c_int], st),
         "gc_diffuse_const": ([vp, gp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_int], st),
         "gc_diffuse_field": ([vp, gp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int, i64p], st),
+        "gc_ch_config_count": ([gp], C.c_int),
+        "gc_ch_config_shape": ([gp, C.c_int, ip, ip, ip], st),
+        "gc_ch_set_config": ([gp, C.c_int, C.c_int], st),
+        "gc_ch_get_config": ([gp, C.c_int, ip], st),
+        "gc_ch_tune": ([vp, gp, C.c_int, ip, C.POINTER(C.c_float)], st),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res
+    _BOUND[:] = list(sig)
     _lib = L
     return L
 
 
-#: every entry point include/gridcalc_hip.h declares (checked by the CPU test-suite)
-def exported_symbols():
+_BOUND = []
+
+
+def bound_symbols():
+    """Names of every C entry point this binding declares a signature for."""
     lib()
-    return sorted(n for n in _SIGNATURES_DOC)
-
-
-_SIGNATURES_DOC = (
-    "gc_last_error gc_version gc_device_count gc_context_create gc_context_destroy "
-    "gc_context_sync gc_context_stream gc_context_scratch_bytes gc_malloc gc_free "
-    "gc_memcpy_htod gc_memcpy_dtoh gc_memcpy_dtod gc_timer_start gc_timer_stop gc_laplacian "
-    "gc_gradient gc_gradient_vec gc_divergence gc_divergence_mat gc_biharmonic "
-    "gc_cell_laplacian gc_first_nonpositive gc_max_face_d gc_integrate gc_ch_free_energy "
-    "gc_gradient_energy gc_axpy gc_euler_update gc_rk4_combine gc_ch_rhs gc_ch_integrate "
-    "gc_diffuse_const gc_diffuse_field").split()
+    return list(_BOUND)
 
 
 def _check(rc):
@@ -382,3 +382,34 @@ def diffuse_field(ctx, g, psi, D, dt, n_steps, integrator=EULER):
     bad = C.c_int64(-1)
     _check(lib().gc_diffuse_field(ctx.handle, C.byref(g), _dt(psi), psi.ptr, D.ptr, dt, n_steps,
                                   integrator, C.byref(bad)))
+
+
+# ---- Cahn-Hilliard stage autotuner --------------------------------------------------------
+
+def ch_config_count(g):
+    return lib().gc_ch_config_count(C.byref(g))
+
+
+def ch_config_shape(g, i):
+    ox, oy, zc = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().gc_ch_config_shape(C.byref(g), i, C.byref(ox), C.byref(oy), C.byref(zc)))
+    return ox.value, oy.value, zc.value
+
+
+def ch_set_config(g, dtype, i):
+    _check(lib().gc_ch_set_config(C.byref(g), dtype, i))
+
+
+def ch_get_config(g, dtype):
+    i = C.c_int()
+    _check(lib().gc_ch_get_config(C.byref(g), dtype, C.byref(i)))
+    return i.value
+
+
+def ch_tune(ctx, g, dtype):
+    """Times every candidate tile shape; returns (best id, [ms per RK4 step])."""
+    n = ch_config_count(g)
+    ms = (C.c_float * max(n, 1))()
+    best = C.c_int()
+    _check(lib().gc_ch_tune(ctx.handle, C.byref(g), dtype, C.byref(best), ms))
+    return best.value, [ms[i] for i in range(n)]

@@ -15,6 +15,10 @@
 // reference forms p2 + p3).  HBM traffic per step: 3 + 4 + 5 + 5 = 17 words per site.
 #include "gc_common.hpp"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace gc {
 
 gc_status launch_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
@@ -166,6 +170,45 @@ struct StageCoef {
   T half_dt, dt, sixth_dt, third_dt;
 };
 
+// The per-site stage update (see the file comment); kv is the right-hand side at site c.
+template <class T>
+__device__ __forceinline__ void stage_update_regs(int stage, size_t c, T kv, T pv, T av, T bv,
+                                                  T* psi, T* ynext, T* a, T* b,
+                                                  const StageCoef<T>& co) {
+  switch (stage) {
+    case kRhsOnly:
+      ynext[c] = kv;
+      break;
+    case kS1:
+      a[c] = kv;
+      ynext[c] = pv + co.half_dt * kv;
+      break;
+    case kS2:
+      b[c] = kv;
+      ynext[c] = pv + co.half_dt * kv;
+      break;
+    case kS3:
+      b[c] = bv + kv;
+      ynext[c] = pv + co.dt * kv;
+      break;
+    case kS4:
+      psi[c] = pv + (co.sixth_dt * (av + kv) + co.third_dt * bv);
+      break;
+    default:  // explicit Euler: ynext = y + dt*f(y), pv = y[c]
+      ynext[c] = pv + co.dt * kv;
+      break;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void stage_update(int stage, size_t c, T kv, const T* y, T* psi,
+                                             T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  const T pv = (stage >= kS1 && stage <= kS4) ? psi[c] : (stage == kEuler ? y[c] : T(0));
+  const T av = stage == kS4 ? a[c] : T(0);
+  const T bv = (stage == kS3 || stage == kS4) ? b[c] : T(0);
+  stage_update_regs(stage, c, kv, pv, av, bv, psi, ynext, a, b, co);
+}
+
 template <class T, class Rhs>
 __global__ __launch_bounds__(256) void k_stage(Rhs rhs, int stage, const T* __restrict__ y,
                                                T* psi, T* __restrict__ ynext, T* __restrict__ a,
@@ -181,28 +224,232 @@ __global__ __launch_bounds__(256) void k_stage(Rhs rhs, int stage, const T* __re
   if (i >= d.nx || j >= d.ny) return;
   const size_t c = lin(d, i, j, k);
   const T kv = rhs(y, d, i, j, k);
-  switch (stage) {
-    case kRhsOnly:
-      ynext[c] = kv;
-      break;
-    case kS1:
-      a[c] = kv;
-      ynext[c] = psi[c] + co.half_dt * kv;
-      break;
-    case kS2:
-      b[c] = kv;
-      ynext[c] = psi[c] + co.half_dt * kv;
-      break;
-    case kS3:
-      b[c] = b[c] + kv;
-      ynext[c] = psi[c] + co.dt * kv;
-      break;
-    case kS4:
-      psi[c] = psi[c] + (co.sixth_dt * (a[c] + kv) + co.third_dt * b[c]);
-      break;
-    default:  // explicit Euler: ynext = y + dt*f(y)
-      ynext[c] = y[c] + co.dt * kv;
-      break;
+  stage_update(stage, c, kv, y, psi, ynext, a, b, co);
+}
+
+// ---- LDS-tiled Cahn-Hilliard stage kernels ----------------------------------------------
+//
+// 3-D (z-march).  A block owns an OX x OY tile of xy-columns and a chunk of ZC planes.
+// It streams y planes (tile + 2-site halo, periodic wrap applied while filling) into a
+// 3-plane LDS ring, computes mu once per site on the tile + 1-site halo into a second
+// 3-plane ring, and evaluates M*lap(mu) plus the stage update on the tile of the plane
+// two steps behind.  Per output plane: one y plane read (+halo, mostly L2 hits) and the
+// stage's point reads/writes -- the HBM minimum for the stage.  The next y plane and the
+// point operands are prefetched into registers one iteration ahead.
+//
+// 2-D (Nz = 1): the same with a single plane; the degenerate z terms of both Laplacians
+// are formed from the centre value, exactly as the reference's wrapped access does.
+
+template <class T>
+struct CHCoef {
+  T M, kappa, ihx2, ihy2, ihz2;
+};
+
+// Order-2 Laplacian in reference order from LDS: x/y neighbours in plane `c`, z
+// neighbours from planes `lo` (k-1) and `hi` (k+1) at the same offset.
+template <class T>
+__device__ __forceinline__ T lap_lds(const T* __restrict__ pc, const T* __restrict__ plo,
+                                     const T* __restrict__ phi, int o, int W, T ihx2, T ihy2,
+                                     T ihz2) {
+  const T cc = pc[o];
+  T dxx = 0, dyy = 0, dzz = 0;
+  dxx += T(1) * pc[o - 1];
+  dyy += T(1) * pc[o - W];
+  dzz += T(1) * plo[o];
+  dxx += T(-2) * cc;
+  dyy += T(-2) * cc;
+  dzz += T(-2) * cc;
+  dxx += T(1) * pc[o + 1];
+  dyy += T(1) * pc[o + W];
+  dzz += T(1) * phi[o];
+  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+}
+
+__device__ __forceinline__ int wrap_any(int x, int n) {
+  x %= n;
+  return x < 0 ? x + n : x;
+}
+
+template <class T, int OX, int OY, int ZC>
+struct Ch3Cfg {
+  static constexpr int NT = 256;
+  static constexpr int YW = OX + 4, YH = OY + 4, YN = YW * YH;  // y plane, halo 2
+  static constexpr int MW = OX + 2, MH = OY + 2, MN = MW * MH;  // mu plane, halo 1
+  static constexpr int YL = (YN + NT - 1) / NT;                 // y loads per thread
+  static constexpr int ML = (MN + NT - 1) / NT;                 // mu sites per thread
+  static constexpr int OR = OX * OY / NT;                       // output sites per thread
+  static_assert(OX % 64 == 0 || NT % OX == 0, "tile width must divide the block");
+  static_assert((OX * OY) % NT == 0, "tile must be a multiple of the block");
+};
+
+template <class T, int OX, int OY, int ZC>
+__global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
+                                                    const T* __restrict__ y, T* psi,
+                                                    T* __restrict__ ynext, T* __restrict__ a,
+                                                    T* __restrict__ b, Dims d, int ntx, int nty,
+                                                    StageCoef<T> co) {
+  using C = Ch3Cfg<T, OX, OY, ZC>;
+  __shared__ T ys[3][C::YN];
+  __shared__ T ms[3][C::MN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned r = bid / (unsigned)ntx;
+  const int ty = (int)(r % (unsigned)nty);
+  const int zc = (int)(r / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
+  const int nout = min(ZC, d.nz - z0);
+
+  // xy part of the global offset of every y-plane slot this thread fills
+  int yoff[C::YL];
+#pragma unroll
+  for (int s = 0; s < C::YL; ++s) {
+    const int idx = t + s * C::NT;
+    if (idx < C::YN) {
+      const int px = idx % C::YW, py = idx / C::YW;
+      yoff[s] = wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny);
+    } else {
+      yoff[s] = -1;
+    }
+  }
+  // output sites of this thread: column ox, rows oy0 + q*(NT/OX)
+  constexpr int ROWS = C::NT / OX;
+  const int ox = t % OX, oy0 = t / OX;
+  const bool xin = x0 + ox < d.nx;
+
+  T yreg[C::YL];
+  auto load_y = [&](int plane) {
+    const size_t base = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < C::YL; ++s)
+      if (yoff[s] >= 0) yreg[s] = y[base + (size_t)yoff[s]];
+  };
+  T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+  auto load_pt = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < C::OR; ++q) {
+      const int gy = y0 + oy0 + q * ROWS;
+      if (xin && gy < d.ny) {
+        const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy + (size_t)d.plane * k;
+        if (stage >= kS1 && stage <= kS4) pv[q] = psi[c];
+        if (stage == kS4) av[q] = a[c];
+        if (stage >= kS3 && stage <= kS4) bv[q] = b[c];
+        if (stage == kEuler) pv[q] = y[c];
+      }
+    }
+  };
+
+  load_y(z0 - 2);
+  const int last = nout + 3;
+  for (int p = 0; p <= last; ++p) {
+    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // slots of p, p-1, p-2
+    T* yw = ys[s0];
+#pragma unroll
+    for (int s = 0; s < C::YL; ++s)
+      if (yoff[s] >= 0) yw[t + s * C::NT] = yreg[s];
+    if (p < last) load_y(z0 - 1 + p);
+    if (p >= 4) load_pt(z0 + p - 4);
+    __syncthreads();
+    if (p >= 2) {  // mu plane z0+p-3 from y planes p-2, p-1, p
+      const T* yc = ys[s1];
+      const T* yb = ys[s2];
+      const T* ya = ys[s0];
+      T* mw = ms[s0];
+#pragma unroll
+      for (int s = 0; s < C::ML; ++s) {
+        const int idx = t + s * C::NT;
+        if (idx < C::MN) {
+          const int mx = idx % C::MW, my = idx / C::MW;
+          const int o = (my + 1) * C::YW + (mx + 1);
+          const T l = lap_lds(yc, yb, ya, o, C::YW, cf.ihx2, cf.ihy2, cf.ihz2);
+          const T pp = yc[o];
+          mw[idx] = -pp + pp * pp * pp - cf.kappa * l;
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 4) {  // output plane k = z0+p-4 from mu planes p-2, p-1, p
+      const int k = z0 + p - 4;
+      const T* mc = ms[s1];
+      const T* mb = ms[s2];
+      const T* ma = ms[s0];
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q) {
+        const int oy = oy0 + q * ROWS;
+        const int gy = y0 + oy;
+        if (xin && gy < d.ny) {
+          const int o = (oy + 1) * C::MW + (ox + 1);
+          const T kv = lap_lds(mc, mb, ma, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy + (size_t)d.plane * k;
+          stage_update_regs(stage, c, kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
+        }
+      }
+    }
+  }
+}
+
+template <class T, int OX, int OY>
+__global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, int stage,
+                                                    const T* __restrict__ y, T* psi,
+                                                    T* __restrict__ ynext, T* __restrict__ a,
+                                                    T* __restrict__ b, Dims d, int ntx,
+                                                    StageCoef<T> co) {
+  using C = Ch3Cfg<T, OX, OY, 1>;
+  __shared__ T ys[C::YN];
+  __shared__ T ms[C::MN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const int ty = (int)(bid / (unsigned)ntx);
+  const int x0 = tx * OX, y0 = ty * OY;
+  constexpr int ROWS = C::NT / OX;
+  const int ox = t % OX, oy0 = t / OX;
+  const bool xin = x0 + ox < d.nx;
+
+#pragma unroll
+  for (int s = 0; s < C::YL; ++s) {
+    const int idx = t + s * C::NT;
+    if (idx < C::YN) {
+      const int px = idx % C::YW, py = idx / C::YW;
+      ys[idx] = y[wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny)];
+    }
+  }
+  T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+#pragma unroll
+  for (int q = 0; q < C::OR; ++q) {
+    const int gy = y0 + oy0 + q * ROWS;
+    if (xin && gy < d.ny) {
+      const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy;
+      if (stage >= kS1 && stage <= kS4) pv[q] = psi[c];
+      if (stage == kS4) av[q] = a[c];
+      if (stage >= kS3 && stage <= kS4) bv[q] = b[c];
+      if (stage == kEuler) pv[q] = y[c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < C::ML; ++s) {
+    const int idx = t + s * C::NT;
+    if (idx < C::MN) {
+      const int mx = idx % C::MW, my = idx / C::MW;
+      const int o = (my + 1) * C::YW + (mx + 1);
+      // Nz = 1: the z neighbours are the centre itself
+      const T l = lap_lds(ys, ys, ys, o, C::YW, cf.ihx2, cf.ihy2, cf.ihz2);
+      const T pp = ys[o];
+      ms[idx] = -pp + pp * pp * pp - cf.kappa * l;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < C::OR; ++q) {
+    const int oy = oy0 + q * ROWS;
+    const int gy = y0 + oy;
+    if (xin && gy < d.ny) {
+      const int o = (oy + 1) * C::MW + (ox + 1);
+      const T kv = lap_lds(ms, ms, ms, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+      const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy;
+      stage_update_regs(stage, c, kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
+    }
   }
 }
 
@@ -214,6 +461,111 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int st
   const dim3 grid((unsigned)(nbx * nby * d.nz));
   hipLaunchKernelGGL((k_stage<T, Rhs>), grid, dim3(TX * TY), 0, ctx->stream, rhs, stage, y, psi,
                      ynext, a, b, d, nbx, nby, co);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// ---- tile shapes of the Cahn-Hilliard stage kernels and their selection ------------------
+//
+// Candidate shapes (tile width OX, tile height OY, z-chunk ZC).  The default is used until
+// gc_ch_tune has measured the candidates for a grid's extents and dtype; the winner is
+// then remembered for that (extents, dtype) for the life of the process.
+
+struct TileCfg {
+  int ox, oy, zc;
+};
+constexpr TileCfg kCh3Cfgs[] = {{64, 4, 16}, {64, 4, 32},  {64, 8, 8},   {64, 8, 16}, {64, 8, 32},
+                                {64, 16, 8}, {64, 16, 16}, {64, 16, 32}, {64, 16, 64}};
+constexpr TileCfg kCh2Cfgs[] = {{64, 4, 1},  {64, 8, 1},   {64, 16, 1},
+                                {64, 32, 1}, {128, 8, 1}, {128, 16, 1}};
+constexpr int kN3 = sizeof(kCh3Cfgs) / sizeof(kCh3Cfgs[0]);
+constexpr int kN2 = sizeof(kCh2Cfgs) / sizeof(kCh2Cfgs[0]);
+constexpr int kDefault3 = 7, kDefault2 = 2;
+
+// Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
+inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
+
+template <class T, int OX, int OY, int ZC>
+void launch_ch3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int stage, const T* y,
+                T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY, nzc = (d.nz + ZC - 1) / ZC;
+  hipLaunchKernelGGL((k_ch_stage3d<T, OX, OY, ZC>), dim3((unsigned)(ntx * nty * nzc)), dim3(256),
+                     0, ctx->stream, cf, stage, y, psi, ynext, a, b, d, ntx, nty, co);
+}
+
+template <class T, int OX, int OY>
+void launch_ch2(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int stage, const T* y,
+                T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY;
+  hipLaunchKernelGGL((k_ch_stage2d<T, OX, OY>), dim3((unsigned)(ntx * nty)), dim3(256), 0,
+                     ctx->stream, cf, stage, y, psi, ynext, a, b, d, ntx, co);
+}
+
+template <class T>
+void launch_ch_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf, int stage,
+                   const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+#define GC_CH3(i, OX, OY, ZC) \
+  case i: launch_ch3<T, OX, OY, ZC>(ctx, d, cf, stage, y, psi, ynext, a, b, co); break;
+#define GC_CH2(i, OX, OY) \
+  case i: launch_ch2<T, OX, OY>(ctx, d, cf, stage, y, psi, ynext, a, b, co); break;
+  if (d.nz == 1) {
+    switch (id) {
+      GC_CH2(0, 64, 4)
+      GC_CH2(1, 64, 8)
+      GC_CH2(2, 64, 16)
+      GC_CH2(3, 64, 32)
+      GC_CH2(4, 128, 8)
+      default: launch_ch2<T, 128, 16>(ctx, d, cf, stage, y, psi, ynext, a, b, co); break;
+    }
+  } else {
+    switch (id) {
+      GC_CH3(0, 64, 4, 16)
+      GC_CH3(1, 64, 4, 32)
+      GC_CH3(2, 64, 8, 8)
+      GC_CH3(3, 64, 8, 16)
+      GC_CH3(4, 64, 8, 32)
+      GC_CH3(5, 64, 16, 8)
+      GC_CH3(6, 64, 16, 16)
+      GC_CH3(7, 64, 16, 32)
+      default: launch_ch3<T, 64, 16, 64>(ctx, d, cf, stage, y, psi, ynext, a, b, co); break;
+    }
+  }
+#undef GC_CH3
+#undef GC_CH2
+}
+
+// Process-wide choice of tile shape per (extents, dtype).
+struct CfgKey {
+  int nx, ny, nz, dt;
+  bool operator<(const CfgKey& o) const {
+    return std::tie(nx, ny, nz, dt) < std::tie(o.nx, o.ny, o.nz, o.dt);
+  }
+};
+std::mutex g_cfg_mu;
+std::map<CfgKey, int> g_cfg;
+
+int ch_config(const Dims& d, int dt) {
+  std::lock_guard<std::mutex> lock(g_cfg_mu);
+  auto it = g_cfg.find(CfgKey{d.nx, d.ny, d.nz, dt});
+  if (it != g_cfg.end()) return it->second;
+  return d.nz == 1 ? kDefault2 : kDefault3;
+}
+
+template <class T>
+gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, int stage,
+                       const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co,
+                       int cfg = -1) {
+  const Dims d = dims_of(g);
+  const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
+  if (ch_tiled(d)) {
+    const int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
+    launch_ch_cfg<T>(ctx, id, d, cf, stage, y, psi, ynext, a, b, co);
+  } else {
+    const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;
+    hipLaunchKernelGGL((k_stage<T, CHRhs<T>>), dim3((unsigned)(nbx * nby * d.nz)),
+                       dim3(TX * TY), 0, ctx->stream, rhs, stage, y, psi, ynext, a, b, d, nbx,
+                       nby, co);
+  }
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -443,6 +795,120 @@ gc_status gc_diffuse_field(gc_context* ctx, const gc_grid* g, gc_dtype dt, void*
     return run_steps<double>(ctx, g, make_fvm<double>(g, D), (double*)psi, step, n_steps,
                              integrator);
   return run_steps<float>(ctx, g, make_fvm<float>(g, D), (float*)psi, step, n_steps, integrator);
+}
+
+}  // extern "C"
+
+// ---- Cahn-Hilliard tile autotuner -------------------------------------------------------
+
+namespace {
+
+template <class T>
+gc_status tune_ch(gc_context* ctx, const gc_grid* g, int* best, float* ms_out) {
+  const Dims d = dims_of(g);
+  const int ncfg = d.nz == 1 ? kN2 : kN3;
+  const size_t n = sites(g);
+  void* scr = nullptr;
+  GC_TRY(scratch(ctx, 5 * n * sizeof(T), &scr));
+  GC_HIP(hipMemsetAsync(scr, 0, 5 * n * sizeof(T), ctx->stream));
+  T* psi = (T*)scr;
+  T* yA = psi + n;
+  T* yB = yA + n;
+  T* a = yB + n;
+  T* b = a + n;
+  const gc_ch_params p{1.0, 1.0};
+  const CHRhs<T> rhs = make_ch<T>(g, &p);
+  StageCoef<T> co{T(0.0005), T(0.001), T(0.001 / 6.0), T(0.001 / 3.0)};
+  // steps per measurement: about 2^24 site-stages of work, at least 2
+  const int steps = (int)std::max<size_t>(2, std::min<size_t>(64, (size_t(1) << 24) / n));
+  float best_ms = 0.0f;
+  int best_id = -1;
+  for (int id = 0; id < ncfg; ++id) {
+    auto step = [&]() -> gc_status {
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS1, psi, psi, yA, a, b, co, id));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS2, yA, psi, yB, a, b, co, id));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS3, yB, psi, yA, a, b, co, id));
+      GC_TRY(launch_stage<T>(ctx, g, rhs, kS4, yA, psi, nullptr, a, b, co, id));
+      return GC_OK;
+    };
+    GC_TRY(step());  // warm-up
+    float ms = 0.0f;
+    GC_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int s = 0; s < steps; ++s) GC_TRY(step());
+    GC_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    GC_HIP(hipEventSynchronize(ctx->ev_stop));
+    GC_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ms /= (float)steps;
+    if (ms_out) ms_out[id] = ms;
+    if (best_id < 0 || ms < best_ms) {
+      best_ms = ms;
+      best_id = id;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    g_cfg[CfgKey{d.nx, d.ny, d.nz, sizeof(T) == 8 ? GC_F64 : GC_F32}] = best_id;
+  }
+  *best = best_id;
+  return GC_OK;
+}
+
+gc_status cfg_checks(const gc_grid* g) {
+  GC_TRY(check_grid(g));
+  if (!ch_tiled(dims_of(g)))
+    return set_error(GC_ERR_UNSUPPORTED, "gridcalc: grids with 1 < Nz < 8 use the untiled kernel");
+  return GC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gc_ch_config_count(const gc_grid* g) {
+  if (!g || check_grid(g) != GC_OK || !ch_tiled(dims_of(g))) return 0;
+  return g->nz == 1 ? kN2 : kN3;
+}
+
+gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc) {
+  GC_TRY(cfg_checks(g));
+  const int n = gc_ch_config_count(g);
+  if (id < 0 || id >= n || !ox || !oy || !zc)
+    return set_error(GC_ERR_INVALID, "gridcalc: config id %d out of range [0, %d)", id, n);
+  const TileCfg c = g->nz == 1 ? kCh2Cfgs[id] : kCh3Cfgs[id];
+  *ox = c.ox;
+  *oy = c.oy;
+  *zc = c.zc;
+  return GC_OK;
+}
+
+gc_status gc_ch_set_config(const gc_grid* g, gc_dtype dt, int id) {
+  GC_TRY(cfg_checks(g));
+  GC_TRY(check_dtype(dt));
+  const int n = gc_ch_config_count(g);
+  if (id < -1 || id >= n)
+    return set_error(GC_ERR_INVALID, "gridcalc: config id %d out of range [-1, %d)", id, n);
+  std::lock_guard<std::mutex> lock(g_cfg_mu);
+  const CfgKey k{g->nx, g->ny, g->nz, (int)dt};
+  if (id < 0) g_cfg.erase(k);
+  else g_cfg[k] = id;
+  return GC_OK;
+}
+
+gc_status gc_ch_get_config(const gc_grid* g, gc_dtype dt, int* id) {
+  GC_TRY(cfg_checks(g));
+  GC_TRY(check_dtype(dt));
+  if (!id) return set_error(GC_ERR_INVALID, "gridcalc: null id");
+  *id = ch_config(dims_of(g), (int)dt);
+  return GC_OK;
+}
+
+gc_status gc_ch_tune(gc_context* ctx, const gc_grid* g, gc_dtype dt, int* best, float* ms) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(cfg_checks(g));
+  GC_TRY(check_dtype(dt));
+  GC_TRY(check_extent(g, 1));
+  if (!best) return set_error(GC_ERR_INVALID, "gridcalc: null best");
+  return dt == GC_F64 ? tune_ch<double>(ctx, g, best, ms) : tune_ch<float>(ctx, g, best, ms);
 }
 
 }  // extern "C"

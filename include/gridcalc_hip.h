@@ -177,6 +177,26 @@ gc_status gc_diffuse_field(gc_context* ctx, const gc_grid* g, gc_dtype dt, void*
                            const void* D, double step, int n_steps, int integrator,
                            int64_t* first_bad);
 
+/* ---- tile-shape autotuner of the fused Cahn-Hilliard stage kernel ------------------
+ * No counterpart in the reference (which has no GPU kernels).  The stage kernel tiles
+ * the grid in (OX x OY) columns marched through chunks of ZC planes (2-D: OX x OY
+ * tiles).  Candidate shapes are numbered 0..gc_ch_config_count()-1 per kernel family
+ * (2-D when Nz == 1, 3-D when Nz >= 8; grids with 1 < Nz < 8 are untiled and have no
+ * candidates).  The choice is remembered per (extents, dtype) for the process. */
+
+/* Number of candidate shapes for this grid's kernel family (0 if untiled). */
+int gc_ch_config_count(const gc_grid* g);
+/* Tile shape of candidate `id`. */
+gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc);
+/* Forces candidate `id` for (extents, dtype); -1 restores the built-in default. */
+gc_status gc_ch_set_config(const gc_grid* g, gc_dtype dt, int id);
+/* Candidate currently used for (extents, dtype). */
+gc_status gc_ch_get_config(const gc_grid* g, gc_dtype dt, int* id);
+/* Times one RK4 step (4 stage kernels) of every candidate on scratch data, keeps the
+ * fastest for (extents, dtype) and returns its id; if ms is not NULL it receives the
+ * measured ms per step of each candidate (gc_ch_config_count entries). */
+gc_status gc_ch_tune(gc_context* ctx, const gc_grid* g, gc_dtype dt, int* best, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

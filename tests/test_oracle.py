@@ -1,0 +1,360 @@
+"""Pins the CPU oracle (oracle/gridcalc_oracle.c) against facts that do not come from its
+own code: exact stencil rationals and moment conditions, the exact discrete symbols of
+every operator on grid Fourier modes, the reference tests' known answers, hand-derived
+known answers for the pairwise summation tree, and the RK4/Euler amplification
+polynomials.  CPU only."""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+# ---- stencil tables ---------------------------------------------------------------------
+
+EXACT = {
+    (2, 2): [Fr(1), Fr(-2), Fr(1)],
+    (2, 4): [Fr(-1, 12), Fr(4, 3), Fr(-5, 2), Fr(4, 3), Fr(-1, 12)],
+    (1, 2): [Fr(-1, 2), Fr(0), Fr(1, 2)],
+    (1, 4): [Fr(1, 12), Fr(-2, 3), Fr(0), Fr(2, 3), Fr(-1, 12)],
+    (4, 2): [Fr(1), Fr(-4), Fr(6), Fr(-4), Fr(1)],
+    (4, 4): [Fr(-1, 6), Fr(2), Fr(-13, 2), Fr(28, 3), Fr(-13, 2), Fr(2), Fr(-1, 6)],
+}
+
+
+@pytest.mark.parametrize("kind,order", list(EXACT))
+def test_stencil_tables_exact(orc, kind, order):
+    r, offs, w = orc.stencil_table(kind, order)
+    ex = EXACT[(kind, order)]
+    assert offs == list(range(-r, r + 1))
+    assert w == [float(x) for x in ex]  # the reference's EXPECT_DOUBLE_EQ audits
+    # moment conditions of a derivative of order `kind` with accuracy `order`:
+    # sum w m^p = kind! for p = kind, 0 for other p < kind + order
+    fact = {1: 1, 2: 2, 4: 24}[kind]
+    for p in range(kind + order):
+        mom = sum(wi * Fr(m) ** p for wi, m in zip(ex, offs))
+        assert mom == (fact if p == kind else 0), (p, mom)
+
+
+# ---- discrete symbols -------------------------------------------------------------------
+
+def _sym1(order, th):   # first derivative, times h
+    return np.sin(th) if order == 2 else 4.0 / 3.0 * np.sin(th) - 1.0 / 6.0 * np.sin(2 * th)
+
+
+def _sym2(order, th):   # second derivative, times h^2
+    if order == 2:
+        return 2.0 * np.cos(th) - 2.0
+    return -1.0 / 6.0 * np.cos(2 * th) + 8.0 / 3.0 * np.cos(th) - 5.0 / 2.0
+
+
+def _sym4(order, th):   # fourth derivative, times h^4
+    if order == 2:
+        return 2.0 * np.cos(2 * th) - 8.0 * np.cos(th) + 6.0
+    return -1.0 / 3.0 * np.cos(3 * th) + 4.0 * np.cos(2 * th) - 13.0 * np.cos(th) + 28.0 / 3.0
+
+
+def _mode(shape, h, k, phase=0.3):
+    nz, ny, nx = shape
+    x = np.arange(nx) * h[0]
+    y = np.arange(ny) * h[1]
+    z = np.arange(nz) * h[2]
+    L = (nx * h[0], ny * h[1], nz * h[2])
+    arg = (2 * np.pi * k[0] / L[0] * x[None, None, :] + 2 * np.pi * k[1] / L[1] * y[None, :, None]
+           + 2 * np.pi * k[2] / L[2] * z[:, None, None] + phase)
+    th = [2 * np.pi * k[a] / (shape[2 - a]) for a in range(3)]  # k_a * h_a
+    return arg, th
+
+
+CASES = [((12, 10, 14), (0.3, 0.2, 0.5), (2, 1, 3)),
+         ((9, 16, 1), (1.0, 0.5, 1.0), (1, 3, 0)),
+         ((20, 7, 11), (0.05, 0.1, 0.2), (3, 2, 4))]
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("case", CASES)
+def test_laplacian_symbol(orc, order, case):
+    (nx, ny, nz), h, k = case
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    f = np.sin(arg)
+    lam = sum(_sym2(order, th[a]) / h[a] ** 2 for a in range(3))
+    out = orc.laplacian(g, f, order)
+    assert np.abs(out - lam * f).max() < 1e-12 * max(1.0, abs(lam))
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("case", CASES)
+def test_gradient_symbol(orc, order, case):
+    (nx, ny, nz), h, k = case
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    out = orc.gradient(g, np.sin(arg), order)
+    for a in range(3):
+        want = _sym1(order, th[a]) / h[a] * np.cos(arg)
+        assert np.abs(out[..., a] - want).max() < 1e-12 / h[a]
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("case", CASES)
+def test_divergence_symbol(orc, order, case):
+    (nx, ny, nz), h, k = case
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    phases = (0.1, 0.7, 1.9)
+    v = np.stack([np.sin(arg + p) for p in phases], axis=-1)
+    want = sum(_sym1(order, th[a]) / h[a] * np.cos(arg + phases[a]) for a in range(3))
+    assert np.abs(orc.divergence(g, v, order) - want).max() < 1e-11 / min(h)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_vector_gradient_and_tensor_divergence_symbol(orc, order):
+    (nx, ny, nz), h, k = CASES[0]
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    phases = (0.1, 0.7, 1.9)
+    v = np.stack([np.sin(arg + p) for p in phases], axis=-1)
+    M = orc.gradient_vec(g, v, order)
+    for comp in range(3):
+        for ax in range(3):  # column-major: M(comp, ax) at comp + 3*ax
+            want = _sym1(order, th[ax]) / h[ax] * np.cos(arg + phases[comp])
+            assert np.abs(M[..., comp + 3 * ax] - want).max() < 1e-11 / min(h)
+    # tensor field with m(r, c) = sin(arg + q_rc): div gives v_r = sum_c d_c m(r, c)
+    q = np.arange(9).reshape(3, 3) * 0.37
+    m = np.stack([np.sin(arg + q[i % 3, i // 3]) for i in range(9)], axis=-1)
+    dv = orc.divergence_mat(g, m, order)
+    for r in range(3):
+        want = sum(_sym1(order, th[c]) / h[c] * np.cos(arg + q[r, c]) for c in range(3))
+        assert np.abs(dv[..., r] - want).max() < 1e-10 / min(h)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("case", CASES)
+def test_biharmonic_symbol(orc, order, case):
+    (nx, ny, nz), h, k = case
+    if order == 4 and min(n for n in (nx, ny, nz) if n > 1) < 7:
+        pytest.skip("extent too small for radius 3")
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    f = np.sin(arg)
+    d2 = [_sym2(order, th[a]) / h[a] ** 2 for a in range(3)]
+    lam = sum(_sym4(order, th[a]) / h[a] ** 4 for a in range(3))
+    lam += 2 * (d2[0] * d2[1] + d2[0] * d2[2] + d2[1] * d2[2])
+    out = orc.biharmonic(g, f, order)
+    assert np.abs(out - lam * f).max() < 1e-11 * max(1.0, abs(lam))
+
+
+def test_degenerate_axis_terms_vanish(orc):
+    """2-D (Nz = 1): the z terms of every operator are exactly zero."""
+    g3 = orc.Grid.make(16, 12, 1, (0.5, 0.25, 0.7))
+    f = np.sin(np.arange(16 * 12)).reshape(1, 12, 16)
+    assert np.all(orc.gradient(g3, f)[..., 2] == 0.0)
+    g1 = orc.Grid.make(16, 12, 1, (0.5, 0.25, 123.0))  # hz must not matter
+    assert np.array_equal(orc.laplacian(g3, f), orc.laplacian(g1, f))
+
+
+def test_extent_rule(orc):
+    for n, ok in [(1, True), (2, False), (3, True), (4, True)]:
+        g = orc.Grid.make(n, 8, 8)
+        if ok:
+            orc.laplacian(g, np.zeros(g.shape), 2)
+        else:
+            with pytest.raises(orc.OracleError):
+                orc.laplacian(g, np.zeros(g.shape), 2)
+    with pytest.raises(orc.OracleError):
+        g = orc.Grid.make(8, 8, 6)
+        orc.biharmonic(g, np.zeros(g.shape), 4)  # radius 3 needs N >= 7
+
+
+# ---- finite volume ----------------------------------------------------------------------
+
+def test_fvm_known_answer(orc):
+    # fvm_cell_laplacian_test.cpp:90-120
+    g = orc.Grid.make(4, 1, 1)
+    out = orc.cell_laplacian(g, [0.0, 1.0, 1.0, 0.0], [1.0, 4.0, 4.0, 1.0])
+    assert out.ravel().tolist() == [1.6, -1.6, -1.6, 1.6]
+
+
+def test_fvm_constant_d_matches_laplacian(orc, gc):
+    # fvm_cell_laplacian_test.cpp:30-70: D = c reduces to c * laplacian
+    g = orc.Grid.make(10, 12, 8, (0.3, 0.2, 0.4))
+    psi = gc.inputs.symmetric(1, g.size).reshape(g.shape)
+    c = 2.5
+    out = orc.cell_laplacian(g, psi, np.full(g.shape, c))
+    assert np.abs(out - c * orc.laplacian(g, psi)).max() < 1e-12 * np.abs(out).max()
+
+
+def test_fvm_conservative(orc, gc):
+    # fluxes telescope: the cell Laplacian sums to zero
+    g = orc.Grid.make(9, 8, 7)
+    psi = gc.inputs.symmetric(2, g.size).reshape(g.shape)
+    D = gc.inputs.lognormal(3, g.size).reshape(g.shape)
+    out = orc.cell_laplacian(g, psi, D)
+    assert abs(out.sum()) < 1e-12 * np.abs(out).sum()
+
+
+def test_fvm_nonpositive(orc):
+    g = orc.Grid.make(4, 4, 4)
+    D = np.ones(g.shape)
+    D.ravel()[9] = -0.5
+    with pytest.raises(orc.OracleError, match="flat index 9"):
+        orc.cell_laplacian(g, np.zeros(g.shape), D)
+
+
+# ---- reductions -------------------------------------------------------------------------
+
+TWO53 = 2.0 ** 53
+
+
+def test_pairwise_leaf_of_64(orc):
+    # n = 64 is one leaf summed left to right: every +1 after 2^53 rounds away
+    x = [TWO53] + [1.0] * 63
+    assert orc.pairwise_sum(x) == TWO53
+    assert orc.neumaier_sum(x) == TWO53 + 63
+
+
+def test_pairwise_65_splits(orc):
+    # n = 65 is split: the right half's 33 ones survive as 32 (ties to even)
+    x = [TWO53] + [1.0] * 64
+    assert orc.pairwise_sum(x) == TWO53 + 32
+
+
+def test_pairwise_split_is_floor_half(orc):
+    # split at n/2 = 32: left = 32 ones, right = 2^53 + 32 ones -> 2^53; total 2^53 + 32.
+    # A split at 33 would give 2^53 + 64.
+    x = [1.0] * 32 + [TWO53] + [1.0] * 32
+    assert orc.pairwise_sum(x) == TWO53 + 32
+
+
+def test_pairwise_deeper_tree(orc):
+    # n = 256: four leaves of 64.  Leaf 1 = 2^53 (the ones above 2^53 round away),
+    # leaf 2 = 64, leaf 3 = -2^53 + 63 (exact: spacing below 2^53 is 1), leaf 4 = 64;
+    # tree (2^53 + 64) + (-2^53 + 127) = 191.  Leaves of 128 would give 127.
+    x = [TWO53] + [1.0] * 63 + [1.0] * 64 + [-TWO53] + [1.0] * 63 + [1.0] * 64
+    assert orc.pairwise_sum(x) == 191.0
+
+
+def test_neumaier_known_answers(orc):
+    assert orc.neumaier_sum([1e100, 1.0, -1e100]) == 1.0
+    assert orc.neumaier_sum([1.0, 1e100, 1.0, -1e100]) == 2.0
+    assert orc.pairwise_sum([1e100, 1.0, -1e100]) == 0.0
+
+
+def test_integrate_known_answers(orc):
+    # integrate_test.cpp:31-48
+    g = orc.Grid.make(8, 16, 4, (0.25, 0.125, 0.5))
+    ones = np.ones(g.shape)
+    vol = (8 * 0.25) * (16 * 0.125) * (4 * 0.5)
+    assert orc.integrate(g, ones, orc.PAIRWISE) == vol
+    assert orc.integrate(g, ones, orc.KAHAN) == vol
+    # integrate_test.cpp:81-101: pairwise vs Kahan within 1e-13 relative
+    n = 64
+    h = 2 * np.pi / n
+    x = np.arange(n) * h
+    f = np.broadcast_to(1.0 + np.sin(x)[None, None, :] * np.cos(x)[None, :, None], (n, n, n))
+    gp = orc.Grid.make(n, n, n, h)
+    a, b = orc.integrate(gp, f, orc.PAIRWISE), orc.integrate(gp, f, orc.KAHAN)
+    assert abs(a - b) / max(abs(a), abs(b)) < 1e-13
+    assert abs(a - (2 * np.pi) ** 3) < 1e-9
+
+
+def test_free_energy_closed_form(orc):
+    """F for psi = A sin(k x) on [0, L)^3: the O2 discrete gradient is A sin(th)/h cos(kx),
+    so F = V*(-A^2/4 + 3A^4/32 + kappa/4 * A^2 sin(th)^2 / h^2) (grid sums of sin^2, sin^4,
+    cos^2 over whole periods)."""
+    n, A, kappa = 16, 0.7, 0.9
+    h = 2 * np.pi / n
+    g = orc.Grid.make(n, n, n, h)
+    x = np.arange(n) * h
+    psi = np.broadcast_to(A * np.sin(2 * x)[None, None, :], (n, n, n)).copy()
+    V = (n * h) ** 3
+    th = 2 * h
+    want = V * (-A * A / 4 + 3 * A ** 4 / 32 + kappa / 4 * A * A * np.sin(th) ** 2 / h ** 2)
+    assert abs(orc.free_energy(g, psi, kappa) - want) < 1e-12 * abs(want) * 10
+    wantE = V * A * A * np.sin(th) ** 2 / h ** 2 / 2
+    assert abs(orc.gradient_energy(g, psi) - wantE) < 1e-11 * wantE
+
+
+# ---- time integration ---------------------------------------------------------------------
+
+def _g_rk4(z):
+    return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_diffusion_eigenmode_amplification(orc, method):
+    """psi0 = sin x sin y sin z is an eigenvector of the O2 Laplacian with eigenvalue
+    lam = 3*(2cos h - 2)/h^2, so n steps multiply it by g(D dt lam)^n exactly
+    (g = 1 + z for Euler, the RK4 polynomial for RK4) up to rounding."""
+    N = 12
+    h = 2 * np.pi / N
+    g = orc.Grid.make(N, N, N, h)
+    x = np.arange(N) * h
+    psi = np.sin(x)[None, None, :] * np.sin(x)[None, :, None] * np.sin(x)[:, None, None]
+    lam = 3 * (2 * np.cos(h) - 2) / h ** 2
+    D, dt, n = 0.1, 0.3 * h * h, 40
+    out = orc.diffuse_const(g, psi, D, dt, n, method)
+    z = D * dt * lam
+    amp = (_g_rk4(z) if method == 1 else 1 + z) ** n
+    assert np.abs(out - amp * psi).max() < 1e-13
+
+
+def test_rk4_time_order(orc):
+    """integrator_test.cpp:154-205 analogue on a grid eigenmode: halving dt cuts the
+    time-discretisation error by ~16 (order 4) against the exact exp(D lam T)."""
+    N = 8
+    h = 2 * np.pi / N
+    g = orc.Grid.make(N, N, N, h)
+    x = np.arange(N) * h
+    psi = np.sin(x)[None, None, :] * np.sin(x)[None, :, None] * np.sin(x)[:, None, None]
+    lam = 3 * (2 * np.cos(h) - 2) / h ** 2
+    D, T = 1.0, 1.0
+    errs = []
+    for n in (4, 8, 16):
+        dt = T / n
+        out = orc.diffuse_const(g, psi, D, dt, n, 1) if D * dt * 3 / h ** 2 <= 0.6963 else None
+        if out is None:
+            continue
+        errs.append(np.abs(out - np.exp(D * lam * T) * psi).max())
+    assert len(errs) >= 2
+    for a, b in zip(errs, errs[1:]):
+        assert 12 < a / b < 20
+
+
+def test_cfl_rule(orc):
+    g = orc.Grid.make(8, 8, 8, 0.5)
+    # D*dt*(3/h^2) = 12*D*dt
+    assert not orc.cfl_violated(g, 1.0, 0.5 / 12, 0.5)
+    assert orc.cfl_violated(g, 1.0, 0.5 / 12 * 1.001, 0.5)
+    with pytest.raises(orc.OracleError):
+        orc.diffuse_const(g, np.zeros(g.shape), 1.0, 0.1, 1, 0)
+
+
+def test_ch_rhs_invariants(orc, gc):
+    # cahn_hilliard_test.cpp:47-78: zero for a constant field; conservative
+    g = orc.Grid.make(8, 8, 8)
+    assert np.abs(orc.ch_rhs(g, np.full(g.shape, 0.42))).max() < 1e-12
+    g = orc.Grid.make(16, 16, 16)
+    psi = gc.inputs.ch_initial(16, 16, 16, 0.3, seed=123)
+    r = orc.ch_rhs(g, psi)
+    assert abs(r.sum()) < 1e-10 * np.abs(r).max() * r.size
+
+
+def test_ch_rhs_linear_symbol(orc):
+    """For small amplitude the CH right-hand side of A sin(kx) is, to O(A^3),
+    M*lam*(-1 - kappa*lam)*A sin(kx) with lam the O2 Laplacian symbol."""
+    n, A, M, kappa = 16, 1e-6, 1.3, 0.7
+    g = orc.Grid.make(n, 1, 1)
+    x = np.arange(n)
+    psi = A * np.sin(2 * np.pi * 3 * x / n).reshape(1, 1, n)
+    lam = 2 * np.cos(2 * np.pi * 3 / n) - 2
+    want = M * lam * (-1 - kappa * lam) * psi
+    assert np.abs(orc.ch_rhs(g, psi, M, kappa) - want).max() < 1e-15
+
+
+def test_ch_free_energy_decreases(orc, gc):
+    g = orc.Grid.make(24, 24, 1)
+    psi = gc.inputs.ch_initial(24, 24, 1, 0.3, seed=5)
+    F = [orc.free_energy(g, psi, 1.0)]
+    for _ in range(4):
+        psi = orc.ch_integrate(g, psi, 1.0, 1.0, 0.02, 60, orc.RK4)
+        F.append(orc.free_energy(g, psi, 1.0))
+    assert all(b <= a for a, b in zip(F, F[1:])), F
