@@ -282,9 +282,47 @@ struct Ch3Cfg {
   static_assert((OX * OY) % NT == 0, "tile must be a multiple of the block");
 };
 
-template <class T, int OX, int OY, int ZC>
-__global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
-                                                    const T* __restrict__ y, T* psi,
+// Stage operands and results as the tiled kernels move them: operands read once and
+// results written once per stage bypass cache residency (non-temporal), so the caches keep
+// the y planes whose halos neighbouring tiles re-read.
+template <int STAGE, class T>
+__device__ __forceinline__ void stage_in(const T* psi, const T* a, const T* b, unsigned c, T& pv,
+                                         T& av, T& bv) {
+  if constexpr (STAGE >= kS2 && STAGE <= kS4) pv = __builtin_nontemporal_load(psi + c);
+  if constexpr (STAGE == kS4) av = __builtin_nontemporal_load(a + c);
+  if constexpr (STAGE == kS3 || STAGE == kS4) bv = __builtin_nontemporal_load(b + c);
+}
+
+template <int STAGE, class T>
+__device__ __forceinline__ void stage_out(unsigned c, T kv, T pv, T av, T bv, T* psi, T* ynext,
+                                          T* a, T* b, const StageCoef<T>& co) {
+  if constexpr (STAGE == kRhsOnly) {
+    __builtin_nontemporal_store(kv, ynext + c);
+  } else if constexpr (STAGE == kS1) {
+    __builtin_nontemporal_store(kv, a + c);
+    __builtin_nontemporal_store(pv + co.half_dt * kv, ynext + c);
+  } else if constexpr (STAGE == kS2) {
+    __builtin_nontemporal_store(kv, b + c);
+    __builtin_nontemporal_store(pv + co.half_dt * kv, ynext + c);
+  } else if constexpr (STAGE == kS3) {
+    __builtin_nontemporal_store(bv + kv, b + c);
+    __builtin_nontemporal_store(pv + co.dt * kv, ynext + c);
+  } else if constexpr (STAGE == kS4) {
+    __builtin_nontemporal_store(pv + (co.sixth_dt * (av + kv) + co.third_dt * bv), psi + c);
+  } else {  // explicit Euler: ynext = y + dt*f(y)
+    __builtin_nontemporal_store(pv + co.dt * kv, ynext + c);
+  }
+}
+
+// S1 and Euler take the stage's base value from the stencil input itself (y = psi), which
+// the kernels already hold in LDS.
+template <int STAGE>
+constexpr bool base_from_y() {
+  return STAGE == kS1 || STAGE == kEuler;
+}
+
+template <class T, int OX, int OY, int ZC, int STAGE>
+__global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __restrict__ y, T* psi,
                                                     T* __restrict__ ynext, T* __restrict__ a,
                                                     T* __restrict__ b, Dims d, int ntx, int nty,
                                                     StageCoef<T> co) {
@@ -300,45 +338,43 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
   const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
   const int nout = min(ZC, d.nz - z0);
 
-  // xy part of the global offset of every y-plane slot this thread fills
-  int yoff[C::YL];
+  // xy part of the (32-bit) offset of every y-plane slot this thread fills; only the
+  // last slot can fall outside the halo tile
+  unsigned yoff[C::YL];
 #pragma unroll
   for (int s = 0; s < C::YL; ++s) {
     const int idx = t + s * C::NT;
-    if (idx < C::YN) {
-      const int px = idx % C::YW, py = idx / C::YW;
-      yoff[s] = wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny);
-    } else {
-      yoff[s] = -1;
-    }
+    const int px = idx % C::YW, py = idx / C::YW;
+    yoff[s] = (s + 1 < C::YL || idx < C::YN)
+                  ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
+                  : 0u;
   }
-  // output sites of this thread: column ox, rows oy0 + q*(NT/OX)
+  constexpr bool kLastYPartial = (C::YN % C::NT) != 0;
+  // output sites: column ox, rows oy0 + q*ROWS
   constexpr int ROWS = C::NT / OX;
   const int ox = t % OX, oy0 = t / OX;
-  const bool xin = x0 + ox < d.nx;
+  unsigned ooff[C::OR];
+  bool oin[C::OR];
+#pragma unroll
+  for (int q = 0; q < C::OR; ++q) {
+    const int gy = y0 + oy0 + q * ROWS;
+    oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
+    ooff[q] = oin[q] ? (unsigned)(x0 + ox + d.nx * gy) : 0u;
+  }
 
   T yreg[C::YL];
   auto load_y = [&](int plane) {
-    const size_t base = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+    const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
 #pragma unroll
     for (int s = 0; s < C::YL; ++s)
-      if (yoff[s] >= 0) yreg[s] = y[base + (size_t)yoff[s]];
+      if (s + 1 < C::YL || !kLastYPartial || t + s * C::NT < C::YN) yreg[s] = yp[yoff[s]];
   };
   T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
-  auto load_pt = [&](int k) {
-#pragma unroll
-    for (int q = 0; q < C::OR; ++q) {
-      const int gy = y0 + oy0 + q * ROWS;
-      if (xin && gy < d.ny) {
-        const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy + (size_t)d.plane * k;
-        if (stage >= kS1 && stage <= kS4) pv[q] = psi[c];
-        if (stage == kS4) av[q] = a[c];
-        if (stage >= kS3 && stage <= kS4) bv[q] = b[c];
-        if (stage == kEuler) pv[q] = y[c];
-      }
-    }
-  };
 
+  // Iteration p stores y plane z0-2+p into ring slot p%3, computes mu plane z0+p-3 from y
+  // planes p-2..p, then the output plane z0+p-4 from mu planes p-2..p.  The two barriers
+  // separate the y store from the mu pass and the mu pass from the output pass; every slot
+  // written in iteration p was last read before the first barrier of iteration p.
   load_y(z0 - 2);
   const int last = nout + 3;
   for (int p = 0; p <= last; ++p) {
@@ -346,9 +382,14 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
     T* yw = ys[s0];
 #pragma unroll
     for (int s = 0; s < C::YL; ++s)
-      if (yoff[s] >= 0) yw[t + s * C::NT] = yreg[s];
+      if (s + 1 < C::YL || !kLastYPartial || t + s * C::NT < C::YN) yw[t + s * C::NT] = yreg[s];
     if (p < last) load_y(z0 - 1 + p);
-    if (p >= 4) load_pt(z0 + p - 4);
+    if (p >= 4) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 + p - 4);
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q)
+        if (oin[q]) stage_in<STAGE>(psi + kb, a + kb, b + kb, ooff[q], pv[q], av[q], bv[q]);
+    }
     __syncthreads();
     if (p >= 2) {  // mu plane z0+p-3 from y planes p-2, p-1, p
       const T* yc = ys[s1];
@@ -358,7 +399,7 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
 #pragma unroll
       for (int s = 0; s < C::ML; ++s) {
         const int idx = t + s * C::NT;
-        if (idx < C::MN) {
+        if (s + 1 < C::ML || idx < C::MN) {
           const int mx = idx % C::MW, my = idx / C::MW;
           const int o = (my + 1) * C::YW + (mx + 1);
           const T l = lap_lds(yc, yb, ya, o, C::YW, cf.ihx2, cf.ihy2, cf.ihz2);
@@ -368,29 +409,28 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, int stage,
       }
     }
     __syncthreads();
-    if (p >= 4) {  // output plane k = z0+p-4 from mu planes p-2, p-1, p
-      const int k = z0 + p - 4;
+    if (p >= 4) {  // output plane z0+p-4 from mu planes p-2, p-1, p
+      const size_t kb = (size_t)d.plane * (size_t)(z0 + p - 4);
       const T* mc = ms[s1];
       const T* mb = ms[s2];
       const T* ma = ms[s0];
 #pragma unroll
       for (int q = 0; q < C::OR; ++q) {
-        const int oy = oy0 + q * ROWS;
-        const int gy = y0 + oy;
-        if (xin && gy < d.ny) {
+        if (oin[q]) {
+          const int oy = oy0 + q * ROWS;
           const int o = (oy + 1) * C::MW + (ox + 1);
           const T kv = lap_lds(mc, mb, ma, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
-          const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy + (size_t)d.plane * k;
-          stage_update_regs(stage, c, kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
+          if constexpr (base_from_y<STAGE>()) pv[q] = ys[s2][(oy + 2) * C::YW + (ox + 2)];
+          stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi + kb, ynext + kb, a + kb,
+                           b + kb, co);
         }
       }
     }
   }
 }
 
-template <class T, int OX, int OY>
-__global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, int stage,
-                                                    const T* __restrict__ y, T* psi,
+template <class T, int OX, int OY, int STAGE>
+__global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, const T* __restrict__ y, T* psi,
                                                     T* __restrict__ ynext, T* __restrict__ a,
                                                     T* __restrict__ b, Dims d, int ntx,
                                                     StageCoef<T> co) {
@@ -404,33 +444,30 @@ __global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, int stage,
   const int x0 = tx * OX, y0 = ty * OY;
   constexpr int ROWS = C::NT / OX;
   const int ox = t % OX, oy0 = t / OX;
-  const bool xin = x0 + ox < d.nx;
 
 #pragma unroll
   for (int s = 0; s < C::YL; ++s) {
     const int idx = t + s * C::NT;
-    if (idx < C::YN) {
+    if (s + 1 < C::YL || idx < C::YN) {
       const int px = idx % C::YW, py = idx / C::YW;
       ys[idx] = y[wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny)];
     }
   }
   T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+  unsigned ooff[C::OR];
+  bool oin[C::OR];
 #pragma unroll
   for (int q = 0; q < C::OR; ++q) {
     const int gy = y0 + oy0 + q * ROWS;
-    if (xin && gy < d.ny) {
-      const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy;
-      if (stage >= kS1 && stage <= kS4) pv[q] = psi[c];
-      if (stage == kS4) av[q] = a[c];
-      if (stage >= kS3 && stage <= kS4) bv[q] = b[c];
-      if (stage == kEuler) pv[q] = y[c];
-    }
+    oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
+    ooff[q] = oin[q] ? (unsigned)(x0 + ox + d.nx * gy) : 0u;
+    if (oin[q]) stage_in<STAGE>(psi, a, b, ooff[q], pv[q], av[q], bv[q]);
   }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < C::ML; ++s) {
     const int idx = t + s * C::NT;
-    if (idx < C::MN) {
+    if (s + 1 < C::ML || idx < C::MN) {
       const int mx = idx % C::MW, my = idx / C::MW;
       const int o = (my + 1) * C::YW + (mx + 1);
       // Nz = 1: the z neighbours are the centre itself
@@ -442,13 +479,12 @@ __global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, int stage,
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < C::OR; ++q) {
-    const int oy = oy0 + q * ROWS;
-    const int gy = y0 + oy;
-    if (xin && gy < d.ny) {
+    if (oin[q]) {
+      const int oy = oy0 + q * ROWS;
       const int o = (oy + 1) * C::MW + (ox + 1);
       const T kv = lap_lds(ms, ms, ms, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
-      const size_t c = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy;
-      stage_update_regs(stage, c, kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
+      if constexpr (base_from_y<STAGE>()) pv[q] = ys[(oy + 2) * C::YW + (ox + 2)];
+      stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
     }
   }
 }
@@ -485,20 +521,38 @@ constexpr int kDefault3 = 7, kDefault2 = 2;
 // Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
 inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
 
+#define GC_STAGE_SWITCH(LAUNCH)      \
+  switch (stage) {                   \
+    case kRhsOnly: LAUNCH(kRhsOnly); break; \
+    case kS1: LAUNCH(kS1); break;    \
+    case kS2: LAUNCH(kS2); break;    \
+    case kS3: LAUNCH(kS3); break;    \
+    case kS4: LAUNCH(kS4); break;    \
+    default: LAUNCH(kEuler); break;  \
+  }
+
 template <class T, int OX, int OY, int ZC>
 void launch_ch3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int stage, const T* y,
                 T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
   const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY, nzc = (d.nz + ZC - 1) / ZC;
-  hipLaunchKernelGGL((k_ch_stage3d<T, OX, OY, ZC>), dim3((unsigned)(ntx * nty * nzc)), dim3(256),
-                     0, ctx->stream, cf, stage, y, psi, ynext, a, b, d, ntx, nty, co);
+  const dim3 grid((unsigned)(ntx * nty * nzc));
+#define GC_L3(S)                                                                         \
+  hipLaunchKernelGGL((k_ch_stage3d<T, OX, OY, ZC, S>), grid, dim3(256), 0, ctx->stream, cf, y, \
+                     psi, ynext, a, b, d, ntx, nty, co)
+  GC_STAGE_SWITCH(GC_L3)
+#undef GC_L3
 }
 
 template <class T, int OX, int OY>
 void launch_ch2(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int stage, const T* y,
                 T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
   const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY;
-  hipLaunchKernelGGL((k_ch_stage2d<T, OX, OY>), dim3((unsigned)(ntx * nty)), dim3(256), 0,
-                     ctx->stream, cf, stage, y, psi, ynext, a, b, d, ntx, co);
+  const dim3 grid((unsigned)(ntx * nty));
+#define GC_L2(S)                                                                         \
+  hipLaunchKernelGGL((k_ch_stage2d<T, OX, OY, S>), grid, dim3(256), 0, ctx->stream, cf, y, psi, \
+                     ynext, a, b, d, ntx, co)
+  GC_STAGE_SWITCH(GC_L2)
+#undef GC_L2
 }
 
 template <class T>
