@@ -1,18 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: fused RK4 Cahn-Hilliard site-updates/s on one MI355X.
 
-Workload (the RK4 configuration of the headline metric): 3-D Cahn-Hilliard on a 256^3
-periodic grid, dx=1, M=1, kappa=0.5, dt=0.005, c0 = 0.1*U(-1,1); stencils and RK4 stages
-in FP32, total mass and free energy reduced in FP64 (compensated) every 500 steps, as the
-reference driver prints diagnostics per snapshot chunk.  One "step" is one RK4 step of the
-whole grid (4 fused stage kernels).  Inputs are resident in HBM before timing starts.
+Default workload (the RK4 configuration of the headline metric): 3-D Cahn-Hilliard on a
+256^3 periodic grid, dx=1, M=1, kappa=0.5, dt=0.005, c0 = 0.1*U(-1,1); stencils and RK4
+stages in FP32, total mass and free energy reduced in FP64 (compensated) every 500 steps,
+as the reference driver prints diagnostics per snapshot chunk.  One "step" is one RK4
+step of the whole grid: 4 fused stage kernels.  Inputs are resident in HBM before timing.
 
 Also reported: the FP32 result's L2/Linf error against an FP64 GPU run of the same steps,
-the FP64 pipeline's own throughput, the roofline position of the stage kernel, and the
-single-core CPU oracle (a faithful C restatement of the reference path, FP64) timed on a
-bounded prefix of the same workload.
+the FP64 pipeline's throughput, the roofline position of the fused step, and the
+single-core CPU oracle (a C restatement of the reference path, FP64) timed on a bounded
+prefix of the same workload.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-fp64]
+  python bench.py --workload sweep [--n 512]      # operator GB/s (3-D operator sweep)
 """
 from __future__ import annotations
 
@@ -29,7 +30,10 @@ sys.path.insert(0, ROOT)
 import __graft_entry__ as entry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-WORDS_PER_SITE_STEP = 16  # algorithmic HBM words per site per fused RK4 step (4 per stage)
+# Algorithmic HBM words per site per fused RK4 step: a low-storage RK4 reads and writes
+# 16 words per site (4 stages x (stencil input + stage state)).  The bit-exact stage
+# pipeline here moves 17 (it keeps k1 and k2+k3 apart as the reference combines them).
+WORDS_PER_SITE_STEP = 16
 
 
 def parse():
@@ -37,24 +41,32 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--n", type=int, default=None)
+    p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy"])
     p.add_argument("--diag-every", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
-    p.add_argument("--cpu-steps", type=int, default=0, help="CPU oracle sample steps (0=auto)")
+    p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=4)
     return p.parse_args()
 
 
-def run_gpu(gc, ctx, g, c0, dtype, steps, warmup, diag_every, kappa, dt):
-    d = gc.DeviceArray.from_numpy(ctx, c0.astype(dtype))
-    if warmup:
-        w = gc.DeviceArray.from_numpy(ctx, c0.astype(dtype))
-        gc.ch_integrate(ctx, g, w, 1.0, kappa, dt, warmup, gc.RK4)
-        ctx.sync()
-        w.free()
+def profiled_traffic(name):
+    """Bytes per fused step measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by the
+    device-copy calibration) for this workload, as recorded in profiles/<name>."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_step ="):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def run_timed(gc, ctx, g, d, steps, diag_every, kappa, dt):
     diags = []
     ctx.sync()
-    t0 = time.perf_counter()
     ctx.timer_start()
     done = 0
     while done < steps:
@@ -65,13 +77,12 @@ def run_gpu(gc, ctx, g, c0, dtype, steps, warmup, diag_every, kappa, dt):
         F = gc.ch_free_energy(ctx, g, d, kappa, gc.KAHAN)
         diags.append((done, mass, F))
     ms = ctx.timer_stop()
-    wall = time.perf_counter() - t0
-    return d, ms, wall, diags
+    return ms, diags
 
 
-def stage_kernel_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
-    """Average duration of the fused stage kernel: HIP events on the context stream around
-    `steps` RK4 steps with nothing else in between (4 launches per step)."""
+def step_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
+    """ms per fused RK4 step: HIP events on the context's stream around `steps` steps and
+    nothing else (4 stage-kernel launches per step)."""
     d = gc.DeviceArray.from_numpy(ctx, c0.astype(dtype))
     gc.ch_integrate(ctx, g, d, 1.0, kappa, dt, 2, gc.RK4)
     ctx.sync()
@@ -79,95 +90,179 @@ def stage_kernel_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
     gc.ch_integrate(ctx, g, d, 1.0, kappa, dt, steps, gc.RK4)
     ms = ctx.timer_stop()
     d.free()
-    return ms / (4 * steps)
+    return ms / steps
 
 
-def cpu_baseline(orc, n, c0, kappa, dt, steps):
-    og = orc.Grid.make(n, n, n, 1.0)
-    t0 = time.perf_counter()
-    orc.ch_integrate(og, c0, 1.0, kappa, dt, steps, orc.RK4)
-    el = time.perf_counter() - t0
-    return n ** 3 * steps / el, el
-
-
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the stencil and "
-              "RK4 path would shard by z-slabs with halo exchange, not built here)", file=sys.stderr)
-        sys.exit(2)
+def ch3d(a):
     gc = entry.load_package()
-    n, kappa, dt = a.n, 0.5, 0.005
+    n = a.n or 256
+    kappa, dt = 0.5, 0.005
     g = gc.Grid.make(n, n, n, 1.0)
     c0 = gc.inputs.ch_initial(n, n, n, 0.1, seed=3)
     sites = n ** 3
-
+    extra = {}
     with gc.Context() as ctx:
-        d32, ms, wall, diags = run_gpu(gc, ctx, g, c0, np.float32, a.steps, a.warmup,
-                                       a.diag_every, kappa, dt)
-        ms_per_step = ms / a.steps
+        tiles = {}
+        for dtype, name in ((gc.F32, "f32"), (gc.F64, "f64")):
+            if not a.no_tune:
+                best, ms = gc.ch_tune(ctx, g, dtype)
+            tiles[name] = gc.ch_config_shape(g, gc.ch_get_config(g, dtype))
+        extra["tile_ox_oy_zc"] = tiles
+
+        d32 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
+        if a.warmup:
+            w = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
+            gc.ch_integrate(ctx, g, w, 1.0, kappa, dt, a.warmup, gc.RK4)
+            ctx.sync()
+            w.free()
+        ms, diags = run_timed(gc, ctx, g, d32, a.steps, a.diag_every, kappa, dt)
         value = sites * a.steps / (ms / 1e3)
-        k_ms = stage_kernel_ms(gc, ctx, g, c0, np.float32, min(a.steps, 200), kappa, dt)
-        bytes_launch = WORDS_PER_SITE_STEP / 4 * 4 * sites
-        achieved = bytes_launch / (k_ms / 1e3) / 1e9
-        extra = {}
+        sms = step_ms(gc, ctx, g, c0, np.float32, min(a.steps, 400), kappa, dt)
+        bytes_step = WORDS_PER_SITE_STEP * 4 * sites
+        achieved = bytes_step / (sms / 1e3) / 1e9
+        extra["diagnostics_fp32"] = [{"step": s, "mass": m, "F": f} for s, m, f in diags]
         if not a.no_fp64:
-            d64, ms64, _, diags64 = run_gpu(gc, ctx, g, c0, np.float64, a.steps, 0,
-                                            a.diag_every, kappa, dt)
+            d64 = gc.DeviceArray.from_numpy(ctx, c0)
+            ms64, diags64 = run_timed(gc, ctx, g, d64, a.steps, a.diag_every, kappa, dt)
             r32 = d32.numpy(ctx).astype(np.float64)
             r64 = d64.numpy(ctx)
-            l2 = float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))
-            linf = float(np.abs(r32 - r64).max() / np.abs(r64).max())
-            k64 = stage_kernel_ms(gc, ctx, g, c0, np.float64, min(a.steps, 100), kappa, dt)
-            extra["fp32_vs_fp64"] = {"l2_rel": l2, "linf_rel": linf}
+            extra["fp32_vs_fp64"] = {
+                "l2_rel": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64)),
+                "linf_rel": float(np.abs(r32 - r64).max() / np.abs(r64).max()),
+                "mass_fp32": diags[-1][1], "mass_fp64": diags64[-1][1],
+                "F_fp32": diags[-1][2], "F_fp64": diags64[-1][2]}
+            sms64 = step_ms(gc, ctx, g, c0, np.float64, min(a.steps, 200), kappa, dt)
             extra["fp64"] = {
                 "site_updates_per_s": sites * a.steps / (ms64 / 1e3),
                 "ms_per_step": ms64 / a.steps,
-                "stage_kernel_ms": k64,
-                "hbm_frac": (WORDS_PER_SITE_STEP / 4 * 8 * sites) / (k64 / 1e3) / 1e9 / HBM_PEAK_GBS,
-                "mass_final": diags64[-1][1], "F_final": diags64[-1][2],
-            }
-        extra["diagnostics_fp32"] = [{"step": s, "mass": m, "F": f} for s, m, f in diags]
+                "step_kernels_ms": sms64,
+                "hbm_GBs_alg": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9,
+                "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS}
 
     cpu = None
     if not a.no_cpu:
         orc = entry.load_oracle()
         orc.lib()
-        steps_cpu = a.cpu_steps or 4
-        cores = 1
         try:
             os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-        except Exception:
+        except (AttributeError, OSError):
             pass
-        v, el = cpu_baseline(orc, n, c0, kappa, dt, steps_cpu)
-        cpu = {"value": v, "unit": "site-updates/s", "cores": cores, "kind": "port",
-               "sample": f"oracle (C restatement of the reference path, FP64, -O2) RK4 on the "
-                         f"same {n}^3 input, {steps_cpu} steps ({el:.1f} s), 1 thread"}
+        og = orc.Grid.make(n, n, n, 1.0)
+        t0 = time.perf_counter()
+        orc.ch_integrate(og, c0, 1.0, kappa, dt, a.cpu_steps, orc.RK4)
+        el = time.perf_counter() - t0
+        cpu = {"value": sites * a.cpu_steps / el, "unit": "site-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"CPU oracle (C restatement of the reference path, FP64, gcc -O2, "
+                         f"1 thread): RK4 on the same {n}^3 input, first {a.cpu_steps} steps "
+                         f"({el:.1f} s)"}
 
-    out = {
+    return {
         "metric": "RK4 site-updates/s",
         "value": value,
         "unit": "site-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"3-D Cahn-Hilliard fused RK4, {n}^3, dx=1, M=1, kappa=0.5, "
-                               "dt=0.005, c0=0.1*U(-1,1); FP32 stages, FP64 compensated mass/F "
-                               f"every {a.diag_every} steps",
-                   "grid": [n, n, n], "dt": dt, "kappa": kappa},
+                               f"dt=0.005, c0=0.1*U(-1,1) seed 3; FP32 stages, FP64 "
+                               f"compensated mass/F every {a.diag_every} steps",
+                   "grid": [n, n, n], "dt": dt, "kappa": kappa, "M": 1.0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage<float, CHRhs<float>>",
-                     "kernel_avg_ms": k_ms, "bytes_per_launch": bytes_launch},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(f"ch3d_rk4_stage_kernels_{n}_f32.md"),
+                     "kernel": "k_ch_stage3d<float,...> x4 (one fused RK4 step = 4 stage launches)",
+                     "step_kernels_ms": sms, "alg_bytes_per_step": bytes_step},
         "cpu_baseline": cpu,
         "extra": extra,
     }
+
+
+def sweep(a):
+    """3-D operator sweep: GB/s per operator from algorithmic bytes, FP64 and FP32, and the
+    FP32 error against FP64."""
+    gc = entry.load_package()
+    n = a.n or 512
+    h = 1.0 / n
+    g = gc.Grid.make(n, n, n, h)
+    u = gc.inputs.sweep_field(n, h, seed=7)
+    rows = []
+    words = {"laplacian": 2, "biharmonic": 2, "gradient": 4, "divergence": 4, "cell_laplacian": 3}
+    # three independent components for the vector field; D ~ lognormal(0, 1)
+    vv = np.empty((n, n, n, 3), np.float64)
+    for c in range(3):
+        vv[..., c] = gc.inputs.sweep_field(n, h, seed=11 + c)
+    D = gc.inputs.lognormal(5, n ** 3).reshape(n, n, n)
+    with gc.Context() as ctx:
+        for dtype, w in ((np.float64, 8), (np.float32, 4)):
+            d_u = gc.DeviceArray.from_numpy(ctx, u.astype(dtype))
+            d_v = gc.DeviceArray.from_numpy(ctx, vv.astype(dtype))
+            d_D = gc.DeviceArray.from_numpy(ctx, D.astype(dtype))
+            outs = {"laplacian": gc.DeviceArray((n, n, n), dtype),
+                    "gradient": gc.DeviceArray((n, n, n, 3), dtype)}
+            for name, order in [("laplacian", 2), ("laplacian", 4), ("gradient", 2), ("gradient", 4),
+                                ("divergence", 2), ("divergence", 4), ("biharmonic", 2),
+                                ("biharmonic", 4), ("cell_laplacian", 2)]:
+                def run():
+                    if name == "laplacian":
+                        gc.laplacian(ctx, g, d_u, order, out=outs["laplacian"])
+                    elif name == "biharmonic":
+                        gc.biharmonic(ctx, g, d_u, order, out=outs["laplacian"])
+                    elif name == "gradient":
+                        gc.gradient(ctx, g, d_u, order, out=outs["gradient"])
+                    elif name == "divergence":
+                        gc.divergence(ctx, g, d_v, order, out=outs["laplacian"])
+                    else:
+                        gc.cell_laplacian(ctx, g, d_u, d_D, out=outs["laplacian"])
+                run()
+                ctx.sync()
+                reps = 10
+                ctx.timer_start()
+                for _ in range(reps):
+                    run()
+                ms = ctx.timer_stop() / reps
+                gbs = words[name] * w * n ** 3 / (ms / 1e3) / 1e9
+                rows.append({"op": name, "order": order, "dtype": "f64" if w == 8 else "f32",
+                             "ms": ms, "GBs_alg": gbs, "hbm_frac": gbs / HBM_PEAK_GBS})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+            for x in (d_u, d_v, d_D, *outs.values()):
+                x.free()
+    return {"metric": "operator GB/s (algorithmic bytes)", "n": n, "rows": rows}
+
+
+def copy(a):
+    """Plain device-to-device copy of a known size (256 MiB): the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    gc = entry.load_package()
+    nbytes = 256 << 20
+    with gc.Context() as ctx:
+        src = gc.DeviceArray((nbytes // 8,), np.float64)
+        dst = gc.DeviceArray((nbytes // 8,), np.float64)
+        dst.copy_from(ctx, src)
+        ctx.sync()
+        reps = 10
+        ctx.timer_start()
+        for _ in range(reps):
+            dst.copy_from(ctx, src)
+        ms = ctx.timer_stop() / reps
+    return {"metric": "device copy GB/s", "bytes_per_copy": nbytes, "ms": ms,
+            "GBs": 2 * nbytes / (ms / 1e3) / 1e9}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the stencil and "
+              "RK4 path would shard by z-slabs with a halo exchange per stage; not built)",
+              file=sys.stderr)
+        sys.exit(2)
+    out = {"ch3d": ch3d, "sweep": sweep, "copy": copy}[a.workload](a)
     print(json.dumps(out))
 
 

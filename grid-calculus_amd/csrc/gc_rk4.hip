@@ -516,7 +516,8 @@ constexpr TileCfg kCh2Cfgs[] = {{64, 4, 1},  {64, 8, 1},   {64, 16, 1},
                                 {64, 32, 1}, {128, 8, 1}, {128, 16, 1}};
 constexpr int kN3 = sizeof(kCh3Cfgs) / sizeof(kCh3Cfgs[0]);
 constexpr int kN2 = sizeof(kCh2Cfgs) / sizeof(kCh2Cfgs[0]);
-constexpr int kDefault3 = 7, kDefault2 = 2;
+// defaults: the measured best on 256^3 (FP32: 64x16x16, FP64: 64x8x32) and 2-D 1024^2
+constexpr int kDefault3F32 = 6, kDefault3F64 = 4, kDefault2 = 2;
 
 // Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
 inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
@@ -602,7 +603,7 @@ int ch_config(const Dims& d, int dt) {
   std::lock_guard<std::mutex> lock(g_cfg_mu);
   auto it = g_cfg.find(CfgKey{d.nx, d.ny, d.nz, dt});
   if (it != g_cfg.end()) return it->second;
-  return d.nz == 1 ? kDefault2 : kDefault3;
+  return d.nz == 1 ? kDefault2 : (dt == GC_F64 ? kDefault3F64 : kDefault3F32);
 }
 
 template <class T>
