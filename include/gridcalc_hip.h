@@ -197,6 +197,47 @@ gc_status gc_ch_get_config(const gc_grid* g, gc_dtype dt, int* id);
  * measured ms per step of each candidate (gc_ch_config_count entries). */
 gc_status gc_ch_tune(gc_context* ctx, const gc_grid* g, gc_dtype dt, int* best, float* ms);
 
+/* ---- batched ensembles of 2-D Cahn-Hilliard grids ---------------------------------
+ * No counterpart in the reference: many independent periodic 2-D grids (uneven extents,
+ * their own M, kappa and cell size, one dtype, one time step) advanced by explicit RK4
+ * with the right-hand side of examples/common/cahn_hilliard.hpp:38-59.  Each RK4 step is
+ * ONE kernel: a block runs all four stages of one (member, tile) work item in LDS
+ * (temporal blocking with an 8-site halo), so a step reads each state once (plus halos)
+ * and writes it once.  Results equal the per-member solve::integrate(..., RK4) arithmetic
+ * up to the sign of an exact zero.  States live in the ensemble's own device pool. */
+
+typedef struct gc_ensemble gc_ensemble;
+
+typedef struct gc_ensemble_member {
+  int32_t nx, ny;
+  double hx, hy;
+  double M, kappa;
+} gc_ensemble_member;
+
+/* Creates an ensemble of n members with zero initial states. */
+gc_status gc_ensemble_create(gc_context* ctx, gc_dtype dt, int n,
+                             const gc_ensemble_member* members, double step,
+                             gc_ensemble** out);
+gc_status gc_ensemble_destroy(gc_ensemble* e);
+/* Copies member i's state (nx*ny values of the ensemble's dtype) from / to the host. */
+gc_status gc_ensemble_set_state(gc_ensemble* e, int i, const void* host);
+gc_status gc_ensemble_get_state(gc_ensemble* e, int i, void* host);
+/* Device pointer of member i's current state (valid until the next step). */
+gc_status gc_ensemble_state_ptr(gc_ensemble* e, int i, void** dev);
+/* Total sites over all members. */
+gc_status gc_ensemble_sites(gc_ensemble* e, size_t* sites);
+/* Advances every member n_steps RK4 steps; use_graph != 0 replays a captured hipGraph of
+ * 20 steps (the first call for a shape captures it). */
+gc_status gc_ensemble_step(gc_ensemble* e, int n_steps, int use_graph);
+/* Tile shapes (OX x OY) of the step kernel: query, force, or time them all (states are
+ * preserved) and keep the fastest; ms (optional) receives ms per step of each shape. */
+gc_status gc_ensemble_config_shape(gc_ensemble* e, int id, int* ox, int* oy);
+gc_status gc_ensemble_set_config(gc_ensemble* e, int id);
+gc_status gc_ensemble_tune(gc_ensemble* e, int* best, float* ms);
+/* Per-member total mass (integrate) and free energy (evaluate with the Cahn-Hilliard
+ * density), FP64 reductions; either output may be NULL. */
+gc_status gc_ensemble_diagnostics(gc_ensemble* e, int method, double* mass, double* F);
+
 #ifdef __cplusplus
 }
 #endif
