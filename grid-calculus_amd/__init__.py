@@ -146,6 +146,17 @@ def lib():
         "gc_ensemble_set_config": ([vp, C.c_int], st),
         "gc_ensemble_tune": ([vp, ip, C.POINTER(C.c_float)], st),
         "gc_ensemble_diagnostics": ([vp, C.c_int, dp, dp], st),
+        "gc_cnn_param_count": ([], C.c_int),
+        "gc_cnn_create": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], st),
+        "gc_cnn_destroy": ([vp], st),
+        "gc_cnn_set_params": ([vp, vp, C.c_double], st),
+        "gc_cnn_get_params": ([vp, vp], st),
+        "gc_cnn_get_grads": ([vp, vp], st),
+        "gc_cnn_forward": ([vp, C.c_int, vp, C.c_int, vp], st),
+        "gc_cnn_loss_grad": ([vp, C.c_int, vp, vp, ip, C.c_int, dp], st),
+        "gc_cnn_adam": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], st),
+        "gc_cnn_predict": ([vp, C.c_int, vp, vp, C.c_int], st),
+        "gc_cast": ([vp, C.c_int, C.c_int, sz, vp, vp], st),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -495,6 +506,77 @@ class Ensemble:
         if self._h:
             lib().gc_ensemble_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- learned corrector (6-layer periodic CNN) ----------------------------------------------
+
+N_CNN_PARAMS = 37601
+
+
+def cast(ctx, src, dst):
+    """Element-wise dtype conversion between two device arrays of equal size."""
+    n = int(np.prod(src.shape))
+    _check(lib().gc_cast(ctx.handle, src.gc_dtype, dst.gc_dtype, n, src.ptr, dst.ptr))
+
+
+class Corrector:
+    """gc_cnn: the 2-D Cahn-Hilliard corrector network on H x W images (H % 8 == 0,
+    W % 32 == 0), batches up to batch_max."""
+
+    def __init__(self, ctx, H, W, batch_max):
+        self.ctx, self.H, self.W, self.batch_max = ctx, H, W, batch_max
+        self._h = C.c_void_p()
+        _check(lib().gc_cnn_create(ctx.handle, H, W, batch_max, C.byref(self._h)))
+        self.out = DeviceArray((batch_max, H, W), np.float32)
+
+    def set_params(self, p, scale):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == N_CNN_PARAMS
+        _check(lib().gc_cnn_set_params(self._h, p.ctypes.data_as(C.c_void_p), scale))
+
+    def params(self):
+        p = np.empty(N_CNN_PARAMS, np.float32)
+        _check(lib().gc_cnn_get_params(self._h, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def grads(self):
+        g = np.empty(N_CNN_PARAMS, np.float32)
+        _check(lib().gc_cnn_get_grads(self._h, g.ctypes.data_as(C.c_void_p)))
+        return g
+
+    def forward(self, c, B):
+        """Raw network output for the first B images of device array c -> numpy (B, H, W)."""
+        _check(lib().gc_cnn_forward(self._h, c.gc_dtype, c.ptr, B, self.out.ptr))
+        return self.out.numpy(self.ctx)[:B]
+
+    def loss_grad(self, c, target, idx=None, B=None):
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, dtype=np.int32)
+            B = idx.size
+            ip = idx.ctypes.data_as(C.POINTER(C.c_int))
+        else:
+            ip = None
+        loss = C.c_double()
+        _check(lib().gc_cnn_loss_grad(self._h, c.gc_dtype, c.ptr, target.ptr, ip, B, C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().gc_cnn_adam(self._h, lr, beta1, beta2, eps))
+
+    def predict(self, c, c_new, B=1):
+        _check(lib().gc_cnn_predict(self._h, c.gc_dtype, c.ptr, c_new.ptr, B))
+
+    def close(self):
+        if self._h:
+            lib().gc_cnn_destroy(self._h)
+            self._h = C.c_void_p()
+        self.out.free()
 
     def __del__(self):
         try:

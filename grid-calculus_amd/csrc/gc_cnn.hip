@@ -1,0 +1,793 @@
+// gc_cnn.hip -- the learned corrector for 2-D Cahn-Hilliard: a 6-layer 3x3 periodic CNN
+// (1 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 channels, GELU, residual output) trained and run
+// on the device.
+//
+//   a1 = gelu(conv(x, W1) + b1),  a_l = gelu(conv(a_{l-1}, W_l) + b_l) (l = 2..5),
+//   out = conv(a5, W6) + b6,      prediction = c + scale * (out - mean(out))
+//
+// conv(x, W)[y, x] = sum_tap W[tap] x[(y+dy) mod H, (x+dx) mod W], tap = 3(dy+1) + (dx+1).
+// Weights and activations are BF16; every accumulation is FP32.  The four 32 -> 32 layers
+// carry 98 % of the FLOPs and run as implicit GEMMs on the matrix cores
+// (v_mfma_f32_32x32x16_bf16): M = 32 sites of a row, N = 32 output channels, K = 9 taps x
+// 32 input channels (18 MFMAs per 32 x 32 output tile).  Backward-data is the same
+// convolution with the taps flipped and the weight matrix transposed; backward-weights
+// is a GEMM over sites whose operands are staged channel-major in LDS.  The 1 -> 32 and
+// 32 -> 1 layers (288 MACs per site) run on the vector ALUs.
+//
+// Parameters are one FP32 vector (37601 values): W1[co][tap], b1, {W_l[co][tap][ci], b_l}
+// for l = 2..5, W6[tap][ci], b6.  Activations are NHWC (32 channels contiguous per site).
+#include "gc_common.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace gc {
+gc_status integrate_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
+                        const void* in, double* sum_out);
+gc_status free_energy_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, double kappa,
+                          int method, const void* psi, double* sum_out);
+}  // namespace gc
+
+namespace {
+
+using namespace gc;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kC = 32;            // channels
+constexpr int kMidW = 9 * 32 * 32;  // weights of a 32 -> 32 layer
+constexpr int kOffW1 = 0, kOffB1 = 288, kOffMid = 320, kMidStride = kMidW + 32;
+constexpr int kOffW6 = kOffMid + 4 * kMidStride, kOffB6 = kOffW6 + 288;
+constexpr int kNParams = kOffB6 + 1;  // 37601
+static_assert(kNParams == 37601, "parameter count");
+
+constexpr int kTR = 8, kTC = 32, kSP = 40;  // conv tile rows, cols, LDS site stride (bf16)
+
+__device__ __forceinline__ float gelu_f(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad_f(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+__device__ __forceinline__ float bf(float v) { return (float)(__bf16)v; }
+
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// ---- layer 1 (1 -> 32), vector ALU -------------------------------------------------------
+// One thread per site; the state value is rounded to BF16 as the network input.
+template <class TS>
+__global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const int* __restrict__ idx,
+                                               const __bf16* __restrict__ w1b,
+                                               const float* __restrict__ b1, float* __restrict__ z1,
+                                               __bf16* __restrict__ a1, int H, int W, int B) {
+  const int HW = H * W;
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= (long)B * HW) return;
+  const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+  const TS* img = c + (size_t)(idx ? idx[b] : b) * HW;
+  float xv[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+    xv[t] = bf((float)img[wrap1(y + t / 3 - 1, H) * W + wrap1(x + t % 3 - 1, W)]);
+#pragma unroll
+  for (int co = 0; co < kC; co += 8) {
+    float zz[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) acc = fmaf((float)w1b[(co + u) * 9 + t], xv[t], acc);
+      zz[u] = acc + b1[co + u];
+    }
+    bf16x8 av;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (z1) z1[(size_t)s * kC + co + u] = zz[u];
+      av[u] = (__bf16)gelu_f(zz[u]);
+    }
+    *(bf16x8*)(a1 + (size_t)s * kC + co) = av;
+  }
+}
+
+// ---- 32 -> 32 layers, matrix cores --------------------------------------------------------
+// MODE 0 (forward): out = bf16(gelu(acc + bias)), zout = acc + bias (optional).
+// MODE 1 (backward-data): out = bf16(acc * gelu'(zprev)).
+// Block: 256 threads, one kTR x kTC site tile of one image; wave w computes rows 2w, 2w+1.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
+                                                  const __bf16* __restrict__ wt,
+                                                  const float* __restrict__ bias,
+                                                  const float* __restrict__ zprev,
+                                                  float* __restrict__ zout,
+                                                  __bf16* __restrict__ out, int H, int W,
+                                                  int tiles_x, int tiles_y) {
+  __shared__ __attribute__((aligned(16))) __bf16 xs[(kTR + 2) * (kTC + 2) * kSP];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tx = blockIdx.x % tiles_x;
+  const int r1 = blockIdx.x / tiles_x;
+  const int ty = r1 % tiles_y, b = r1 / tiles_y;
+  const int x0 = tx * kTC, y0 = ty * kTR;
+  const size_t img = (size_t)b * H * W;
+  constexpr int NS = (kTR + 2) * (kTC + 2);
+  for (int q = t; q < NS * 4; q += 256) {
+    const int site = q >> 2, ch = (q & 3) * 8;
+    const int ly = site / (kTC + 2), lx = site % (kTC + 2);
+    const int gy = wrap1(y0 + ly - 1, H), gx = wrap1(x0 + lx - 1, W);
+    *(bf16x8*)(xs + site * kSP + ch) = *(const bf16x8*)(in + (img + (size_t)gy * W + gx) * kC + ch);
+  }
+  const int j = lane & 31, kh = (lane >> 5) * 8;
+  bf16x8 wreg[18];
+#pragma unroll
+  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r = wave * 2 + rr;
+    f32x16 acc = {};
+#pragma unroll
+    for (int kc = 0; kc < 18; ++kc) {
+      const int tap = kc >> 1, dy = tap / 3 - 1, dx = tap % 3 - 1;
+      const int site = (r + 1 + dy) * (kTC + 2) + (j + 1 + dx);
+      const bf16x8 a = *(const bf16x8*)(xs + site * kSP + (kc & 1) * 16 + kh);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wreg[kc], acc, 0, 0, 0);
+    }
+    const size_t row = img + (size_t)(y0 + r) * W + x0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int xi = (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+      const size_t o = (row + xi) * kC + j;
+      float v = acc[q];
+      if (MODE == 0) {
+        v += bias[j];
+        if (zout) zout[o] = v;
+        out[o] = (__bf16)gelu_f(v);
+      } else {
+        out[o] = (__bf16)(v * gelu_grad_f(zprev[o]));
+      }
+    }
+  }
+}
+
+// Backward-weights of a 32 -> 32 layer: dW[co][tap][ci] = sum_s dz[s][co] x[s+off][ci],
+// db[co] = sum_s dz[s][co].  Block: one tile; operands staged channel-major in LDS (x in
+// three copies shifted by dx so every 16-byte read is aligned).  Per-block partial sums go
+// to part[block][9248] (reduced later in a fixed order).
+__global__ __launch_bounds__(256) void k_conv_mid_bwdw(const __bf16* __restrict__ dz,
+                                                       const __bf16* __restrict__ x,
+                                                       float* __restrict__ part, int H, int W,
+                                                       int tiles_x, int tiles_y, int ntiles) {
+  constexpr int CS = kTC + 8;  // column stride of the channel-major planes (bf16)
+  __shared__ __attribute__((aligned(16))) __bf16 dzT[kC * kTR * CS];
+  __shared__ __attribute__((aligned(16))) __bf16 xT[3][kC * (kTR + 2) * CS];
+  __shared__ float red[4][kC * 32];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int j = lane & 31, kh = (lane >> 5) * 8;
+  f32x16 acc[9];
+#pragma unroll
+  for (int tp = 0; tp < 9; ++tp) acc[tp] = f32x16{};
+  float dbacc = 0.0f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % tiles_x;
+    const int r1 = tile / tiles_x;
+    const int ty = r1 % tiles_y, b = r1 / tiles_y;
+    const int x0 = tx * kTC, y0 = ty * kTR;
+    const size_t img = (size_t)b * H * W;
+    __syncthreads();
+    for (int q = t; q < kTR * kTC * 4; q += 256) {  // dz tile, transposed
+      const int site = q >> 2, ch = (q & 3) * 8;
+      const int ly = site / kTC, lx = site % kTC;
+      const bf16x8 v = *(const bf16x8*)(dz + (img + (size_t)(y0 + ly) * W + x0 + lx) * kC + ch);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) dzT[(ch + u) * kTR * CS + ly * CS + lx] = v[u];
+    }
+    for (int q = t; q < (kTR + 2) * (kTC + 2) * 4; q += 256) {  // x tile + halo, 3 shifts
+      const int site = q >> 2, ch = (q & 3) * 8;
+      const int ly = site / (kTC + 2), lx = site % (kTC + 2);  // lx: column x0 + lx - 1
+      const int gy = wrap1(y0 + ly - 1, H), gx = wrap1(x0 + lx - 1, W);
+      const bf16x8 v = *(const bf16x8*)(x + (img + (size_t)gy * W + gx) * kC + ch);
+#pragma unroll
+      for (int sh = 0; sh < 3; ++sh) {  // copy sh holds column (x0 + c + sh - 1) at c
+        const int c = lx - sh;
+        if (c >= 0 && c < kTC) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) xT[sh][(ch + u) * (kTR + 2) * CS + ly * CS + c] = v[u];
+        }
+      }
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 2; ++rr) {
+      const int r = wave * 2 + rr;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // 16-site chunks of the row
+        const bf16x8 a = *(const bf16x8*)(dzT + j * kTR * CS + r * CS + h * 16 + kh);
+#pragma unroll
+        for (int tp = 0; tp < 9; ++tp) {
+          const int dy = tp / 3 - 1, sh = tp % 3;  // dx = sh - 1
+          const bf16x8 bb =
+              *(const bf16x8*)(xT[sh] + j * (kTR + 2) * CS + (r + 1 + dy) * CS + h * 16 + kh);
+          acc[tp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[tp], 0, 0, 0);
+        }
+      }
+    }
+    // bias gradient: thread (co = t % 32, row = t / 32)
+    {
+      const int co = t & 31, ly = t >> 5;
+      float s = 0.0f;
+      for (int lx = 0; lx < kTC; ++lx) s += (float)dzT[co * kTR * CS + ly * CS + lx];
+      dbacc += s;
+    }
+  }
+  // reduce the four waves' accumulators tap by tap: acc[tp][q] -> D[co = row i][ci = j]
+  float* out = part + (size_t)blockIdx.x * (kMidW + kC);
+  for (int tp = 0; tp < 9; ++tp) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int co = (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+      red[wave][co * 32 + j] = acc[tp][q];
+    }
+    __syncthreads();
+    for (int e = t; e < kC * 32; e += 256) {
+      const int co = e >> 5, ci = e & 31;
+      out[co * 288 + tp * 32 + ci] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    }
+  }
+  __syncthreads();
+  red[0][t] = dbacc;
+  __syncthreads();
+  if (t < kC) {
+    float s = 0.0f;
+    for (int ly = 0; ly < kTR; ++ly) s += red[0][ly * 32 + t];
+    out[kMidW + t] = s;
+  }
+}
+
+// ---- layer 6 (32 -> 1), vector ALU ---------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
+                                               const __bf16* __restrict__ w6b,
+                                               const float* __restrict__ b6,
+                                               float* __restrict__ out, int H, int W, int B) {
+  const int HW = H * W;
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= (long)B * HW) return;
+  const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+  const size_t img = (size_t)b * HW;
+  float acc = 0.0f;
+  for (int tp = 0; tp < 9; ++tp) {
+    const size_t n = img + (size_t)wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W);
+#pragma unroll
+    for (int ch = 0; ch < kC; ch += 8) {
+      const bf16x8 v = *(const bf16x8*)(a5 + n * kC + ch);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = fmaf((float)w6b[tp * kC + ch + u], (float)v[u], acc);
+    }
+  }
+  out[s] = acc + b6[0];
+}
+
+// Loss on the normalised increment: d = out - (target - c)/scale; L = mean(d^2);
+// dout = 2 d / N.  Per-block partial sums of d^2 in double.
+template <class TS>
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ out, const TS* __restrict__ c,
+                                              const TS* __restrict__ target,
+                                              const int* __restrict__ idx, float inv_scale,
+                                              float* __restrict__ dout, double* __restrict__ part,
+                                              int HW, int B) {
+  __shared__ double sh[4];
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  double d2 = 0.0;
+  if (s < (long)B * HW) {
+    const int b = (int)(s / HW), p = (int)(s % HW);
+    const size_t src = (size_t)(idx ? idx[b] : b) * HW + p;
+    const float delta = (float)((double)target[src] - (double)c[src]) * inv_scale;
+    const float d = out[s] - delta;
+    dout[s] = 2.0f * d / (float)((long)B * HW);
+    d2 = (double)d * (double)d;
+  }
+  for (int st = 32; st >= 1; st >>= 1) d2 += __shfl_xor(d2, st);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// Backward-data of layer 6 fused with gelu': dz5[s][ci] = gelu'(z5) sum_tap W6[tap][ci]
+// dout[s - off(tap)].
+__global__ __launch_bounds__(256) void k_conv6_bwdd(const float* __restrict__ dout,
+                                                    const __bf16* __restrict__ w6b,
+                                                    const float* __restrict__ z5,
+                                                    __bf16* __restrict__ dz5, int H, int W, int B) {
+  const int HW = H * W;
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= (long)B * HW) return;
+  const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+  const size_t img = (size_t)b * HW;
+  float dv[9];
+#pragma unroll
+  for (int tp = 0; tp < 9; ++tp)
+    dv[tp] = dout[img + (size_t)wrap1(y - (tp / 3 - 1), H) * W + wrap1(x - (tp % 3 - 1), W)];
+#pragma unroll
+  for (int ch = 0; ch < kC; ch += 8) {
+    bf16x8 o;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int tp = 0; tp < 9; ++tp) acc = fmaf((float)w6b[tp * kC + ch + u], dv[tp], acc);
+      o[u] = (__bf16)(acc * gelu_grad_f(z5[(size_t)s * kC + ch + u]));
+    }
+    *(bf16x8*)(dz5 + (size_t)s * kC + ch) = o;
+  }
+}
+
+// Backward-weights of layer 6: thread e < 288 owns W6[tap = e/32][ci = e%32], e = 288 owns
+// b6.  A block sums over a contiguous range of sites.
+__global__ __launch_bounds__(320) void k_conv6_bwdw(const float* __restrict__ dout,
+                                                    const __bf16* __restrict__ a5,
+                                                    float* __restrict__ part, int H, int W,
+                                                    long nsites, long per_block) {
+  const int e = threadIdx.x;
+  const int tp = e / 32, ci = e % 32;
+  const int HW = H * W;
+  float acc = 0.0f;
+  const long s0 = (long)blockIdx.x * per_block;
+  const long s1 = min(nsites, s0 + per_block);
+  for (long s = s0; s < s1; ++s) {
+    const float d = dout[s];
+    if (e < 288) {
+      const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+      const size_t n = (size_t)b * HW + (size_t)wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W);
+      acc = fmaf(d, (float)a5[n * kC + ci], acc);
+    } else if (e == 288) {
+      acc += d;
+    }
+  }
+  if (e < 289) part[(size_t)blockIdx.x * 289 + e] = acc;
+}
+
+// Backward-weights of layer 1: thread e < 288 owns W1[co = e/9][tap = e%9], 288 <= e < 320
+// owns b1[e-288].
+template <class TS>
+__global__ __launch_bounds__(320) void k_conv1_bwdw(const __bf16* __restrict__ dz1,
+                                                    const TS* __restrict__ c,
+                                                    const int* __restrict__ idx,
+                                                    float* __restrict__ part, int H, int W,
+                                                    long nsites, long per_block) {
+  const int e = threadIdx.x;
+  const int co = e < 288 ? e / 9 : e - 288, tp = e % 9;
+  const int HW = H * W;
+  float acc = 0.0f;
+  const long s0 = (long)blockIdx.x * per_block;
+  const long s1 = min(nsites, s0 + per_block);
+  for (long s = s0; s < s1; ++s) {
+    const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+    const float d = (float)dz1[(size_t)s * kC + co];
+    if (e < 288) {
+      const TS* img = c + (size_t)(idx ? idx[b] : b) * HW;
+      const float xv = bf((float)img[wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W)]);
+      acc = fmaf(d, xv, acc);
+    } else {
+      acc += d;
+    }
+  }
+  part[(size_t)blockIdx.x * 320 + e] = acc;
+}
+
+// out[i] = sum over p of part[p][i], p in order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, int nparts,
+                                                      int width, float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= width) return;
+  float s = 0.0f;
+  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * width + i];
+  out[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, int n,
+                                              float lr, float b1, float b2, float eps, float bc1,
+                                              float bc2) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// FP32 master parameters -> BF16 working copies: W1, W6, the 32 -> 32 matrices [co][k] and
+// their flipped transposes [ci][tap'*32 + co] = W[co][(8 - tap')*32 + ci] (backward-data).
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf16* __restrict__ w1b,
+                                              __bf16* __restrict__ w6b, __bf16* __restrict__ wmid,
+                                              __bf16* __restrict__ wmidT) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < 288) {
+    w1b[i] = (__bf16)p[kOffW1 + i];
+    w6b[i] = (__bf16)p[kOffW6 + i];
+  }
+  if (i < 4 * kMidW) {
+    const int l = i / kMidW, e = i % kMidW;
+    const float* W = p + kOffMid + l * kMidStride;
+    wmid[i] = (__bf16)W[e];
+    const int ci = e / 288, k = e % 288, tpp = k / 32, co = k % 32;
+    wmidT[i] = (__bf16)W[co * 288 + (8 - tpp) * 32 + ci];
+  }
+}
+
+// Residual update with the mean of the increment removed (mass conservation):
+// c_new = c + scale * (out - sum(out)/N).
+template <class TS>
+__global__ __launch_bounds__(256) void k_apply(const TS* __restrict__ c, TS* __restrict__ cn,
+                                               const float* __restrict__ out,
+                                               const double* __restrict__ sum_out, double scale,
+                                               long n) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const double mean = sum_out[0] / (double)n;
+  cn[s] = (TS)((double)c[s] + scale * ((double)out[s] - mean));
+}
+
+__global__ __launch_bounds__(256) void k_sum_f32(const float* __restrict__ x, long n,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += x[i];
+  for (int st = 32; st >= 1; st >>= 1) s += __shfl_xor(s, st);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void k_sum_final(const double* __restrict__ part, int n, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    out[0] = s;
+  }
+}
+
+template <class TS, class TD>
+__global__ __launch_bounds__(256) void k_cast(const TS* __restrict__ a, TD* __restrict__ b, long n) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s < n) b[s] = (TD)a[s];
+}
+
+}  // namespace
+
+struct gc_cnn {
+  gc_context* ctx = nullptr;
+  int H = 0, W = 0, Bmax = 0;
+  double scale = 1.0;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  __bf16 *w1b = nullptr, *w6b = nullptr, *wmid = nullptr, *wmidT = nullptr;
+  float* z[5] = {};
+  __bf16* a[5] = {};
+  __bf16 *dzA = nullptr, *dzB = nullptr;
+  float *out = nullptr, *dout = nullptr;
+  float* part = nullptr;
+  size_t part_floats = 0;
+  double* dpart = nullptr;
+  int* idx = nullptr;
+  int adam_t = 0;
+  bool packed = false;
+};
+
+namespace {
+
+constexpr int kPartBlocks = 512;  // blocks of the weight-gradient kernels
+
+gc_status cnn_alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return set_error(GC_ERR_NOMEM, "gridcalc: CNN allocation of %zu bytes failed", bytes);
+  }
+  return GC_OK;
+}
+
+gc_status pack(gc_cnn* n) {
+  hipLaunchKernelGGL(k_pack, dim3((4 * kMidW + 255) / 256), dim3(256), 0, n->ctx->stream, n->p,
+                     n->w1b, n->w6b, n->wmid, n->wmidT);
+  GC_LAUNCHED();
+  n->packed = true;
+  return GC_OK;
+}
+
+gc_status check_cnn(gc_cnn* n, int B) {
+  if (!n) return set_error(GC_ERR_INVALID, "gridcalc: null CNN");
+  GC_TRY(check_ctx(n->ctx));
+  if (B < 1 || B > n->Bmax)
+    return set_error(GC_ERR_INVALID, "gridcalc: batch %d outside [1, %d]", B, n->Bmax);
+  return GC_OK;
+}
+
+gc_status upload_idx(gc_cnn* n, const int* idx, int B) {
+  if (!idx) return GC_OK;
+  GC_HIP(hipMemcpyAsync(n->idx, idx, B * sizeof(int), hipMemcpyHostToDevice, n->ctx->stream));
+  return GC_OK;
+}
+
+// Forward pass on B images (c[idx[b]] or c[b]); train keeps the pre-activations.
+template <class TS>
+gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
+  if (!n->packed) GC_TRY(pack(n));
+  const long ns = (long)B * n->H * n->W;
+  const unsigned g1 = (unsigned)((ns + 255) / 256);
+  hipStream_t st = n->ctx->stream;
+  hipLaunchKernelGGL((k_conv1<TS>), dim3(g1), dim3(256), 0, st, c, d_idx, n->w1b, n->p + kOffB1,
+                     train ? n->z[0] : nullptr, n->a[0], n->H, n->W, B);
+  GC_LAUNCHED();
+  const int tx = n->W / kTC, ty = n->H / kTR;
+  for (int l = 0; l < 4; ++l) {
+    hipLaunchKernelGGL((k_conv_mid<0>), dim3((unsigned)(tx * ty * B)), dim3(256), 0, st, n->a[l],
+                       n->wmid + (size_t)l * kMidW, n->p + kOffMid + l * kMidStride + kMidW,
+                       nullptr, train ? n->z[l + 1] : nullptr, n->a[l + 1], n->H, n->W, tx, ty);
+    GC_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_conv6, dim3(g1), dim3(256), 0, st, n->a[4], n->w6b, n->p + kOffB6, n->out,
+                     n->H, n->W, B);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// Backward pass from dout; gradients into n->g.
+template <class TS>
+gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
+  hipStream_t st = n->ctx->stream;
+  const long ns = (long)B * n->H * n->W;
+  const unsigned g1 = (unsigned)((ns + 255) / 256);
+  const long per = (ns + kPartBlocks - 1) / kPartBlocks;
+  // layer 6
+  hipLaunchKernelGGL(k_conv6_bwdw, dim3(kPartBlocks), dim3(320), 0, st, n->dout, n->a[4], n->part,
+                     n->H, n->W, ns, per);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_parts, dim3((289 + 255) / 256), dim3(256), 0, st, n->part,
+                     kPartBlocks, 289, n->g + kOffW6);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_conv6_bwdd, dim3(g1), dim3(256), 0, st, n->dout, n->w6b, n->z[4], n->dzA,
+                     n->H, n->W, B);
+  GC_LAUNCHED();
+  const int tx = n->W / kTC, ty = n->H / kTR, ntiles = tx * ty * B;
+  __bf16* dz = n->dzA;
+  __bf16* dzn = n->dzB;
+  for (int l = 3; l >= 0; --l) {  // layer l+2 in 1-based numbering: a[l] -> z[l+1]
+    hipLaunchKernelGGL(k_conv_mid_bwdw, dim3(kPartBlocks), dim3(256), 0, st, dz, n->a[l],
+                       n->part, n->H, n->W, tx, ty, ntiles);
+    GC_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_parts, dim3((kMidStride + 255) / 256), dim3(256), 0, st, n->part,
+                       kPartBlocks, kMidStride, n->g + kOffMid + l * kMidStride);
+    GC_LAUNCHED();
+    hipLaunchKernelGGL((k_conv_mid<1>), dim3((unsigned)ntiles), dim3(256), 0, st, dz,
+                       n->wmidT + (size_t)l * kMidW, nullptr, n->z[l], nullptr, dzn, n->H, n->W,
+                       tx, ty);
+    GC_LAUNCHED();
+    __bf16* tmp = dz;
+    dz = dzn;
+    dzn = tmp;
+  }
+  hipLaunchKernelGGL((k_conv1_bwdw<TS>), dim3(kPartBlocks), dim3(320), 0, st, dz, c, d_idx,
+                     n->part, n->H, n->W, ns, per);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_parts, dim3((320 + 255) / 256), dim3(256), 0, st, n->part,
+                     kPartBlocks, 320, n->g + kOffW1);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+template <class TS>
+gc_status loss_and_dout(gc_cnn* n, const TS* c, const TS* target, const int* d_idx, int B,
+                        double* loss) {
+  const long ns = (long)B * n->H * n->W;
+  const unsigned g1 = (unsigned)((ns + 255) / 256);
+  hipLaunchKernelGGL((k_loss<TS>), dim3(g1), dim3(256), 0, n->ctx->stream, n->out, c, target,
+                     d_idx, (float)(1.0 / n->scale), n->dout, n->dpart, n->H * n->W, B);
+  GC_LAUNCHED();
+  if (loss) {
+    std::vector<double> h(g1);
+    GC_HIP(hipMemcpyAsync(h.data(), n->dpart, g1 * sizeof(double), hipMemcpyDeviceToHost,
+                          n->ctx->stream));
+    GC_HIP(hipStreamSynchronize(n->ctx->stream));
+    double s = 0.0;
+    for (double x : h) s += x;
+    *loss = s / (double)ns;
+  }
+  return GC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** out) {
+  GC_TRY(check_ctx(ctx));
+  if (!out) return set_error(GC_ERR_INVALID, "gridcalc: null CNN pointer");
+  if (H < kTR || W < kTC || H % kTR || W % kTC || batch_max < 1)
+    return set_error(GC_ERR_UNSUPPORTED,
+                     "gridcalc: the corrector needs H a multiple of %d and W a multiple of %d",
+                     kTR, kTC);
+  *out = nullptr;
+  gc_cnn* n = new gc_cnn();
+  n->ctx = ctx;
+  n->H = H;
+  n->W = W;
+  n->Bmax = batch_max;
+  const size_t ns = (size_t)batch_max * H * W;
+  const size_t np4 = kNParams * sizeof(float);
+  gc_status st = GC_OK;
+  auto A = [&](void** p, size_t b) {
+    if (st == GC_OK) st = cnn_alloc(p, b);
+  };
+  A((void**)&n->p, np4);
+  A((void**)&n->g, np4);
+  A((void**)&n->m, np4);
+  A((void**)&n->v, np4);
+  A((void**)&n->w1b, 288 * 2);
+  A((void**)&n->w6b, 288 * 2);
+  A((void**)&n->wmid, 4 * kMidW * 2);
+  A((void**)&n->wmidT, 4 * kMidW * 2);
+  for (int l = 0; l < 5; ++l) {
+    A((void**)&n->z[l], ns * kC * sizeof(float));
+    A((void**)&n->a[l], ns * kC * 2);
+  }
+  A((void**)&n->dzA, ns * kC * 2);
+  A((void**)&n->dzB, ns * kC * 2);
+  A((void**)&n->out, ns * sizeof(float));
+  A((void**)&n->dout, ns * sizeof(float));
+  n->part_floats = (size_t)kPartBlocks * kMidStride;
+  A((void**)&n->part, n->part_floats * sizeof(float));
+  A((void**)&n->dpart, ((ns + 255) / 256 + 1024) * sizeof(double));
+  A((void**)&n->idx, batch_max * sizeof(int));
+  if (st == GC_OK && (hipMemsetAsync(n->p, 0, np4, ctx->stream) != hipSuccess ||
+                      hipMemsetAsync(n->m, 0, np4, ctx->stream) != hipSuccess ||
+                      hipMemsetAsync(n->v, 0, np4, ctx->stream) != hipSuccess))
+    st = set_error(GC_ERR_DEVICE, "gridcalc: CNN initialisation failed");
+  if (st != GC_OK) {
+    gc_cnn_destroy(n);
+    return st;
+  }
+  *out = n;
+  return GC_OK;
+}
+
+gc_status gc_cnn_destroy(gc_cnn* n) {
+  if (!n) return GC_OK;
+  if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+  void* ptrs[] = {n->p, n->g, n->m, n->v, n->w1b, n->w6b, n->wmid, n->wmidT, n->dzA, n->dzB,
+                  n->out, n->dout, n->part, n->dpart, n->idx};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int l = 0; l < 5; ++l) {
+    if (n->z[l]) (void)hipFree(n->z[l]);
+    if (n->a[l]) (void)hipFree(n->a[l]);
+  }
+  delete n;
+  return GC_OK;
+}
+
+int gc_cnn_param_count(void) { return kNParams; }
+
+gc_status gc_cnn_set_params(gc_cnn* n, const float* host, double scale) {
+  GC_TRY(check_cnn(n, 1));
+  if (!host) return set_error(GC_ERR_INVALID, "gridcalc: null parameters");
+  if (!(scale > 0.0)) return set_error(GC_ERR_INVALID, "gridcalc: output scale must be > 0");
+  GC_HIP(hipMemcpyAsync(n->p, host, kNParams * sizeof(float), hipMemcpyHostToDevice,
+                        n->ctx->stream));
+  GC_HIP(hipMemsetAsync(n->m, 0, kNParams * sizeof(float), n->ctx->stream));
+  GC_HIP(hipMemsetAsync(n->v, 0, kNParams * sizeof(float), n->ctx->stream));
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  n->scale = scale;
+  n->adam_t = 0;
+  n->packed = false;
+  return GC_OK;
+}
+
+gc_status gc_cnn_get_params(gc_cnn* n, float* host) {
+  GC_TRY(check_cnn(n, 1));
+  if (!host) return set_error(GC_ERR_INVALID, "gridcalc: null parameters");
+  GC_HIP(hipMemcpyAsync(host, n->p, kNParams * sizeof(float), hipMemcpyDeviceToHost,
+                        n->ctx->stream));
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  return GC_OK;
+}
+
+gc_status gc_cnn_get_grads(gc_cnn* n, float* host) {
+  GC_TRY(check_cnn(n, 1));
+  if (!host) return set_error(GC_ERR_INVALID, "gridcalc: null gradient buffer");
+  GC_HIP(hipMemcpyAsync(host, n->g, kNParams * sizeof(float), hipMemcpyDeviceToHost,
+                        n->ctx->stream));
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  return GC_OK;
+}
+
+gc_status gc_cnn_forward(gc_cnn* n, gc_dtype dt, const void* c, int B, float* out_dev) {
+  GC_TRY(check_cnn(n, B));
+  GC_TRY(check_dtype(dt));
+  if (!c) return set_error(GC_ERR_INVALID, "gridcalc: null input");
+  if (dt == GC_F64) GC_TRY(forward<double>(n, (const double*)c, nullptr, B, false));
+  else GC_TRY(forward<float>(n, (const float*)c, nullptr, B, false));
+  if (out_dev)
+    GC_HIP(hipMemcpyAsync(out_dev, n->out, (size_t)B * n->H * n->W * sizeof(float),
+                          hipMemcpyDeviceToDevice, n->ctx->stream));
+  return GC_OK;
+}
+
+gc_status gc_cnn_loss_grad(gc_cnn* n, gc_dtype dt, const void* c, const void* target,
+                           const int* idx, int B, double* loss) {
+  GC_TRY(check_cnn(n, B));
+  GC_TRY(check_dtype(dt));
+  if (!c || !target) return set_error(GC_ERR_INVALID, "gridcalc: null data");
+  GC_TRY(upload_idx(n, idx, B));
+  const int* d_idx = idx ? n->idx : nullptr;
+  if (dt == GC_F64) {
+    GC_TRY(forward<double>(n, (const double*)c, d_idx, B, true));
+    GC_TRY(loss_and_dout<double>(n, (const double*)c, (const double*)target, d_idx, B, loss));
+    return backward<double>(n, (const double*)c, d_idx, B);
+  }
+  GC_TRY(forward<float>(n, (const float*)c, d_idx, B, true));
+  GC_TRY(loss_and_dout<float>(n, (const float*)c, (const float*)target, d_idx, B, loss));
+  return backward<float>(n, (const float*)c, d_idx, B);
+}
+
+gc_status gc_cnn_adam(gc_cnn* n, double lr, double beta1, double beta2, double eps) {
+  GC_TRY(check_cnn(n, 1));
+  n->adam_t += 1;
+  const float bc1 = (float)(1.0 - std::pow(beta1, n->adam_t));
+  const float bc2 = (float)(1.0 - std::pow(beta2, n->adam_t));
+  hipLaunchKernelGGL(k_adam, dim3((kNParams + 255) / 256), dim3(256), 0, n->ctx->stream, n->p,
+                     n->g, n->m, n->v, kNParams, (float)lr, (float)beta1, (float)beta2, (float)eps,
+                     bc1, bc2);
+  GC_LAUNCHED();
+  return pack(n);
+}
+
+gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int B) {
+  GC_TRY(check_cnn(n, B));
+  GC_TRY(check_dtype(dt));
+  if (!c || !c_new) return set_error(GC_ERR_INVALID, "gridcalc: null state");
+  const long HW = (long)n->H * n->W;
+  hipStream_t st = n->ctx->stream;
+  if (dt == GC_F64) GC_TRY(forward<double>(n, (const double*)c, nullptr, B, false));
+  else GC_TRY(forward<float>(n, (const float*)c, nullptr, B, false));
+  for (int b = 0; b < B; ++b) {
+    const unsigned nb = 256;
+    hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, st, n->out + b * HW, HW, n->dpart);
+    GC_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart, (int)nb, n->dpart + nb);
+    GC_LAUNCHED();
+    const unsigned g1 = (unsigned)((HW + 255) / 256);
+    if (dt == GC_F64)
+      hipLaunchKernelGGL((k_apply<double>), dim3(g1), dim3(256), 0, st, (const double*)c + b * HW,
+                         (double*)c_new + b * HW, n->out + b * HW, n->dpart + nb, n->scale, HW);
+    else
+      hipLaunchKernelGGL((k_apply<float>), dim3(g1), dim3(256), 0, st, (const float*)c + b * HW,
+                         (float*)c_new + b * HW, n->out + b * HW, n->dpart + nb, n->scale, HW);
+    GC_LAUNCHED();
+  }
+  return GC_OK;
+}
+
+gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const void* src,
+                  void* dst) {
+  GC_TRY(check_ctx(ctx));
+  GC_TRY(check_dtype(from));
+  GC_TRY(check_dtype(to));
+  if (n == 0) return GC_OK;
+  if (!src || !dst) return set_error(GC_ERR_INVALID, "gridcalc: null pointer");
+  const unsigned g1 = (unsigned)((n + 255) / 256);
+  if (from == GC_F64 && to == GC_F32)
+    hipLaunchKernelGGL((k_cast<double, float>), dim3(g1), dim3(256), 0, ctx->stream,
+                       (const double*)src, (float*)dst, (long)n);
+  else if (from == GC_F32 && to == GC_F64)
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(g1), dim3(256), 0, ctx->stream,
+                       (const float*)src, (double*)dst, (long)n);
+  else
+    GC_HIP(hipMemcpyAsync(dst, src, n * dsize(from), hipMemcpyDeviceToDevice, ctx->stream));
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+}  // extern "C"

@@ -238,6 +238,43 @@ gc_status gc_ensemble_tune(gc_ensemble* e, int* best, float* ms);
  * density), FP64 reductions; either output may be NULL. */
 gc_status gc_ensemble_diagnostics(gc_ensemble* e, int method, double* mass, double* F);
 
+/* ---- learned corrector for 2-D Cahn-Hilliard ---------------------------------------
+ * No counterpart in the reference.  A 6-layer 3x3 periodic CNN (1 -> 32 x5 -> 1 channels,
+ * GELU, residual output) that maps a 2-D state c(t) to c(t + 16 dt):
+ *   c_new = c + scale * (out - mean(out))      (the mean removal conserves mass)
+ * BF16 weights and activations, FP32 accumulation; the 32 -> 32 layers run on the matrix
+ * cores.  Parameters: one FP32 vector of gc_cnn_param_count() values in the order
+ * W1[co][tap] b1 {W_l[co][tap][ci] b_l}x4 W6[tap][ci] b6 (tap = 3(dy+1) + (dx+1)).
+ * Images are H x W with H a multiple of 8 and W a multiple of 32. */
+
+typedef struct gc_cnn gc_cnn;
+
+int gc_cnn_param_count(void);
+gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** out);
+gc_status gc_cnn_destroy(gc_cnn* n);
+/* Host FP32 parameters and the output scale; resets the optimiser state. */
+gc_status gc_cnn_set_params(gc_cnn* n, const float* host, double scale);
+gc_status gc_cnn_get_params(gc_cnn* n, float* host);
+/* Gradient of the last gc_cnn_loss_grad call (host FP32, param-count values). */
+gc_status gc_cnn_get_grads(gc_cnn* n, float* host);
+/* Network output `out` (before scale and mean removal) for B device images of dtype dt;
+ * out_dev (device FP32, B*H*W) may be NULL. */
+gc_status gc_cnn_forward(gc_cnn* n, gc_dtype dt, const void* c, int B, float* out_dev);
+/* One training evaluation: images c[idx[b]] and targets target[idx[b]] (device arrays of
+ * dtype dt; idx is a host array of B indices, or NULL for 0..B-1).  Loss is the mean of
+ * (out - (target - c)/scale)^2; its gradient goes to the parameter gradient buffer.  If
+ * loss is not NULL it receives the loss (synchronous). */
+gc_status gc_cnn_loss_grad(gc_cnn* n, gc_dtype dt, const void* c, const void* target,
+                           const int* idx, int B, double* loss);
+/* Adam update of the parameters from the current gradient. */
+gc_status gc_cnn_adam(gc_cnn* n, double lr, double beta1, double beta2, double eps);
+/* c_new = c + scale*(out - mean(out)) for B device images (c_new may not alias c). */
+gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int B);
+
+/* Element-wise conversion between GC_F64 and GC_F32 device arrays. */
+gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const void* src,
+                  void* dst);
+
 #ifdef __cplusplus
 }
 #endif

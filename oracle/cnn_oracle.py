@@ -1,0 +1,154 @@
+"""FP64 numpy reference of the learned Cahn-Hilliard corrector network (test infrastructure:
+used only by tests/ to check the GPU kernels' forward pass and gradients).
+
+Network (6 layers, 3x3 periodic convolutions, 32 channels, GELU, residual):
+    a1 = gelu(conv(x, W1) + b1)            1 -> 32
+    a_l = gelu(conv(a_{l-1}, W_l) + b_l)   32 -> 32, l = 2..5
+    out = conv(a5, W6) + b6                32 -> 1
+    prediction = c + scale * out
+Parameters are one flat float vector in this order (37601 values):
+    W1[co][tap] (32*9), b1 (32), {W_l[co][tap][ci] (32*9*32), b_l (32)} for l = 2..5,
+    W6[tap][ci] (9*32), b6 (1)
+with tap = 3*(dy+1) + (dx+1) for the offset (dy, dx) in {-1,0,1}^2 and conv(x, W)[y, x] =
+sum_tap W[tap] * x[(y+dy) mod H, (x+dx) mod W] (cross-correlation, periodic padding).
+Arrays are (B, H, W) for single-channel fields and (B, H, W, C) for 32-channel activations.
+
+No code of the reference is involved: the reference has no network; this module defines it.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.special import erf
+
+C = 32
+TAPS = [(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+N_PARAMS = 32 * 9 + 32 + 4 * (32 * 9 * 32 + 32) + 9 * 32 + 1
+
+
+def split(p):
+    p = np.asarray(p, dtype=np.float64)
+    assert p.size == N_PARAMS
+    o = 0
+    W1 = p[o:o + 288].reshape(32, 9); o += 288
+    b1 = p[o:o + 32]; o += 32
+    mids = []
+    for _ in range(4):
+        W = p[o:o + 9216].reshape(32, 9, 32); o += 9216
+        b = p[o:o + 32]; o += 32
+        mids.append((W, b))
+    W6 = p[o:o + 288].reshape(9, 32); o += 288
+    b6 = p[o:o + 1]; o += 1
+    return W1, b1, mids, W6, b6
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+
+
+def shift(x, dy, dx):
+    """x[(y+dy) mod H, (x+dx) mod W] over axes 1, 2."""
+    return np.roll(np.roll(x, -dy, axis=1), -dx, axis=2)
+
+
+def forward(p, x, keep=False, q=None):
+    """x: (B, H, W) network input.  Returns out (B, H, W) and, with keep, the cache.
+    q, if given, is applied to the input, the weights and every activation (e.g.
+    bf16_round, to model a device that stores those in bfloat16)."""
+    W1, b1, mids, W6, b6 = split(p)
+    if q is not None:
+        x = q(x)
+        W1, W6 = q(W1), q(W6)
+        mids = [(q(W), b) for W, b in mids]
+    act = (lambda z: q(gelu(z))) if q is not None else gelu
+    z1 = np.zeros(x.shape + (C,))
+    for t, (dy, dx) in enumerate(TAPS):
+        z1 += shift(x, dy, dx)[..., None] * W1[:, t]
+    z1 += b1
+    zs, acts = [z1], [act(z1)]
+    for W, b in mids:
+        a = acts[-1]
+        z = np.zeros_like(a)
+        for t, (dy, dx) in enumerate(TAPS):
+            z += shift(a, dy, dx) @ W[:, t, :].T
+        z += b
+        zs.append(z)
+        acts.append(act(z))
+    out = np.zeros(x.shape)
+    for t, (dy, dx) in enumerate(TAPS):
+        out += shift(acts[-1], dy, dx) @ W6[t]
+    out += b6[0]
+    if keep:
+        return out, (x, zs, acts)
+    return out
+
+
+def backward(p, cache, dout, q=None):
+    """Gradient of sum(dout * out) with respect to the flat parameters (q as in forward;
+    applied here to the weights and to every pre-activation gradient)."""
+    W1, b1, mids, W6, b6 = split(p)
+    if q is not None:
+        W1, W6 = q(W1), q(W6)
+        mids = [(q(W), b) for W, b in mids]
+    qg = q if q is not None else (lambda a: a)
+    x, zs, acts = cache
+    g = []
+    # last layer
+    a5 = acts[-1]
+    dW6 = np.zeros_like(W6)
+    da = np.zeros_like(a5)
+    for t, (dy, dx) in enumerate(TAPS):
+        dW6[t] = np.einsum("bhw,bhwc->c", dout, shift(a5, dy, dx))
+        da += shift(dout, -dy, -dx)[..., None] * W6[t]
+    db6 = np.array([dout.sum()])
+    mid_grads = []
+    for li in range(3, -1, -1):
+        W, b = mids[li]
+        dz = qg(da * gelu_grad(zs[li + 1]))
+        a_in = acts[li]
+        dW = np.zeros_like(W)
+        da = np.zeros_like(a_in)
+        for t, (dy, dx) in enumerate(TAPS):
+            dW[:, t, :] = np.einsum("bhwo,bhwi->oi", dz, shift(a_in, dy, dx))
+            da += shift(dz, -dy, -dx) @ W[:, t, :]
+        mid_grads.insert(0, (dW, dz.sum(axis=(0, 1, 2))))
+    dz1 = qg(da * gelu_grad(zs[0]))
+    dW1 = np.zeros_like(W1)
+    for t, (dy, dx) in enumerate(TAPS):
+        dW1[:, t] = np.einsum("bhwc,bhw->c", dz1, shift(x, dy, dx))
+    db1 = dz1.sum(axis=(0, 1, 2))
+    g.append(dW1.ravel())
+    g.append(db1)
+    for dW, db in mid_grads:
+        g.append(dW.ravel())
+        g.append(db)
+    g.append(dW6.ravel())
+    g.append(db6)
+    return np.concatenate(g)
+
+
+def init_params(seed=0):
+    """He-style initialisation (uniform), small last layer."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    parts.append(rng.uniform(-1, 1, 288) * math.sqrt(6.0 / 9.0))
+    parts.append(np.zeros(32))
+    for _ in range(4):
+        parts.append(rng.uniform(-1, 1, 9216) * math.sqrt(6.0 / 288.0))
+        parts.append(np.zeros(32))
+    parts.append(rng.uniform(-1, 1, 288) * math.sqrt(6.0 / 288.0) * 0.1)
+    parts.append(np.zeros(1))
+    return np.concatenate(parts)
+
+
+def bf16_round(a):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    a32 = np.asarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)

@@ -1,0 +1,80 @@
+"""Pins the FP64 reference of the corrector network (oracle/cnn_oracle.py): its forward pass
+against torch's CPU conv2d with circular padding, and its gradients against central finite
+differences.  CPU only."""
+import importlib.util
+import math
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def cnn():
+    spec = importlib.util.spec_from_file_location("cnn_oracle", os.path.join(ROOT, "oracle", "cnn_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_param_count(cnn):
+    assert cnn.N_PARAMS == 37601
+
+
+def test_forward_matches_torch_conv2d(cnn):
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    p = cnn.init_params(3)
+    p[-1] = 0.2
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, size=(2, 7, 9))
+    W1, b1, mids, W6, b6 = cnn.split(p)
+    t = torch.tensor(x, dtype=torch.float64)[:, None]
+
+    def conv(inp, w, b):  # w: (co, ci, 3, 3)
+        return F.conv2d(F.pad(inp, (1, 1, 1, 1), mode="circular"), w, b)
+
+    w1 = torch.tensor(W1.reshape(32, 1, 3, 3))
+    a = F.gelu(conv(t, w1, torch.tensor(b1)))
+    for W, b in mids:  # W[co][tap][ci] -> (co, ci, 3, 3)
+        w = torch.tensor(np.transpose(W.reshape(32, 3, 3, 32), (0, 3, 1, 2)).copy())
+        a = F.gelu(conv(a, w, torch.tensor(b)))
+    w6 = torch.tensor(np.transpose(W6.reshape(3, 3, 32), (2, 0, 1))[None].copy())
+    out_t = conv(a, w6, torch.tensor(b6))[:, 0].numpy()
+    out = cnn.forward(p, x)
+    assert np.abs(out - out_t).max() < 1e-12
+
+
+def test_gradients_match_finite_differences(cnn):
+    p = cnn.init_params(5)
+    p[-289:-1] *= 10.0  # make the last layer matter
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, size=(1, 5, 6))
+    w = rng.normal(size=(1, 5, 6))
+    out, cache = cnn.forward(p, x, keep=True)
+    g = cnn.backward(p, cache, w)
+    idx = list(range(0, 288, 37)) + [300, 320 + 17, 320 + 9216 + 5, 4 * 9248 + 320 - 3,
+                                    4 * 9248 + 320 + 7, cnn.N_PARAMS - 1]
+    eps = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = ((w * cnn.forward(pp, x)).sum() - (w * cnn.forward(pm, x)).sum()) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_gelu_derivative(cnn):
+    z = np.linspace(-4, 4, 41)
+    fd = (cnn.gelu(z + 1e-6) - cnn.gelu(z - 1e-6)) / 2e-6
+    assert np.abs(fd - cnn.gelu_grad(z)).max() < 1e-8
+    assert abs(cnn.gelu(1.0) - 0.8413447460685429) < 1e-15  # 1 * Phi(1)
+
+
+def test_bf16_round(cnn):
+    assert cnn.bf16_round(1.0 + 2 ** -9) == 1.0           # ties to even
+    assert cnn.bf16_round(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
+    assert cnn.bf16_round(-3.140625) == -3.140625
+    assert math.isclose(cnn.bf16_round(0.1), 0.10009765625)
