@@ -14,6 +14,8 @@ prefix of the same workload.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-fp64]
   python bench.py --workload sweep [--n 512]      # operator GB/s (3-D operator sweep)
+  python bench.py --workload ensemble             # 64 uneven 2-D members, graph-replayed
+  python bench.py --workload learned [--n-ic 4]   # learned corrector vs plain RK4 to T=20
 """
 from __future__ import annotations
 
@@ -42,7 +44,10 @@ def parse():
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
-    p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble"])
+    p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
+                                                          "learned"])
+    p.add_argument("--n-ic", type=int, default=4)
+    p.add_argument("--no-learned", action="store_true")
     p.add_argument("--diag-every", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
@@ -138,6 +143,9 @@ def ch3d(a):
                 "step_kernels_ms": sms64,
                 "hbm_GBs_alg": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9,
                 "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS}
+
+    if not a.no_learned:
+        extra["learned"] = learned(a)
 
     cpu = None
     if not a.no_cpu:
@@ -281,6 +289,75 @@ def ensemble(a):
             "F_increases": sum(1 for f0, f1 in zip(F0, F) if f1 > f0)}
 
 
+def learned(a):
+    """Learned corrector against plain RK4 on 2-D Cahn-Hilliard, 256^2, dx=1, M=1, kappa=1,
+    dt=0.01, from U(-0.05, 0.05) to T = 20 (2000 RK4 steps = 125 macro steps of 16).  The
+    learned path advances the FP64 state with the trained network and falls back to plain
+    RK4 for a macro step whose proposal drifts the mass by more than 1e-6 or raises the free
+    energy.  Error: relative L2 against the plain FP64 GPU RK4 result (bit-identical to the
+    CPU oracle's RK4; the parity tests pin that).  Times are wall-clock, device-synchronised,
+    after one untimed run of each path; n_ic initial states, median time, max error."""
+    gc = entry.load_package()
+    from gridcalc_amd import corrector as cr  # noqa: E402
+    params, meta = cr.load_params(256, 256)
+    if params is None:
+        return {"metric": "learned vs plain time-to-T", "error": "no trained parameters"}
+    cfg = dict(cr.CONFIG)
+    H, W, dt, sub = cfg["H"], cfg["W"], cfg["dt"], cfg["substeps"]
+    n_steps = 2000
+    g = gc.Grid.make(W, H, 1)
+    seeds = [991001 + i for i in range(a.n_ic)]
+    res = {"plain_fp64": [], "plain_fp32": [], "learned_fp64": []}
+    errs, errs32, fallbacks = [], [], []
+    with gc.Context() as ctx:
+        net = gc.Corrector(ctx, H, W, 1)
+        net.set_params(params, meta["scale"])
+
+        def timed(fn):
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = fn()
+            ctx.sync()
+            return time.perf_counter() - t0, r
+
+        for k, seed in enumerate(seeds):
+            c0 = cr.initial_state(H, W, seed).astype(np.float64)
+            ref = gc.DeviceArray.from_numpy(ctx, c0)
+            f32 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
+            lrn = gc.DeviceArray.from_numpy(ctx, c0)
+            if k == 0:  # untimed first run of each path
+                for arr in (ref, f32):
+                    gc.ch_integrate(ctx, g, arr, 1.0, 1.0, dt, 32, gc.RK4)
+                net.integrate(g, lrn, 1.0, 1.0, dt, sub, 2)
+                ref.upload(ctx, c0)
+                f32.upload(ctx, c0.astype(np.float32))
+                lrn.upload(ctx, c0)
+            t, _ = timed(lambda: gc.ch_integrate(ctx, g, ref, 1.0, 1.0, dt, n_steps, gc.RK4))
+            res["plain_fp64"].append(t)
+            t, _ = timed(lambda: gc.ch_integrate(ctx, g, f32, 1.0, 1.0, dt, n_steps, gc.RK4))
+            res["plain_fp32"].append(t)
+            t, nf = timed(lambda: net.integrate(g, lrn, 1.0, 1.0, dt, sub, n_steps // sub, 1e-6))
+            res["learned_fp64"].append(t)
+            fallbacks.append(nf)
+            r, l, q = ref.numpy(ctx), lrn.numpy(ctx), f32.numpy(ctx).astype(np.float64)
+            errs.append(float(np.linalg.norm(l - r) / np.linalg.norm(r)))
+            errs32.append(float(np.linalg.norm(q - r) / np.linalg.norm(r)))
+            for arr in (ref, f32, lrn):
+                arr.free()
+        net.close()
+    med = {k: float(np.median(v)) for k, v in res.items()}
+    return {"metric": "learned vs plain time-to-T", "T": n_steps * dt, "grid": [H, W],
+            "macro_steps": n_steps // sub, "n_ic": len(seeds),
+            "seconds": med,
+            "speedup_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64"],
+            "speedup_vs_plain_fp32": med["plain_fp32"] / med["learned_fp64"],
+            "rel_l2_learned_vs_fp64": max(errs), "rel_l2_plain_fp32_vs_fp64": max(errs32),
+            "fallbacks": fallbacks, "target_rel_l2": 1e-3,
+            "target_met": bool(max(errs) <= 1e-3 and med["plain_fp64"] / med["learned_fp64"] >= 5),
+            "training": {"seconds": meta.get("seconds"), "final_loss": meta["history"][-1],
+                         "pairs": "snapshot m -> m+1 of 2000 FP64 GPU RK4 trajectories"}}
+
+
 def copy(a):
     """Plain device-to-device copy of a known size (256 MiB): the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -307,7 +384,8 @@ def main():
               "RK4 path would shard by z-slabs with a halo exchange per stage; not built)",
               file=sys.stderr)
         sys.exit(2)
-    out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble}[a.workload](a)
+    out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble,
+           "learned": learned}[a.workload](a)
     print(json.dumps(out))
 
 

@@ -13,6 +13,7 @@ fields and (Nz, Ny, Nx, 9) for column-major Mat3 fields.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 import subprocess
 
@@ -157,6 +158,8 @@ def lib():
         "gc_cnn_adam": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], st),
         "gc_cnn_predict": ([vp, C.c_int, vp, vp, C.c_int], st),
         "gc_cast": ([vp, C.c_int, C.c_int, sz, vp, vp], st),
+        "gc_learned_integrate": ([vp, gp, C.c_int, C.POINTER(CHParams), vp, C.c_double, C.c_int,
+                                  C.c_int, C.c_double, ip], st),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -193,7 +196,13 @@ class Context:
 
     def __init__(self, stream=None):
         self._h = C.c_void_p()
+        self._dependents = []  # weak references to objects that use this context's stream
         _check(lib().gc_context_create(C.byref(self._h), stream))
+
+    def _register(self, obj):
+        """obj.close() runs before the context is destroyed (a gc_cnn or gc_ensemble must not
+        outlive its context)."""
+        self._dependents.append(weakref.ref(obj))
 
     @property
     def handle(self):
@@ -215,6 +224,11 @@ class Context:
 
     def close(self):
         if self._h:
+            for r in self._dependents:
+                obj = r()
+                if obj is not None:
+                    obj.close()
+            self._dependents = []
             lib().gc_context_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -460,6 +474,7 @@ class Ensemble:
             for m in self.members])
         self._h = C.c_void_p()
         _check(lib().gc_ensemble_create(ctx.handle, dtype, len(members), arr, dt, C.byref(self._h)))
+        ctx._register(self)
 
     def set_state(self, i, a):
         m = self.members[i]
@@ -534,6 +549,7 @@ class Corrector:
         self._h = C.c_void_p()
         _check(lib().gc_cnn_create(ctx.handle, H, W, batch_max, C.byref(self._h)))
         self.out = DeviceArray((batch_max, H, W), np.float32)
+        ctx._register(self)
 
     def set_params(self, p, scale):
         p = np.ascontiguousarray(p, dtype=np.float32)
@@ -571,6 +587,14 @@ class Corrector:
 
     def predict(self, c, c_new, B=1):
         _check(lib().gc_cnn_predict(self._h, c.gc_dtype, c.ptr, c_new.ptr, B))
+
+    def integrate(self, g, psi, M, kappa, dt, substeps, n_macro, mass_tol=1e-6):
+        """Learned macro steps with the plain-RK4 fallback; returns the fallback count."""
+        P = CHParams(M, kappa)
+        nf = C.c_int()
+        _check(lib().gc_learned_integrate(self._h, C.byref(g), psi.gc_dtype, C.byref(P), psi.ptr,
+                                          dt, substeps, n_macro, mass_tol, C.byref(nf)))
+        return nf.value
 
     def close(self):
         if self._h:

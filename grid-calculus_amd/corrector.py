@@ -1,0 +1,242 @@
+"""Training data, training and evaluation of the learned Cahn-Hilliard corrector.
+
+Configuration (the learned-corrector workload): 2-D Cahn-Hilliard on a 256x256 periodic
+grid, dx = 1, M = 1, kappa = 1, dt = 0.01; the network maps c(t) to c(t + 16 dt).
+Training trajectories start from U(-0.05, 0.05) noise and are advanced by the FP64 GPU RK4
+(the batched ensemble kernel, bit-identical per member to solve::integrate with the
+Cahn-Hilliard right-hand side).  Snapshots every 16 steps are kept on the device in FP32;
+a training pair is (snapshot m, snapshot m+1) of one trajectory.
+
+    python -m gridcalc_amd.corrector  (or corrector.main()) trains and writes
+    data/corrector_<H>x<W>.npy (FP32 parameters) and .json (scale, history, config).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (F32, F64, RK4, Context, Corrector, DeviceArray, Ensemble, Grid, N_CNN_PARAMS,
+               _check, ch_integrate, inputs, lib)
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000, n_steps=2000,
+              batch_traj=100, minibatch=16, iters_per_batch=3000, lr_max=2e-3, lr_min=2e-5,
+              seed=12345)
+
+
+def params_path(H, W):
+    return os.path.join(DATA_DIR, f"corrector_{H}x{W}.npy")
+
+
+def load_params(H=256, W=256):
+    """Trained parameters and metadata, or (None, None) if not trained."""
+    p = params_path(H, W)
+    if not os.path.exists(p):
+        return None, None
+    meta = json.load(open(p[:-4] + ".json"))
+    return np.load(p), meta
+
+
+def init_params(seed=0):
+    """He-style uniform initialisation (last layer scaled down)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.uniform(-1, 1, 288) * math.sqrt(6.0 / 9.0), np.zeros(32)]
+    for _ in range(4):
+        parts += [rng.uniform(-1, 1, 9216) * math.sqrt(6.0 / 288.0), np.zeros(32)]
+    parts += [rng.uniform(-1, 1, 288) * math.sqrt(6.0 / 288.0) * 0.1, np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == N_CNN_PARAMS
+    return p
+
+
+def initial_state(H, W, seed):
+    """U(-0.05, 0.05) noise."""
+    return inputs.symmetric(seed, H * W, 0.05).reshape(1, H, W)
+
+
+def state_ptr(ens, i):
+    p = C.c_void_p()
+    _check(lib().gc_ensemble_state_ptr(ens._h, i, C.byref(p)))
+    return p
+
+
+def generate(ctx, n_traj, n_steps, seed, cfg=CONFIG, log=print):
+    """Runs n_traj FP64 trajectories for n_steps; returns a device FP32 array of shape
+    (n_traj, n_macro + 1, H, W) with the snapshots every `substeps` steps."""
+    H, W, sub = cfg["H"], cfg["W"], cfg["substeps"]
+    n_macro = n_steps // sub
+    member = dict(nx=W, ny=H, kappa=cfg["kappa"], M=cfg["M"])
+    data = DeviceArray((n_traj, n_macro + 1, H, W), np.float32)
+    ens = Ensemble(ctx, [member] * n_traj, cfg["dt"], F64)
+    for i in range(n_traj):
+        ens.set_state(i, initial_state(H, W, seed + i))
+    t0 = time.perf_counter()
+    hw = H * W
+    for m in range(n_macro + 1):
+        for i in range(n_traj):
+            dst = C.c_void_p(data.ptr.value + ((i * (n_macro + 1) + m) * hw) * 4)
+            _check(lib().gc_cast(ctx.handle, F64, F32, hw, state_ptr(ens, i), dst))
+        if m < n_macro:
+            ens.step(sub)
+    ctx.sync()
+    ens.close()
+    log(f"  generated {n_traj} trajectories x {n_steps} steps in {time.perf_counter() - t0:.1f} s")
+    return data, n_macro
+
+
+def snapshot(ctx, data, n_macro, i, m):
+    """Host copy of snapshot m of trajectory i."""
+    H, W = data.shape[2], data.shape[3]
+    out = np.empty((H, W), np.float32)
+    src = C.c_void_p(data.ptr.value + ((i * (n_macro + 1) + m) * H * W) * 4)
+    _check(lib().gc_memcpy_dtoh(ctx.handle, out.ctypes.data_as(C.c_void_p), src, out.nbytes))
+    return out
+
+
+def increment_scale(ctx, data, n_macro, n_sample=64, seed=0):
+    """RMS of (snapshot m+1 - snapshot m) over a sample of pairs."""
+    rng = np.random.default_rng(seed)
+    n_traj = data.shape[0]
+    sq, cnt = 0.0, 0
+    for _ in range(n_sample):
+        i, m = int(rng.integers(n_traj)), int(rng.integers(n_macro))
+        d = snapshot(ctx, data, n_macro, i, m + 1).astype(np.float64) - snapshot(ctx, data, n_macro, i, m)
+        sq += float((d * d).sum())
+        cnt += d.size
+    return math.sqrt(sq / cnt)
+
+
+def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
+    cfg = dict(cfg)
+    H, W = cfg["H"], cfg["W"]
+    n_batches = cfg["n_traj"] // cfg["batch_traj"]
+    total_iters = n_batches * cfg["iters_per_batch"]
+    rng = np.random.default_rng(cfg["seed"])
+    history = []
+    t_start = time.perf_counter()
+    with Context() as ctx:
+        net = Corrector(ctx, H, W, cfg["minibatch"])
+        scale = None
+        it = 0
+        for bi in range(n_batches):
+            data, n_macro = generate(ctx, cfg["batch_traj"], cfg["n_steps"],
+                                     cfg["seed"] + 1000 * bi, cfg, log)
+            if scale is None:
+                scale = increment_scale(ctx, data, n_macro)
+                net.set_params(init_params(cfg["seed"]), scale)
+                log(f"  increment scale {scale:.4e}")
+            per_traj = n_macro + 1
+            target = C.c_void_p(data.ptr.value + H * W * 4)  # snapshot m+1 of pair index m
+
+            class _Shifted:  # device array view starting one snapshot later
+                ptr = target
+                gc_dtype = F32
+
+            losses = []
+            for _ in range(cfg["iters_per_batch"]):
+                tr = rng.integers(0, cfg["batch_traj"], cfg["minibatch"])
+                m = rng.integers(0, n_macro, cfg["minibatch"])
+                idx = (tr * per_traj + m).astype(np.int32)
+                frac = it / max(1, total_iters - 1)
+                lr = cfg["lr_min"] + 0.5 * (cfg["lr_max"] - cfg["lr_min"]) * (1 + math.cos(math.pi * frac))
+                loss = net.loss_grad(data, _Shifted, idx=idx)
+                net.adam(lr)
+                losses.append(loss)
+                it += 1
+            history.append(float(np.mean(losses)))
+            log(f"  batch {bi + 1}/{n_batches}: mean loss {history[-1]:.4e} "
+                f"(last {losses[-1]:.4e}), {time.perf_counter() - t_start:.0f} s")
+            data.free()
+        params = net.params()
+        net.close()
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, f"corrector_{H}x{W}.npy")
+    np.save(path, params)
+    meta = dict(scale=scale, history=history, config=cfg, seconds=time.perf_counter() - t_start)
+    json.dump(meta, open(path[:-4] + ".json", "w"), indent=1)
+    log(f"  wrote {path}")
+    return params, meta
+
+
+def one_step_errors(net, cfg=CONFIG, n_traj=8, seed=777001, log=print):
+    """Relative L2 error of one macro step (substeps RK4 steps) on held-out trajectories:
+    returns (network, identity) where identity is the error of c(t+16 dt) ~ c(t), i.e. of the
+    method without the network.  Errors are over all macro steps of n_traj trajectories."""
+    ctx = net.ctx
+    data, n_macro = generate(ctx, n_traj, cfg["n_steps"], seed, cfg, log)
+    H, W = cfg["H"], cfg["W"]
+    hw = H * W
+    cur = DeviceArray((1, H, W), np.float32)
+    nxt = DeviceArray((1, H, W), np.float32)
+    e_net = e_id = norm = 0.0
+    for i in range(n_traj):
+        for m in range(n_macro):
+            src = C.c_void_p(data.ptr.value + ((i * (n_macro + 1) + m) * hw) * 4)
+            _check(lib().gc_memcpy_dtod(ctx.handle, cur.ptr, src, hw * 4))
+            net.predict(cur, nxt, 1)
+            a = snapshot(ctx, data, n_macro, i, m).astype(np.float64)
+            b = snapshot(ctx, data, n_macro, i, m + 1).astype(np.float64)
+            p = nxt.numpy(ctx).astype(np.float64).reshape(H, W)
+            e_net += float(((p - b) ** 2).sum())
+            e_id += float(((a - b) ** 2).sum())
+            norm += float((b * b).sum())
+    cur.free()
+    nxt.free()
+    data.free()
+    return math.sqrt(e_net / norm), math.sqrt(e_id / norm)
+
+
+def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6):
+    """Runs the learned integrator and the plain FP64 RK4 from the same U(-0.05, 0.05) state
+    over n_steps steps (default cfg n_steps, i.e. T = n_steps * dt); returns a dict with the
+    relative L2 and max errors of the learned result against the RK4 result, the fallback
+    count and both wall times (device synchronised)."""
+    ctx = net.ctx
+    H, W, dt, sub = cfg["H"], cfg["W"], cfg["dt"], cfg["substeps"]
+    n_steps = cfg["n_steps"] if n_steps is None else n_steps
+    assert n_steps % sub == 0
+    g = Grid.make(W, H, 1)
+    c0 = initial_state(H, W, seed).astype(np.float64)
+    ref = DeviceArray.from_numpy(ctx, c0)
+    lrn = DeviceArray.from_numpy(ctx, c0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ch_integrate(ctx, g, ref, cfg["M"], cfg["kappa"], dt, n_steps, RK4)
+    ctx.sync()
+    t_rk4 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nf = net.integrate(g, lrn, cfg["M"], cfg["kappa"], dt, sub, n_steps // sub, mass_tol)
+    ctx.sync()
+    t_lrn = time.perf_counter() - t0
+    r, l = ref.numpy(ctx), lrn.numpy(ctx)
+    ref.free()
+    lrn.free()
+    return dict(rel_l2=float(np.linalg.norm(l - r) / np.linalg.norm(r)),
+                max_abs=float(np.abs(l - r).max()), fallbacks=nf, n_macro=n_steps // sub,
+                seconds_rk4=t_rk4, seconds_learned=t_lrn,
+                mass_drift=float(abs(l.mean() - c0.mean())))
+
+
+def main(argv=None):
+    """key=value arguments override CONFIG; out=<dir> changes the output directory."""
+    argv = sys.argv[1:] if argv is None else argv
+    cfg = dict(CONFIG)
+    out_dir = DATA_DIR
+    for a in argv:
+        k, v = a.split("=", 1)
+        if k == "out":
+            out_dir = v
+        else:
+            cfg[k] = type(CONFIG[k])(v)
+    train(cfg, out_dir=out_dir)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,7 +6,8 @@
 //   out = conv(a5, W6) + b6,      prediction = c + scale * (out - mean(out))
 //
 // conv(x, W)[y, x] = sum_tap W[tap] x[(y+dy) mod H, (x+dx) mod W], tap = 3(dy+1) + (dx+1).
-// Weights and activations are BF16; every accumulation is FP32.  The four 32 -> 32 layers
+// The 32 -> 32 layers take BF16 weights and activations; the 1 -> 32 and 32 -> 1 layers
+// take the state and their weights in FP32; every accumulation is FP32.  The four 32 -> 32 layers
 // carry 98 % of the FLOPs and run as implicit GEMMs on the matrix cores
 // (v_mfma_f32_32x32x16_bf16): M = 32 sites of a row, N = 32 output channels, K = 9 taps x
 // 32 input channels (18 MFMAs per 32 x 32 output tile).  Backward-data is the same
@@ -18,6 +19,7 @@
 // for l = 2..5, W6[tap][ci], b6.  Activations are NHWC (32 channels contiguous per site).
 #include "gc_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -51,15 +53,15 @@ __device__ __forceinline__ float gelu_grad_f(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
          z * expf(-0.5f * z * z) * 0.39894228040143268f;
 }
-__device__ __forceinline__ float bf(float v) { return (float)(__bf16)v; }
 
 __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ---- layer 1 (1 -> 32), vector ALU -------------------------------------------------------
-// One thread per site; the state value is rounded to BF16 as the network input.
+// One thread per site; the state and W1 enter in FP32 (a BF16-rounded input would carry
+// noise of 2^-9 |c| that the network answers as if it were part of the state).
 template <class TS>
 __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const int* __restrict__ idx,
-                                               const __bf16* __restrict__ w1b,
+                                               const float* __restrict__ w1,
                                                const float* __restrict__ b1, float* __restrict__ z1,
                                                __bf16* __restrict__ a1, int H, int W, int B) {
   const int HW = H * W;
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const i
   float xv[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t)
-    xv[t] = bf((float)img[wrap1(y + t / 3 - 1, H) * W + wrap1(x + t % 3 - 1, W)]);
+    xv[t] = (float)img[wrap1(y + t / 3 - 1, H) * W + wrap1(x + t % 3 - 1, W)];
 #pragma unroll
   for (int co = 0; co < kC; co += 8) {
     float zz[8];
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const i
     for (int u = 0; u < 8; ++u) {
       float acc = 0.0f;
 #pragma unroll
-      for (int t = 0; t < 9; ++t) acc = fmaf((float)w1b[(co + u) * 9 + t], xv[t], acc);
+      for (int t = 0; t < 9; ++t) acc = fmaf(w1[(co + u) * 9 + t], xv[t], acc);
       zz[u] = acc + b1[co + u];
     }
     bf16x8 av;
@@ -244,9 +246,9 @@ __global__ __launch_bounds__(256) void k_conv_mid_bwdw(const __bf16* __restrict_
   }
 }
 
-// ---- layer 6 (32 -> 1), vector ALU ---------------------------------------------------------
+// ---- layer 6 (32 -> 1), vector ALU, FP32 weights ------------------------------------------
 __global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
-                                               const __bf16* __restrict__ w6b,
+                                               const float* __restrict__ w6,
                                                const float* __restrict__ b6,
                                                float* __restrict__ out, int H, int W, int B) {
   const int HW = H * W;
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
     for (int ch = 0; ch < kC; ch += 8) {
       const bf16x8 v = *(const bf16x8*)(a5 + n * kC + ch);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc = fmaf((float)w6b[tp * kC + ch + u], (float)v[u], acc);
+      for (int u = 0; u < 8; ++u) acc = fmaf(w6[tp * kC + ch + u], (float)v[u], acc);
     }
   }
   out[s] = acc + b6[0];
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ out, con
 // Backward-data of layer 6 fused with gelu': dz5[s][ci] = gelu'(z5) sum_tap W6[tap][ci]
 // dout[s - off(tap)].
 __global__ __launch_bounds__(256) void k_conv6_bwdd(const float* __restrict__ dout,
-                                                    const __bf16* __restrict__ w6b,
+                                                    const float* __restrict__ w6,
                                                     const float* __restrict__ z5,
                                                     __bf16* __restrict__ dz5, int H, int W, int B) {
   const int HW = H * W;
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(256) void k_conv6_bwdd(const float* __restrict__ do
     for (int u = 0; u < 8; ++u) {
       float acc = 0.0f;
 #pragma unroll
-      for (int tp = 0; tp < 9; ++tp) acc = fmaf((float)w6b[tp * kC + ch + u], dv[tp], acc);
+      for (int tp = 0; tp < 9; ++tp) acc = fmaf(w6[tp * kC + ch + u], dv[tp], acc);
       o[u] = (__bf16)(acc * gelu_grad_f(z5[(size_t)s * kC + ch + u]));
     }
     *(bf16x8*)(dz5 + (size_t)s * kC + ch) = o;
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(320) void k_conv1_bwdw(const __bf16* __restrict__ d
     const float d = (float)dz1[(size_t)s * kC + co];
     if (e < 288) {
       const TS* img = c + (size_t)(idx ? idx[b] : b) * HW;
-      const float xv = bf((float)img[wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W)]);
+      const float xv = (float)img[wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W)];
       acc = fmaf(d, xv, acc);
     } else {
       acc += d;
@@ -398,16 +400,12 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
   p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
-// FP32 master parameters -> BF16 working copies: W1, W6, the 32 -> 32 matrices [co][k] and
+// FP32 master parameters -> BF16 working copies of the 32 -> 32 matrices [co][k] and
 // their flipped transposes [ci][tap'*32 + co] = W[co][(8 - tap')*32 + ci] (backward-data).
-__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf16* __restrict__ w1b,
-                                              __bf16* __restrict__ w6b, __bf16* __restrict__ wmid,
+// W1 and W6 are used in FP32 directly.
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf16* __restrict__ wmid,
                                               __bf16* __restrict__ wmidT) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < 288) {
-    w1b[i] = (__bf16)p[kOffW1 + i];
-    w6b[i] = (__bf16)p[kOffW6 + i];
-  }
   if (i < 4 * kMidW) {
     const int l = i / kMidW, e = i % kMidW;
     const float* W = p + kOffMid + l * kMidStride;
@@ -462,7 +460,7 @@ struct gc_cnn {
   int H = 0, W = 0, Bmax = 0;
   double scale = 1.0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  __bf16 *w1b = nullptr, *w6b = nullptr, *wmid = nullptr, *wmidT = nullptr;
+  __bf16 *wmid = nullptr, *wmidT = nullptr;
   float* z[5] = {};
   __bf16* a[5] = {};
   __bf16 *dzA = nullptr, *dzB = nullptr;
@@ -471,6 +469,8 @@ struct gc_cnn {
   size_t part_floats = 0;
   double* dpart = nullptr;
   int* idx = nullptr;
+  void* hist = nullptr;     // learned integration: proposals of one chunk
+  double* diag = nullptr;   // learned integration: (mass, free energy) per proposal
   int adam_t = 0;
   bool packed = false;
 };
@@ -491,7 +491,7 @@ gc_status cnn_alloc(void** p, size_t bytes) {
 
 gc_status pack(gc_cnn* n) {
   hipLaunchKernelGGL(k_pack, dim3((4 * kMidW + 255) / 256), dim3(256), 0, n->ctx->stream, n->p,
-                     n->w1b, n->w6b, n->wmid, n->wmidT);
+                     n->wmid, n->wmidT);
   GC_LAUNCHED();
   n->packed = true;
   return GC_OK;
@@ -518,7 +518,7 @@ gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
   const long ns = (long)B * n->H * n->W;
   const unsigned g1 = (unsigned)((ns + 255) / 256);
   hipStream_t st = n->ctx->stream;
-  hipLaunchKernelGGL((k_conv1<TS>), dim3(g1), dim3(256), 0, st, c, d_idx, n->w1b, n->p + kOffB1,
+  hipLaunchKernelGGL((k_conv1<TS>), dim3(g1), dim3(256), 0, st, c, d_idx, n->p + kOffW1, n->p + kOffB1,
                      train ? n->z[0] : nullptr, n->a[0], n->H, n->W, B);
   GC_LAUNCHED();
   const int tx = n->W / kTC, ty = n->H / kTR;
@@ -528,7 +528,7 @@ gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
                        nullptr, train ? n->z[l + 1] : nullptr, n->a[l + 1], n->H, n->W, tx, ty);
     GC_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_conv6, dim3(g1), dim3(256), 0, st, n->a[4], n->w6b, n->p + kOffB6, n->out,
+  hipLaunchKernelGGL(k_conv6, dim3(g1), dim3(256), 0, st, n->a[4], n->p + kOffW6, n->p + kOffB6, n->out,
                      n->H, n->W, B);
   GC_LAUNCHED();
   return GC_OK;
@@ -548,7 +548,7 @@ gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
   hipLaunchKernelGGL(k_reduce_parts, dim3((289 + 255) / 256), dim3(256), 0, st, n->part,
                      kPartBlocks, 289, n->g + kOffW6);
   GC_LAUNCHED();
-  hipLaunchKernelGGL(k_conv6_bwdd, dim3(g1), dim3(256), 0, st, n->dout, n->w6b, n->z[4], n->dzA,
+  hipLaunchKernelGGL(k_conv6_bwdd, dim3(g1), dim3(256), 0, st, n->dout, n->p + kOffW6, n->z[4], n->dzA,
                      n->H, n->W, B);
   GC_LAUNCHED();
   const int tx = n->W / kTC, ty = n->H / kTR, ntiles = tx * ty * B;
@@ -625,8 +625,6 @@ gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** o
   A((void**)&n->g, np4);
   A((void**)&n->m, np4);
   A((void**)&n->v, np4);
-  A((void**)&n->w1b, 288 * 2);
-  A((void**)&n->w6b, 288 * 2);
   A((void**)&n->wmid, 4 * kMidW * 2);
   A((void**)&n->wmidT, 4 * kMidW * 2);
   for (int l = 0; l < 5; ++l) {
@@ -656,8 +654,8 @@ gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** o
 gc_status gc_cnn_destroy(gc_cnn* n) {
   if (!n) return GC_OK;
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
-  void* ptrs[] = {n->p, n->g, n->m, n->v, n->w1b, n->w6b, n->wmid, n->wmidT, n->dzA, n->dzB,
-                  n->out, n->dout, n->part, n->dpart, n->idx};
+  void* ptrs[] = {n->p, n->g, n->m, n->v, n->wmid, n->wmidT, n->dzA, n->dzB,
+                  n->out, n->dout, n->part, n->dpart, n->idx, n->hist, n->diag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 5; ++l) {
@@ -787,6 +785,190 @@ gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const v
   else
     GC_HIP(hipMemcpyAsync(dst, src, n * dsize(from), hipMemcpyDeviceToDevice, ctx->stream));
   GC_LAUNCHED();
+  return GC_OK;
+}
+
+}  // extern "C"
+
+// ---- learned time stepping with the plain-RK4 fallback -------------------------------------
+//
+// One macro step advances a 2-D state by `substeps` RK4 steps of size `step`: the network
+// proposes c_new = c + scale*(out - mean(out)); the proposal is accepted unless the mean of
+// the state drifts by more than mass_tol from its initial value or the free energy
+// increases (Cahn-Hilliard is a gradient flow: F never increases along exact solutions);
+// otherwise the macro step is recomputed with the plain fused RK4 pipeline.  Mass and free
+// energy come from one fused FP64 compensated reduction per macro step.
+
+namespace {
+
+struct Comp2 {
+  double ms, mc, fs, fc;
+};
+
+__device__ __forceinline__ void neu(double& s, double& c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+// Sum of c and of the Cahn-Hilliard density (examples/common/cahn_hilliard.hpp:65-67) over a
+// 2-D periodic grid, compensated, in one pass.
+template <class T>
+__global__ __launch_bounds__(256) void k_mass_fe(const T* __restrict__ c, int nx, int ny,
+                                                 double ihx, double ihy, double kappa,
+                                                 Comp2* __restrict__ part) {
+  __shared__ Comp2 sh[4];
+  const long n = (long)nx * ny;
+  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+  for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256) {
+    const int i = (int)(s % nx), j = (int)(s / nx);
+    const double p = (double)c[s];
+    double dx = 0.0, dy = 0.0;
+    dx += -0.5 * (double)c[(size_t)j * nx + wrap1(i - 1, nx)];
+    dy += -0.5 * (double)c[(size_t)wrap1(j - 1, ny) * nx + i];
+    dx += 0.0 * p;
+    dy += 0.0 * p;
+    dx += 0.5 * (double)c[(size_t)j * nx + wrap1(i + 1, nx)];
+    dy += 0.5 * (double)c[(size_t)wrap1(j + 1, ny) * nx + i];
+    const double gx = dx * ihx, gy = dy * ihy, gz = 0.0;
+    const double sq = gx * gx + gy * gy + gz * gz;
+    neu(ms, mc, p);
+    neu(fs, fc, -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * sq);
+  }
+  Comp2 v{ms, mc, fs, fc};
+  for (int st = 1; st < 64; st <<= 1) {
+    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
+    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
+    double s1 = lo.ms, c1 = lo.mc + hi.mc;
+    neu(s1, c1, hi.ms);
+    double s2 = lo.fs, c2 = lo.fc + hi.fc;
+    neu(s2, c2, hi.fs);
+    v = Comp2{s1, c1, s2, c2};
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Comp2 r = sh[0];
+    for (int w = 1; w < 4; ++w) {
+      double c1 = r.mc + sh[w].mc, c2 = r.fc + sh[w].fc;
+      neu(r.ms, c1, sh[w].ms);
+      neu(r.fs, c2, sh[w].fs);
+      r.mc = c1;
+      r.fc = c2;
+    }
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ void k_mass_fe_final(const Comp2* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      mc += part[b].mc;
+      fc += part[b].fc;
+      neu(ms, mc, part[b].ms);
+      neu(fs, fc, part[b].fs);
+    }
+    out[0] = ms + mc;
+    out[1] = fs + fc;
+  }
+}
+
+constexpr int kMFBlocks = 128;
+constexpr int kChunk = 16;  // macro steps proposed between two host decisions
+
+// Enqueues (sum of c, sum of the density) of one 2-D state into out_dev[0..1].
+gc_status mass_fe_async(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c, double kappa,
+                        double* out_dev) {
+  hipStream_t st = n->ctx->stream;
+  Comp2* part = (Comp2*)n->dpart;
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_mass_fe<double>), dim3(kMFBlocks), dim3(256), 0, st, (const double*)c,
+                       g->nx, g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part);
+  else
+    hipLaunchKernelGGL((k_mass_fe<float>), dim3(kMFBlocks), dim3(256), 0, st, (const float*)c,
+                       g->nx, g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_mass_fe_final, dim3(1), dim3(64), 0, st, part, kMFBlocks, out_dev);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+gc_status fetch(gc_cnn* n, double* host, const double* dev, int count) {
+  GC_HIP(hipMemcpyAsync(host, dev, count * sizeof(double), hipMemcpyDeviceToHost, n->ctx->stream));
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  return GC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Macro steps are proposed in chunks of up to kChunk without waiting for the host: proposal k
+// of a chunk is written to history slot k and its mass and free energy to diag[k].  One copy
+// of diag per chunk then lets the host apply the acceptance rule in order; the state after the
+// last accepted proposal becomes psi, and the first rejected macro step (if any) is recomputed
+// by plain RK4 before the next chunk starts.  The decisions, and so the result, are those of
+// checking every proposal as it is made.
+gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
+                               void* psi, double step, int substeps, int n_macro,
+                               double mass_tol, int* n_fallback) {
+  GC_TRY(check_cnn(n, 1));
+  GC_TRY(check_grid(g));
+  GC_TRY(check_dtype(dt));
+  if (!p || !psi) return set_error(GC_ERR_INVALID, "gridcalc: null argument");
+  if (g->nz != 1 || g->nx != n->W || g->ny != n->H)
+    return set_error(GC_ERR_INVALID, "gridcalc: the corrector was built for %d x %d x 1 grids",
+                     n->W, n->H);
+  if (n_macro < 0 || substeps < 1 || !(step >= 0.0))
+    return set_error(GC_ERR_INVALID, "gridcalc: bad learned-integration arguments");
+  const size_t bytes = sites(g) * dsize(dt);
+  if (!n->hist) GC_TRY(cnn_alloc(&n->hist, (size_t)(kChunk + 1) * sites(g) * sizeof(double)));
+  if (!n->diag) GC_TRY(cnn_alloc((void**)&n->diag, 2 * (kChunk + 1) * sizeof(double)));
+  char* hist = (char*)n->hist;
+  const double vol = (g->hx * g->hy) * g->hz;
+  const double vol_total = (double)sites(g) * vol;
+  double h[2 * (kChunk + 1)];
+  GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
+  GC_TRY(fetch(n, h, n->diag, 2));
+  const double Minit = vol * h[0];
+  double Fprev = vol * h[1];
+  int fallbacks = 0;
+  int m = 0;
+  while (m < n_macro) {
+    const int K = std::min(kChunk, n_macro - m);
+    const void* src = psi;
+    for (int k = 1; k <= K; ++k) {
+      void* dst = hist + (size_t)k * bytes;
+      GC_TRY(gc_cnn_predict(n, dt, src, dst, 1));
+      GC_TRY(mass_fe_async(n, g, dt, dst, p->kappa, n->diag + 2 * k));
+      src = dst;
+    }
+    GC_TRY(fetch(n, h + 2, n->diag + 2, 2 * K));
+    int good = 0;
+    for (int k = 1; k <= K; ++k) {
+      const double Mk = vol * h[2 * k], Fk = vol * h[2 * k + 1];
+      if (!(std::isfinite(Fk) && Fk <= Fprev && std::fabs(Mk - Minit) / vol_total <= mass_tol))
+        break;
+      Fprev = Fk;
+      good = k;
+    }
+    if (good > 0)
+      GC_HIP(hipMemcpyAsync(psi, hist + (size_t)good * bytes, bytes, hipMemcpyDeviceToDevice,
+                            n->ctx->stream));
+    m += good;
+    if (good < K) {
+      ++fallbacks;
+      GC_TRY(gc_ch_integrate(n->ctx, g, dt, p, psi, step, substeps, GC_RK4));
+      GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
+      GC_TRY(fetch(n, h, n->diag, 2));
+      Fprev = vol * h[1];
+      m += 1;
+    }
+  }
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  if (n_fallback) *n_fallback = fallbacks;
   return GC_OK;
 }
 

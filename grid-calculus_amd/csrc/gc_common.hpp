@@ -67,6 +67,12 @@ inline size_t dsize(gc_dtype dt) { return dt == GC_F64 ? 8 : 4; }
 
 // ---- device helpers ------------------------------------------------------------
 
+// Periodic wrap for any index.
+__device__ __forceinline__ int wrap_any(int x, int n) {
+  x %= n;
+  return x < 0 ? x + n : x;
+}
+
 // Periodic wrap for an index at most one extent outside [0, n) -- which is all a
 // stencil of radius r reaches once n >= 2r+1 -- plus the degenerate n == 1 axis.
 __device__ __forceinline__ int wrap(int i, int n) {

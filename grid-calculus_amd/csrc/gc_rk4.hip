@@ -265,11 +265,6 @@ __device__ __forceinline__ T lap_lds(const T* __restrict__ pc, const T* __restri
   return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
 }
 
-__device__ __forceinline__ int wrap_any(int x, int n) {
-  x %= n;
-  return x < 0 ? x + n : x;
-}
-
 template <class T, int OX, int OY, int ZC>
 struct Ch3Cfg {
   static constexpr int NT = 256;

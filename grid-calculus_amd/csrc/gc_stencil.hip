@@ -64,51 +64,6 @@ struct Nb {
   }
 };
 
-// diff/Laplacian.hpp:56-71
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_laplacian(const T* __restrict__ f, T* __restrict__ out,
-                                                   Dims d, int nbx, int nby, T ihx2, T ihy2,
-                                                   T ihz2) {
-  using S = D2<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T dxx = 0, dyy = 0, dzz = 0;
-#pragma unroll
-  for (int m = 0; m < S::N; ++m) {
-    const int o = m - S::R;
-    const T w = T(S::w(m));
-    dxx += w * f[nb.at(o, 0, 0)];
-    dyy += w * f[nb.at(0, o, 0)];
-    dzz += w * f[nb.at(0, 0, o)];
-  }
-  out[nb.c()] = dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
-}
-
-// diff/Gradient.hpp:56-70 (scalar -> Vec3, AoS)
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_gradient(const T* __restrict__ f, T* __restrict__ out,
-                                                  Dims d, int nbx, int nby, T ihx, T ihy,
-                                                  T ihz) {
-  using S = D1<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T dx = 0, dy = 0, dz = 0;
-#pragma unroll
-  for (int m = 0; m < S::N; ++m) {
-    const int o = m - S::R;
-    const T w = T(S::w(m));
-    dx += w * f[nb.at(o, 0, 0)];
-    dy += w * f[nb.at(0, o, 0)];
-    dz += w * f[nb.at(0, 0, o)];
-  }
-  T* o3 = out + 3 * nb.c();
-  o3[0] = dx * ihx;
-  o3[1] = dy * ihy;
-  o3[2] = dz * ihz;
-}
-
 // diff/Gradient.hpp:113-134 (Vec3 -> Mat3, column-major: M(comp, axis) at comp + 3*axis)
 template <class T, int O>
 __global__ __launch_bounds__(256) void k_gradient_vec(const T* __restrict__ v,
@@ -182,43 +137,191 @@ __global__ __launch_bounds__(256) void k_divergence_mat(const T* __restrict__ m9
   }
 }
 
-// diff/Biharmonic.hpp:67-101: S4 loop, mixed loop (mb outer, ma inner), final
-// expression in the reference's order.
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_biharmonic(const T* __restrict__ f, T* __restrict__ out,
-                                                    Dims d, int nbx, int nby, T ihx2, T ihy2,
-                                                    T ihz2, T ihx4, T ihy4, T ihz4) {
-  using S2 = D2<O>;
-  using S4 = D4<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T dx4 = 0, dy4 = 0, dz4 = 0;
-#pragma unroll
-  for (int m = 0; m < S4::N; ++m) {
-    const int o = m - S4::R;
-    const T w = T(S4::w(m));
-    dx4 += w * f[nb.at(o, 0, 0)];
-    dy4 += w * f[nb.at(0, o, 0)];
-    dz4 += w * f[nb.at(0, 0, o)];
+// ---- z-marching LDS engine for the scalar-input operators --------------------------------
+//
+// A block owns an OX x OY tile of xy-columns and a chunk of consecutive planes.  It keeps the
+// 2R+1 planes z-R..z+R of every input field (tile + R-site halo, periodic wrap applied while
+// filling) in an LDS ring, so each input value is read from memory once per block plus the
+// halo; the next plane is fetched into registers while the current output plane is formed.
+// The operators read neighbours through an accessor and keep the reference's per-axis
+// accumulation order, so results equal the one-site-at-a-time evaluation bit for bit.
+
+template <int R, int NF, int OX, int OY>
+struct ZTile {
+  static constexpr int NT = 256;
+  static constexpr int PW = OX + 2 * R, PH = OY + 2 * R, PN = PW * PH, NP = 2 * R + 1;
+  static constexpr int PL = (PN + NT - 1) / NT;  // plane slots filled per thread
+  static constexpr int ROWS = NT / OX;
+  static constexpr int OR = OX * OY / NT;        // output sites per thread per plane
+  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
+};
+
+// Neighbour access into the ring: F(field, dx, dy, dz) at the current output site.
+template <class T, int R, int PN, int NP, int PW>
+struct RingAt {
+  const T* ring;     // [NF][NP][PN]
+  const int* slot;   // slot of plane z + dz at slot[dz + R]
+  int base;          // (oy + R) * PW + (ox + R)
+  __device__ __forceinline__ T operator()(int f, int dx, int dy, int dz) const {
+    return ring[(f * NP + slot[dz + R]) * PN + base + dy * PW + dx];
   }
-  T dxy = 0, dxz = 0, dyz = 0;
+};
+
+template <class T, int OX, int OY, class Op>
+__global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
+                                                const T* __restrict__ f1, T* __restrict__ out,
+                                                Dims d, int ntx, int nty, int zlen) {
+  constexpr int R = Op::R, NF = Op::NF;
+  using Z = ZTile<R, NF, OX, OY>;
+  __shared__ T ring[NF * Z::NP * Z::PN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+
+  unsigned poff[Z::PL];
+  bool pin[Z::PL];
 #pragma unroll
-  for (int mb = 0; mb < S2::N; ++mb) {
-    const int ob = mb - S2::R;
-    const double wb = S2::w(mb);
+  for (int s = 0; s < Z::PL; ++s) {
+    const int idx = t + s * Z::NT;
+    pin[s] = idx < Z::PN;
+    const int px = idx % Z::PW, py = idx / Z::PW;
+    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
+                     : 0u;
+  }
+  T reg[NF][Z::PL];
+  auto load = [&](int plane) {
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
 #pragma unroll
-    for (int ma = 0; ma < S2::N; ++ma) {
-      const int oa = ma - S2::R;
-      const T wab = T(S2::w(ma) * wb);
-      dxy += wab * f[nb.at(oa, ob, 0)];
-      dxz += wab * f[nb.at(oa, 0, ob)];
-      dyz += wab * f[nb.at(0, oa, ob)];
+    for (int s = 0; s < Z::PL; ++s) {
+      if (pin[s]) {
+        reg[0][s] = f0[zo + poff[s]];
+        if constexpr (NF > 1) reg[NF - 1][s] = f1[zo + poff[s]];
+      }
     }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int s = 0; s < Z::PL; ++s)
+        if (pin[s]) ring[(f * Z::NP + slot) * Z::PN + t + s * Z::NT] = reg[f][s];
+  };
+
+  // plane z0 - R + i lives in slot i % NP
+  for (int i = 0; i < 2 * R; ++i) {
+    load(z0 - R + i);
+    store(i);
   }
-  out[nb.c()] = dx4 * ihx4 + dy4 * ihy4 + dz4 * ihz4 +
-                T(2) * (dxy * ihx2 * ihy2 + dxz * ihx2 * ihz2 + dyz * ihy2 * ihz2);
+  load(z0 + R);
+  const int ox = t % OX, oy0 = t / OX;
+  for (int k = 0; k < nout; ++k) {
+    store((k + 2 * R) % Z::NP);
+    __syncthreads();
+    if (k + 1 < nout) load(z0 + k + 1 + R);
+    int slot[Z::NP];
+#pragma unroll
+    for (int j = 0; j < Z::NP; ++j) slot[j] = (k + j) % Z::NP;
+    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
+#pragma unroll
+    for (int q = 0; q < Z::OR; ++q) {
+      const int oy = oy0 + q * Z::ROWS;
+      if (x0 + ox < d.nx && y0 + oy < d.ny) {
+        const RingAt<T, R, Z::PN, Z::NP, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
+        op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
+      }
+    }
+    __syncthreads();
+  }
 }
+
+// diff/Laplacian.hpp:56-71
+template <class T, int O>
+struct OpLap {
+  static constexpr int R = D2<O>::R, NF = 1;
+  T ihx2, ihy2, ihz2;
+  template <class A>
+  __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
+    using S = D2<O>;
+    T dxx = 0, dyy = 0, dzz = 0;
+#pragma unroll
+    for (int m = 0; m < S::N; ++m) {
+      const int o = m - S::R;
+      const T w = T(S::w(m));
+      dxx += w * F(0, o, 0, 0);
+      dyy += w * F(0, 0, o, 0);
+      dzz += w * F(0, 0, 0, o);
+    }
+    out[c] = dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+  }
+};
+
+// diff/Gradient.hpp:56-70 (scalar -> Vec3, AoS)
+template <class T, int O>
+struct OpGrad {
+  static constexpr int R = D1<O>::R, NF = 1;
+  T ihx, ihy, ihz;
+  template <class A>
+  __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
+    using S = D1<O>;
+    T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+    for (int m = 0; m < S::N; ++m) {
+      const int o = m - S::R;
+      const T w = T(S::w(m));
+      dx += w * F(0, o, 0, 0);
+      dy += w * F(0, 0, o, 0);
+      dz += w * F(0, 0, 0, o);
+    }
+    T* o3 = out + 3 * c;
+    o3[0] = dx * ihx;
+    o3[1] = dy * ihy;
+    o3[2] = dz * ihz;
+  }
+};
+
+// diff/Biharmonic.hpp:67-101: S4 loop, mixed loop (mb outer, ma inner), final expression in
+// the reference's order.
+template <class T, int O>
+struct OpBih {
+  static constexpr int R = D4<O>::R, NF = 1;
+  static_assert(D4<O>::R >= D2<O>::R, "ring radius");
+  T ihx2, ihy2, ihz2, ihx4, ihy4, ihz4;
+  template <class A>
+  __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
+    using S2 = D2<O>;
+    using S4 = D4<O>;
+    T dx4 = 0, dy4 = 0, dz4 = 0;
+#pragma unroll
+    for (int m = 0; m < S4::N; ++m) {
+      const int o = m - S4::R;
+      const T w = T(S4::w(m));
+      dx4 += w * F(0, o, 0, 0);
+      dy4 += w * F(0, 0, o, 0);
+      dz4 += w * F(0, 0, 0, o);
+    }
+    T dxy = 0, dxz = 0, dyz = 0;
+#pragma unroll
+    for (int mb = 0; mb < S2::N; ++mb) {
+      const int ob = mb - S2::R;
+      const double wb = S2::w(mb);
+#pragma unroll
+      for (int ma = 0; ma < S2::N; ++ma) {
+        const int oa = ma - S2::R;
+        const T wab = T(S2::w(ma) * wb);
+        dxy += wab * F(0, oa, ob, 0);
+        dxz += wab * F(0, oa, 0, ob);
+        dyz += wab * F(0, 0, oa, ob);
+      }
+    }
+    out[c] = dx4 * ihx4 + dy4 * ihy4 + dz4 * ihz4 +
+             T(2) * (dxy * ihx2 * ihy2 + dxz * ihx2 * ihz2 + dyz * ihy2 * ihz2);
+  }
+};
 
 // fvm/CellLaplacian.hpp:48-50
 template <class T>
@@ -226,31 +329,37 @@ __device__ __forceinline__ T hmean(T a, T b) {
   return T(2) * a * b / (a + b);
 }
 
-// fvm/CellLaplacian.hpp:115-141
+// fvm/CellLaplacian.hpp:115-141; field 0 = psi, field 1 = D
 template <class T>
-__global__ __launch_bounds__(256) void k_cell_laplacian(const T* __restrict__ psi,
-                                                        const T* __restrict__ D,
-                                                        T* __restrict__ out, Dims d, int nbx,
-                                                        int nby, T ihx2, T ihy2, T ihz2) {
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  const size_t c = nb.c();
-  const T pc = psi[c];
-  const T Dc = D[c];
-  const size_t xp = nb.at(1, 0, 0), xm = nb.at(-1, 0, 0);
-  const size_t yp = nb.at(0, 1, 0), ym = nb.at(0, -1, 0);
-  const size_t zp = nb.at(0, 0, 1), zm = nb.at(0, 0, -1);
-  const T dxp = hmean(Dc, D[xp]);
-  const T dxm = hmean(D[xm], Dc);
-  const T dyp = hmean(Dc, D[yp]);
-  const T dym = hmean(D[ym], Dc);
-  const T dzp = hmean(Dc, D[zp]);
-  const T dzm = hmean(D[zm], Dc);
-  const T rx = (dxp * (psi[xp] - pc) - dxm * (pc - psi[xm])) * ihx2;
-  const T ry = (dyp * (psi[yp] - pc) - dym * (pc - psi[ym])) * ihy2;
-  const T rz = (dzp * (psi[zp] - pc) - dzm * (pc - psi[zm])) * ihz2;
-  out[c] = rx + ry + rz;
+struct OpCell {
+  static constexpr int R = 1, NF = 2;
+  T ihx2, ihy2, ihz2;
+  template <class A>
+  __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
+    const T pc = F(0, 0, 0, 0);
+    const T Dc = F(1, 0, 0, 0);
+    const T dxp = hmean(Dc, F(1, 1, 0, 0));
+    const T dxm = hmean(F(1, -1, 0, 0), Dc);
+    const T dyp = hmean(Dc, F(1, 0, 1, 0));
+    const T dym = hmean(F(1, 0, -1, 0), Dc);
+    const T dzp = hmean(Dc, F(1, 0, 0, 1));
+    const T dzm = hmean(F(1, 0, 0, -1), Dc);
+    const T rx = (dxp * (F(0, 1, 0, 0) - pc) - dxm * (pc - F(0, -1, 0, 0))) * ihx2;
+    const T ry = (dyp * (F(0, 0, 1, 0) - pc) - dym * (pc - F(0, 0, -1, 0))) * ihy2;
+    const T rz = (dzp * (F(0, 0, 0, 1) - pc) - dzm * (pc - F(0, 0, 0, -1))) * ihz2;
+    out[c] = rx + ry + rz;
+  }
+};
+
+constexpr int kZOX = 64, kZOY = 8, kZLen = 32;
+
+template <class T, class Op>
+void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, const T* f1,
+                   T* out) {
+  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + kZOY - 1) / kZOY;
+  const int nzc = (d.nz + kZLen - 1) / kZLen;
+  hipLaunchKernelGGL((k_zmarch<T, kZOX, kZOY, Op>), dim3((unsigned)(ntx * nty * nzc)), dim3(256),
+                     0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen);
 }
 
 template <class T>
@@ -269,14 +378,11 @@ inline Inv<double> inv_h2(const gc_grid* g) {
 template <class T>
 static gc_status run_laplacian(gc_context* ctx, const gc_grid* g, int order, const T* in, T* out) {
   const Dims d = dims_of(g);
-  const Launch L = launch_for(d);
   const Inv<double> ih = inv_h2(g);
   if (order == 2)
-    hipLaunchKernelGGL((k_laplacian<T, 2>), L.grid, L.block, 0, ctx->stream, in, out, d, L.nbx,
-                       L.nby, T(ih.x), T(ih.y), T(ih.z));
+    zmarch_launch<T>(ctx, d, OpLap<T, 2>{T(ih.x), T(ih.y), T(ih.z)}, in, nullptr, out);
   else
-    hipLaunchKernelGGL((k_laplacian<T, 4>), L.grid, L.block, 0, ctx->stream, in, out, d, L.nbx,
-                       L.nby, T(ih.x), T(ih.y), T(ih.z));
+    zmarch_launch<T>(ctx, d, OpLap<T, 4>{T(ih.x), T(ih.y), T(ih.z)}, in, nullptr, out);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -313,8 +419,7 @@ void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void
   const T x = T(ih.x), y = T(ih.y), z = T(ih.z);
   switch (which) {
     case 0:
-      hipLaunchKernelGGL((k_gradient<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
-                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      zmarch_launch<T>(ctx, d, OpGrad<T, O>{x, y, z}, (const T*)in, nullptr, (T*)out);
       break;
     case 1:
       hipLaunchKernelGGL((k_gradient_vec<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
@@ -349,12 +454,12 @@ gc_status first_deriv(int which, gc_context* ctx, const gc_grid* g, gc_dtype dt,
 template <class T, int O>
 void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
   const Dims d = dims_of(g);
-  const Launch L = launch_for(d);
   const Inv<double> i2 = inv_h2(g);
   // inv_h4 = inv_h2 * inv_h2 in double (Biharmonic.hpp:62-64), then cast
-  hipLaunchKernelGGL((k_biharmonic<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
-                     (T*)out, d, L.nbx, L.nby, T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x),
-                     T(i2.y * i2.y), T(i2.z * i2.z));
+  zmarch_launch<T>(ctx, d,
+                   OpBih<T, O>{T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x), T(i2.y * i2.y),
+                               T(i2.z * i2.z)},
+                   (const T*)in, nullptr, (T*)out);
 }
 
 }  // namespace
@@ -363,16 +468,13 @@ namespace gc {
 gc_status launch_cell_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* psi,
                                 const void* D, void* out) {
   const Dims d = dims_of(g);
-  const Launch L = launch_for(d);
   const Inv<double> ih = inv_h2(g);
   if (dt == GC_F64)
-    hipLaunchKernelGGL((k_cell_laplacian<double>), L.grid, L.block, 0, ctx->stream,
-                       (const double*)psi, (const double*)D, (double*)out, d, L.nbx, L.nby, ih.x,
-                       ih.y, ih.z);
+    zmarch_launch<double>(ctx, d, OpCell<double>{ih.x, ih.y, ih.z}, (const double*)psi,
+                          (const double*)D, (double*)out);
   else
-    hipLaunchKernelGGL((k_cell_laplacian<float>), L.grid, L.block, 0, ctx->stream,
-                       (const float*)psi, (const float*)D, (float*)out, d, L.nbx, L.nby,
-                       float(ih.x), float(ih.y), float(ih.z));
+    zmarch_launch<float>(ctx, d, OpCell<float>{float(ih.x), float(ih.y), float(ih.z)},
+                         (const float*)psi, (const float*)D, (float*)out);
   GC_LAUNCHED();
   return GC_OK;
 }

@@ -242,14 +242,16 @@ gc_status gc_ensemble_diagnostics(gc_ensemble* e, int method, double* mass, doub
  * No counterpart in the reference.  A 6-layer 3x3 periodic CNN (1 -> 32 x5 -> 1 channels,
  * GELU, residual output) that maps a 2-D state c(t) to c(t + 16 dt):
  *   c_new = c + scale * (out - mean(out))      (the mean removal conserves mass)
- * BF16 weights and activations, FP32 accumulation; the 32 -> 32 layers run on the matrix
- * cores.  Parameters: one FP32 vector of gc_cnn_param_count() values in the order
+ * The 32 -> 32 layers run on the matrix cores with BF16 weights and activations; the
+ * 1 -> 32 and 32 -> 1 layers take the state and their weights in FP32; every accumulation
+ * is FP32.  Parameters: one FP32 vector of gc_cnn_param_count() values in the order
  * W1[co][tap] b1 {W_l[co][tap][ci] b_l}x4 W6[tap][ci] b6 (tap = 3(dy+1) + (dx+1)).
  * Images are H x W with H a multiple of 8 and W a multiple of 32. */
 
 typedef struct gc_cnn gc_cnn;
 
 int gc_cnn_param_count(void);
+/* The network runs on ctx's stream; destroy it before ctx. */
 gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** out);
 gc_status gc_cnn_destroy(gc_cnn* n);
 /* Host FP32 parameters and the output scale; resets the optimiser state. */
@@ -270,6 +272,18 @@ gc_status gc_cnn_loss_grad(gc_cnn* n, gc_dtype dt, const void* c, const void* ta
 gc_status gc_cnn_adam(gc_cnn* n, double lr, double beta1, double beta2, double eps);
 /* c_new = c + scale*(out - mean(out)) for B device images (c_new may not alias c). */
 gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int B);
+
+/* Advances a 2-D Cahn-Hilliard state (grid H x W x 1, matching the network) by n_macro
+ * macro steps of `substeps` RK4 steps of size `step`.  Each macro step takes the
+ * network's proposal unless the state's mean drifts from its initial value by more than
+ * mass_tol or the free energy increases; then it recomputes the macro step with
+ * gc_ch_integrate (RK4).  n_fallback (may be NULL) receives the number of recomputed
+ * macro steps.  psi is updated in place.  Proposals are made up to 16 macro steps ahead
+ * of the host's decisions (one synchronisation per 16 macro steps); the result is that of
+ * deciding after every macro step. */
+gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
+                               void* psi, double step, int substeps, int n_macro,
+                               double mass_tol, int* n_fallback);
 
 /* Element-wise conversion between GC_F64 and GC_F32 device arrays. */
 gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const void* src,

@@ -58,12 +58,11 @@ def shift(x, dy, dx):
 
 def forward(p, x, keep=False, q=None):
     """x: (B, H, W) network input.  Returns out (B, H, W) and, with keep, the cache.
-    q, if given, is applied to the input, the weights and every activation (e.g.
-    bf16_round, to model a device that stores those in bfloat16)."""
+    q, if given, is applied to the weights of the four 32 -> 32 layers and to every
+    activation (e.g. bf16_round, to model a device that stores those in bfloat16 and keeps
+    the input, W1 and W6 in float32)."""
     W1, b1, mids, W6, b6 = split(p)
     if q is not None:
-        x = q(x)
-        W1, W6 = q(W1), q(W6)
         mids = [(q(W), b) for W, b in mids]
     act = (lambda z: q(gelu(z))) if q is not None else gelu
     z1 = np.zeros(x.shape + (C,))
@@ -93,7 +92,6 @@ def backward(p, cache, dout, q=None):
     applied here to the weights and to every pre-activation gradient)."""
     W1, b1, mids, W6, b6 = split(p)
     if q is not None:
-        W1, W6 = q(W1), q(W6)
         mids = [(q(W), b) for W, b in mids]
     qg = q if q is not None else (lambda a: a)
     x, zs, acts = cache

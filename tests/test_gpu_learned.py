@@ -1,0 +1,169 @@
+"""Learned time stepping with the plain-RK4 fallback (gc_learned_integrate), on the GPU.
+
+The acceptance rule: a macro step's proposal is kept unless the free energy rises or the mean
+drifts by more than mass_tol; otherwise the macro step is recomputed by the fused RK4, so a
+run in which every proposal is rejected is bit-identical to plain RK4, and a network that
+proposes no change leaves the state untouched.  The trained-network tests show what the
+network gains over the method without it (c(t + 16 dt) ~ c(t)) on held-out trajectories."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = W = 256
+DT = 0.01
+
+
+def _ic(gc, seed, dtype=np.float64):
+    return (0.05 * gc.inputs.symmetric(seed, H * W, 1.0)).reshape(1, H, W).astype(dtype)
+
+
+def _zero_net(gc):
+    return np.zeros(gc.N_CNN_PARAMS, np.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_all_rejected_is_plain_rk4(gc, ctx, dtype):
+    """Random weights with a large output scale raise the free energy at every macro step:
+    every step falls back, and the result equals plain RK4 bit for bit."""
+    g = gc.Grid.make(W, H, 1)
+    rng = np.random.default_rng(0)
+    p = rng.uniform(-0.5, 0.5, gc.N_CNN_PARAMS).astype(np.float32)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(p, 1.0)
+    c0 = _ic(gc, 11, dtype)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    b = gc.DeviceArray.from_numpy(ctx, c0)
+    nf = net.integrate(g, a, 1.0, 1.0, DT, 16, 5)
+    gc.ch_integrate(ctx, g, b, 1.0, 1.0, DT, 80, gc.RK4)
+    assert nf == 5
+    assert np.array_equal(a.numpy(ctx), b.numpy(ctx))
+    net.close()
+
+
+def test_zero_network_keeps_state(gc, ctx):
+    """A network with all-zero parameters proposes c + 0: accepted (F unchanged, no drift)
+    and the state is unchanged."""
+    g = gc.Grid.make(W, H, 1)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(_zero_net(gc), 1e-2)
+    c0 = _ic(gc, 12)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    nf = net.integrate(g, a, 1.0, 1.0, DT, 16, 4)
+    assert nf == 0
+    assert np.array_equal(a.numpy(ctx), c0)
+    net.close()
+
+
+def test_bias_only_network_is_mass_free(gc, ctx):
+    """A constant network output is removed by the mean projection: no change either."""
+    g = gc.Grid.make(W, H, 1)
+    p = _zero_net(gc)
+    p[-1] = 3.0  # b6
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(p, 1e-2)
+    c0 = _ic(gc, 13)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    assert net.integrate(g, a, 1.0, 1.0, DT, 16, 3) == 0
+    assert np.abs(a.numpy(ctx) - c0).max() < 1e-9
+    net.close()
+
+
+def test_mass_tolerance_forces_fallback(gc, ctx):
+    """With mass_tol < 0 no proposal can pass: every macro step is plain RK4."""
+    g = gc.Grid.make(W, H, 1)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(_zero_net(gc), 1e-2)
+    c0 = _ic(gc, 14)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    b = gc.DeviceArray.from_numpy(ctx, c0)
+    assert net.integrate(g, a, 1.0, 1.0, DT, 16, 3, mass_tol=-1.0) == 3
+    gc.ch_integrate(ctx, g, b, 1.0, 1.0, DT, 48, gc.RK4)
+    assert np.array_equal(a.numpy(ctx), b.numpy(ctx))
+    net.close()
+
+
+def test_mixed_decisions_match_stepwise_rule(gc, ctx):
+    """A random network with a tiny output scale proposes c + O(1e-6) changes that sometimes
+    raise the free energy: accepted and rejected macro steps interleave across several
+    16-step proposal chunks, so a rejection inside a chunk discards the proposals after it.
+    The result must equal a host loop that decides after every macro step (free energy and
+    mass from the library's reductions)."""
+    g = gc.Grid.make(W, H, 1)
+    rng = np.random.default_rng(5)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(rng.uniform(-0.3, 0.3, gc.N_CNN_PARAMS).astype(np.float32), 1e-6)
+    c0 = _ic(gc, 15)
+    n_macro, tol = 48, 1e-6
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    nf = net.integrate(g, a, 1.0, 1.0, DT, 16, n_macro, tol)
+
+    psi = gc.DeviceArray.from_numpy(ctx, c0)
+    prop = gc.DeviceArray.from_numpy(ctx, c0)
+    F0 = gc.ch_free_energy(ctx, g, psi, 1.0, gc.KAHAN)
+    m_init = gc.integrate(ctx, g, psi, gc.KAHAN)
+    want_nf = 0
+    for _ in range(n_macro):
+        net.predict(psi, prop, 1)
+        F1 = gc.ch_free_energy(ctx, g, prop, 1.0, gc.KAHAN)
+        m1 = gc.integrate(ctx, g, prop, gc.KAHAN)
+        if np.isfinite(F1) and F1 <= F0 and abs(m1 - m_init) / (H * W) <= tol:
+            psi.copy_from(ctx, prop)
+            F0 = F1
+        else:
+            want_nf += 1
+            gc.ch_integrate(ctx, g, psi, 1.0, 1.0, DT, 16, gc.RK4)
+            F0 = gc.ch_free_energy(ctx, g, psi, 1.0, gc.KAHAN)
+    print(f"fallbacks {nf} of {n_macro}")
+    assert 1 <= want_nf <= n_macro - 1
+    assert nf == want_nf
+    assert np.array_equal(a.numpy(ctx), psi.numpy(ctx))
+    net.close()
+
+
+def test_rejects_wrong_grid(gc, ctx):
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(_zero_net(gc), 1e-2)
+    a = gc.DeviceArray((1, 128, 256), np.float64)
+    with pytest.raises(gc.GridcalcError, match="corrector was built"):
+        net.integrate(gc.Grid.make(256, 128, 1), a, 1.0, 1.0, DT, 16, 1)
+    net.close()
+
+
+@pytest.fixture(scope="module")
+def trained(gc):
+    from gridcalc_amd import corrector as cr
+    p, meta = cr.load_params(H, W)
+    if p is None:
+        pytest.skip("no trained corrector parameters in the package data")
+    return cr, p, meta
+
+
+def test_trained_network_beats_identity_one_step(gc, ctx, trained):
+    """Held-out trajectories (seeds never used in training): the trained network's one
+    macro-step error is well below that of the method without it, and below the untrained
+    network's."""
+    cr, p, meta = trained
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(p, meta["scale"])
+    e_net, e_id = cr.one_step_errors(net, n_traj=2, log=lambda *a: None)
+    net.set_params(cr.init_params(cr.CONFIG["seed"]), meta["scale"])
+    e_init, _ = cr.one_step_errors(net, n_traj=2, log=lambda *a: None)
+    net.close()
+    print(f"one macro step rel L2: trained {e_net:.3e}, identity {e_id:.3e}, untrained {e_init:.3e}")
+    assert e_net < 0.02 * e_id
+    assert e_net < 0.02 * e_init
+
+
+def test_trained_learned_integration(gc, ctx, trained):
+    """T = 20 from a held-out initial state: mass is conserved to the tolerance, and the
+    learned path stays within 2 % (relative L2) of the FP64 RK4 solution."""
+    cr, p, meta = trained
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(p, meta["scale"])
+    r = cr.integration_error(net, n_steps=2000)
+    net.close()
+    print("learned integration to T=20:", r)
+    assert r["mass_drift"] <= 1e-6
+    assert r["rel_l2"] < 2e-2
+    assert r["fallbacks"] < r["n_macro"]

@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# (nx, ny, nz, h): cubic, anisotropic/ragged, 2-D, 1-D strip, degenerate x
+# (nx, ny, nz, h): cubic, anisotropic/ragged, 2-D, 1-D strip, degenerate x, multi-tile
 SHAPES = [
     (16, 16, 16, (1.0, 1.0, 1.0)),
     (17, 12, 9, (0.3, 0.17, 0.5)),
@@ -19,6 +19,7 @@ SHAPES = [
     (70, 1, 1, (0.1, 1.0, 1.0)),
     (1, 13, 11, (1.0, 0.2, 0.3)),
     (130, 7, 8, (1.0 / 130, 1.0 / 7, 1.0 / 8)),
+    (72, 20, 70, (0.5, 0.25, 0.125)),  # several xy tiles and z chunks, ragged last chunk
 ]
 
 
