@@ -133,6 +133,7 @@ def lib():
         "gc_diffuse_field": ([vp, gp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int, i64p], st),
         "gc_ch_config_count": ([gp], C.c_int),
         "gc_ch_config_shape": ([gp, C.c_int, ip, ip, ip], st),
+        "gc_ch_config_fused": ([gp, C.c_int], C.c_int),
         "gc_ch_set_config": ([gp, C.c_int, C.c_int], st),
         "gc_ch_get_config": ([gp, C.c_int, ip], st),
         "gc_ch_tune": ([vp, gp, C.c_int, ip, C.POINTER(C.c_float)], st),
@@ -430,6 +431,14 @@ def ch_config_shape(g, i):
     ox, oy, zc = C.c_int(), C.c_int(), C.c_int()
     _check(lib().gc_ch_config_shape(C.byref(g), i, C.byref(ox), C.byref(oy), C.byref(zc)))
     return ox.value, oy.value, zc.value
+
+
+def ch_config_fused(g, i):
+    """True if 3-D candidate i runs RK4 as two two-stage kernels."""
+    r = lib().gc_ch_config_fused(C.byref(g), i)
+    if r < 0:
+        raise GridcalcError(1, f"gridcalc: config id {i} out of range")
+    return bool(r)
 
 
 def ch_set_config(g, dtype, i):

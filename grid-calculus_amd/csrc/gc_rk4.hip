@@ -18,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 namespace gc {
 
@@ -368,8 +369,9 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __res
 
   // Iteration p stores y plane z0-2+p into ring slot p%3, computes mu plane z0+p-3 from y
   // planes p-2..p, then the output plane z0+p-4 from mu planes p-2..p.  The two barriers
-  // separate the y store from the mu pass and the mu pass from the output pass; every slot
-  // written in iteration p was last read before the first barrier of iteration p.
+  // separate the y store from the mu pass and the mu pass from the output pass.  A y slot
+  // written in iteration p was last read before the second barrier of iteration p-1, and a
+  // mu slot before the first barrier of iteration p; the output pass reads no y slot.
   load_y(z0 - 2);
   const int last = nout + 3;
   for (int p = 0; p <= last; ++p) {
@@ -386,6 +388,15 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __res
         if (oin[q]) stage_in<STAGE>(psi + kb, a + kb, b + kb, ooff[q], pv[q], av[q], bv[q]);
     }
     __syncthreads();
+    // S1/Euler: the base value of output plane z0+p-4 is y plane p-2 (slot s2), which the
+    // store of iteration p+1 overwrites; read it before the second barrier.
+    if constexpr (base_from_y<STAGE>()) {
+      if (p >= 4) {
+#pragma unroll
+        for (int q = 0; q < C::OR; ++q)
+          if (oin[q]) pv[q] = ys[s2][(oy0 + q * ROWS + 2) * C::YW + (ox + 2)];
+      }
+    }
     if (p >= 2) {  // mu plane z0+p-3 from y planes p-2, p-1, p
       const T* yc = ys[s1];
       const T* yb = ys[s2];
@@ -415,7 +426,6 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __res
           const int oy = oy0 + q * ROWS;
           const int o = (oy + 1) * C::MW + (ox + 1);
           const T kv = lap_lds(mc, mb, ma, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
-          if constexpr (base_from_y<STAGE>()) pv[q] = ys[s2][(oy + 2) * C::YW + (ox + 2)];
           stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi + kb, ynext + kb, a + kb,
                            b + kb, co);
         }
@@ -496,6 +506,207 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int st
   return GC_OK;
 }
 
+// ---- two RK4 stages per pass (3-D) ---------------------------------------------------------
+//
+// PAIR 0 runs S1+S2, PAIR 1 runs S3+S4.  The first stage of a pair is evaluated on the tile
+// plus a 2-site halo, so its stage value y' = base + c*k stays in LDS and the second stage
+// runs on the tile from it; the stencil input is read with a 4-site halo.  Per site and
+// step: PAIR 0 reads psi and writes k1, k2 and y3 (4 W); PAIR 1 reads y3, psi, k1 and k2 and
+// writes the new state into the other state buffer (5 W) -- 9 W per step instead of 17:
+//   PAIR 0: y = psi; k1 on tile+2; y2 = psi + dt/2 k1 (LDS); k2 on tile;
+//           a = k1, b = k2, out = psi + dt/2 k2
+//   PAIR 1: y = y3; k3 on tile+2; y4 = psi + dt k3 (LDS); k4 on tile;
+//           out = psi + (dt/6 (k1 + k4) + dt/3 (k2 + k3))
+// Every value is formed with the single-stage kernels' arithmetic, so the results are the
+// same bits.  PAIR 1 must not write psi in place: neighbouring tiles and z-chunks still read
+// it as their halo.
+//
+// Iteration p of the z-march stores stencil-input plane z0-4+p, forms mu_A on plane z0-5+p
+// (tile+3), k_A and y' on plane z0-6+p (tile+2), mu_B on plane z0-7+p (tile+1) and the
+// outputs on plane z0-8+p.  Each quantity lives in a 3-plane LDS ring; four barriers per
+// iteration separate the passes, and every ring slot written in a pass was last read before
+// the preceding barrier.
+
+template <int OX, int OY, int NT>
+struct PairCfg {
+  static constexpr int W4 = OX + 8, H4 = OY + 8, N4 = W4 * H4;  // stencil input
+  static constexpr int W3 = OX + 6, H3 = OY + 6, N3 = W3 * H3;  // mu of stage A
+  static constexpr int W2 = OX + 4, H2 = OY + 4, N2 = W2 * H2;  // k_A and y'
+  static constexpr int W1 = OX + 2, H1 = OY + 2, N1 = W1 * H1;  // mu of stage B
+  static constexpr int L4 = (N4 + NT - 1) / NT, L3 = (N3 + NT - 1) / NT;
+  static constexpr int L2 = (N2 + NT - 1) / NT, L1 = (N1 + NT - 1) / NT;
+  static constexpr int OR = OX * OY / NT;  // output sites per thread
+  static constexpr int ROWS = NT / OX;
+  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
+};
+
+template <class T, int OX, int OY, int ZC, int NT, int PAIR>
+__global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restrict__ y,
+                                                  const T* __restrict__ psi,
+                                                  T* __restrict__ out, T* __restrict__ a,
+                                                  T* __restrict__ b, Dims d, int ntx, int nty,
+                                                  StageCoef<T> co) {
+  using C = PairCfg<OX, OY, NT>;
+  __shared__ T ys[3][C::N4];
+  __shared__ T mas[3][C::N3];
+  __shared__ T y2s[3][C::N2];
+  __shared__ T mbs[3][C::N1];
+  __shared__ T kq[PAIR == 1 ? 3 : 1][PAIR == 1 ? OX * OY : 1];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned r = bid / (unsigned)ntx;
+  const int ty = (int)(r % (unsigned)nty);
+  const int zc = (int)(r / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
+  const int nout = min(ZC, d.nz - z0);
+
+  unsigned yoff[C::L4];
+#pragma unroll
+  for (int s = 0; s < C::L4; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % C::W4, py = idx / C::W4;
+    yoff[s] = idx < C::N4 ? (unsigned)(wrap_any(x0 + px - 4, d.nx) + d.nx * wrap_any(y0 + py - 4, d.ny))
+                          : 0u;
+  }
+  // PAIR 1: psi at the tile+2 sites of stage A
+  unsigned eoff[PAIR == 1 ? C::L2 : 1];
+  if constexpr (PAIR == 1) {
+#pragma unroll
+    for (int s = 0; s < C::L2; ++s) {
+      const int idx = t + s * NT;
+      const int px = idx % C::W2, py = idx / C::W2;
+      eoff[s] = idx < C::N2 ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
+                            : 0u;
+    }
+  }
+  const int ox = t % OX, oy0 = t / OX;
+  unsigned ooff[C::OR];
+  bool oin[C::OR];
+#pragma unroll
+  for (int q = 0; q < C::OR; ++q) {
+    const int gy = y0 + oy0 + q * C::ROWS;
+    oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
+    ooff[q] = oin[q] ? (unsigned)(x0 + ox + d.nx * gy) : 0u;
+  }
+
+  T yreg[C::L4];
+  auto load_y = [&](int plane) {
+    const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < C::L4; ++s)
+      if (t + s * NT < C::N4) yreg[s] = yp[yoff[s]];
+  };
+  T pe[PAIR == 1 ? C::L2 : 1];
+  T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+
+  load_y(z0 - 4);
+  const int last = nout + 7;
+  for (int p = 0; p <= last; ++p) {
+    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // slots of p, p-1, p-2
+#pragma unroll
+    for (int s = 0; s < C::L4; ++s)
+      if (t + s * NT < C::N4) ys[s0][t + s * NT] = yreg[s];
+    if (p < last) load_y(z0 - 3 + p);
+    if constexpr (PAIR == 1) {
+      if (p >= 4) {
+        const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 6 + p, d.nz);
+#pragma unroll
+        for (int s = 0; s < C::L2; ++s)
+          if (t + s * NT < C::N2) pe[s] = pp[eoff[s]];
+      }
+    }
+    if (p >= 8) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q) {
+        if (oin[q]) {
+          pv[q] = __builtin_nontemporal_load(psi + kb + ooff[q]);
+          if constexpr (PAIR == 1) {
+            av[q] = __builtin_nontemporal_load(a + kb + ooff[q]);
+            bv[q] = __builtin_nontemporal_load(b + kb + ooff[q]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 2) {  // mu_A on plane z0-5+p, tile+3
+#pragma unroll
+      for (int s = 0; s < C::L3; ++s) {
+        const int idx = t + s * NT;
+        if (idx < C::N3) {
+          const int mx = idx % C::W3, my = idx / C::W3;
+          const int o = (my + 1) * C::W4 + (mx + 1);
+          const T l = lap_lds(ys[s1], ys[s2], ys[s0], o, C::W4, cf.ihx2, cf.ihy2, cf.ihz2);
+          const T yy = ys[s1][o];
+          mas[s0][idx] = -yy + yy * yy * yy - cf.kappa * l;
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 4) {  // k_A and y' on plane z0-6+p, tile+2
+      const int zA = z0 - 6 + p;
+      const bool own = zA >= z0 && zA < z0 + nout;
+#pragma unroll
+      for (int s = 0; s < C::L2; ++s) {
+        const int idx = t + s * NT;
+        if (idx < C::N2) {
+          const int mx = idx % C::W2, my = idx / C::W2;
+          const int o = (my + 1) * C::W3 + (mx + 1);
+          const T kv = lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const int lx = mx - 2, ly = my - 2;  // tile coordinates
+          const bool in_tile = lx >= 0 && lx < OX && ly >= 0 && ly < OY;
+          if constexpr (PAIR == 0) {
+            const T base = ys[s2][(my + 2) * C::W4 + (mx + 2)];
+            y2s[s0][idx] = base + co.half_dt * kv;
+            if (own && in_tile && x0 + lx < d.nx && y0 + ly < d.ny)
+              __builtin_nontemporal_store(
+                  kv, a + (size_t)d.plane * (size_t)zA + (size_t)(x0 + lx) + (size_t)d.nx * (y0 + ly));
+          } else {
+            y2s[s0][idx] = pe[s] + co.dt * kv;
+            if (in_tile) kq[s0][ly * OX + lx] = kv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 6) {  // mu_B on plane z0-7+p, tile+1
+#pragma unroll
+      for (int s = 0; s < C::L1; ++s) {
+        const int idx = t + s * NT;
+        if (idx < C::N1) {
+          const int mx = idx % C::W1, my = idx / C::W1;
+          const int o = (my + 1) * C::W2 + (mx + 1);
+          const T l = lap_lds(y2s[s1], y2s[s2], y2s[s0], o, C::W2, cf.ihx2, cf.ihy2, cf.ihz2);
+          const T yy = y2s[s1][o];
+          mbs[s0][idx] = -yy + yy * yy * yy - cf.kappa * l;
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 8) {  // outputs on plane z0-8+p
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q) {
+        if (oin[q]) {
+          const int oy = oy0 + q * C::ROWS;
+          const int o = (oy + 1) * C::W1 + (ox + 1);
+          const T kv = lap_lds(mbs[s1], mbs[s2], mbs[s0], o, C::W1, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          if constexpr (PAIR == 0) {
+            __builtin_nontemporal_store(kv, b + kb + ooff[q]);
+            __builtin_nontemporal_store(pv[q] + co.half_dt * kv, out + kb + ooff[q]);
+          } else {
+            const T k3 = kq[s2][oy * OX + ox];
+            const T bsum = bv[q] + k3;
+            __builtin_nontemporal_store(pv[q] + (co.sixth_dt * (av[q] + kv) + co.third_dt * bsum),
+                                        out + kb + ooff[q]);
+          }
+        }
+      }
+    }
+  }
+}
+
 // ---- tile shapes of the Cahn-Hilliard stage kernels and their selection ------------------
 //
 // Candidate shapes (tile width OX, tile height OY, z-chunk ZC).  The default is used until
@@ -516,6 +727,15 @@ constexpr int kDefault3F32 = 6, kDefault3F64 = 4, kDefault2 = 2;
 
 // Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
 inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
+
+// 3-D two-stage (fused pair) shapes: ids kN3 .. kN3 + kNP - 1; block size NT threads.
+struct PairTile {
+  int ox, oy, zc, nt;
+};
+constexpr PairTile kChPairCfgs[] = {{64, 8, 32, 512}, {64, 8, 64, 512}, {32, 16, 32, 512},
+                                    {32, 16, 64, 512}};
+constexpr int kNP = sizeof(kChPairCfgs) / sizeof(kChPairCfgs[0]);
+inline bool is_pair(const Dims& d, int id) { return d.nz > 1 && id >= kN3; }
 
 #define GC_STAGE_SWITCH(LAUNCH)      \
   switch (stage) {                   \
@@ -584,6 +804,30 @@ void launch_ch_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf, 
 #undef GC_CH2
 }
 
+template <class T, int OX, int OY, int ZC, int NT>
+void launch_pair3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int pair, const T* y,
+                  const T* psi, T* out, T* a, T* b, const StageCoef<T>& co) {
+  const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY, nzc = (d.nz + ZC - 1) / ZC;
+  const dim3 grid((unsigned)(ntx * nty * nzc));
+  if (pair == 0)
+    hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 0>), grid, dim3(NT), 0, ctx->stream, cf, y,
+                       psi, out, a, b, d, ntx, nty, co);
+  else
+    hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 1>), grid, dim3(NT), 0, ctx->stream, cf, y,
+                       psi, out, a, b, d, ntx, nty, co);
+}
+
+template <class T>
+void launch_pair_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf, int pair,
+                     const T* y, const T* psi, T* out, T* a, T* b, const StageCoef<T>& co) {
+  switch (id - kN3) {
+    case 0: launch_pair3<T, 64, 8, 32, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 1: launch_pair3<T, 64, 8, 64, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 2: launch_pair3<T, 32, 16, 32, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    default: launch_pair3<T, 32, 16, 64, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+  }
+}
+
 // Process-wide choice of tile shape per (extents, dtype).
 struct CfgKey {
   int nx, ny, nz, dt;
@@ -608,7 +852,8 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
   const Dims d = dims_of(g);
   const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
   if (ch_tiled(d)) {
-    const int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
+    int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
+    if (is_pair(d, id)) id = sizeof(T) == 8 ? kDefault3F64 : kDefault3F32;  // one stage only
     launch_ch_cfg<T>(ctx, id, d, cf, stage, y, psi, ynext, a, b, co);
   } else {
     const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;
@@ -616,6 +861,19 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
                        dim3(TX * TY), 0, ctx->stream, rhs, stage, y, psi, ynext, a, b, d, nbx,
                        nby, co);
   }
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// One RK4 step with the two-stage kernels: psi_in -> psi_out (distinct buffers).
+template <class T>
+gc_status pair_step(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, int id,
+                    const T* psi_in, T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
+  const Dims d = dims_of(g);
+  const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
+  launch_pair_cfg<T>(ctx, id, d, cf, 0, psi_in, psi_in, y3, a, b, co);
+  GC_LAUNCHED();
+  launch_pair_cfg<T>(ctx, id, d, cf, 1, y3, psi_in, psi_out, a, b, co);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -639,6 +897,23 @@ gc_status run_steps(gc_context* ctx, const gc_grid* g, const Rhs& rhs, T* psi, d
     T* yB = yA + n;
     T* a = yB + n;
     T* b = a + n;
+    if constexpr (std::is_same<Rhs, CHRhs<T>>::value) {
+      const Dims d = dims_of(g);
+      const int id = ch_tiled(d) ? ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32) : -1;
+      if (id >= 0 && is_pair(d, id)) {  // two stages per pass; yA holds the other state
+        T* cur = psi;
+        T* other = yA;
+        for (int s = 0; s < n_steps; ++s) {
+          GC_TRY(pair_step<T>(ctx, g, rhs, id, cur, other, yB, a, b, co));
+          T* tmp = cur;
+          cur = other;
+          other = tmp;
+        }
+        if (cur != psi)
+          GC_HIP(hipMemcpyAsync(psi, cur, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+        return GC_OK;
+      }
+    }
     for (int s = 0; s < n_steps; ++s) {
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS1, psi, psi, yA, a, b, co));
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS2, yA, psi, yB, a, b, co));
@@ -856,7 +1131,7 @@ namespace {
 template <class T>
 gc_status tune_ch(gc_context* ctx, const gc_grid* g, int* best, float* ms_out) {
   const Dims d = dims_of(g);
-  const int ncfg = d.nz == 1 ? kN2 : kN3;
+  const int ncfg = d.nz == 1 ? kN2 : kN3 + kNP;
   const size_t n = sites(g);
   void* scr = nullptr;
   GC_TRY(scratch(ctx, 5 * n * sizeof(T), &scr));
@@ -873,8 +1148,15 @@ gc_status tune_ch(gc_context* ctx, const gc_grid* g, int* best, float* ms_out) {
   const int steps = (int)std::max<size_t>(2, std::min<size_t>(64, (size_t(1) << 24) / n));
   float best_ms = 0.0f;
   int best_id = -1;
+  T* bufs[2] = {psi, yA};
+  int cur = 0;
   for (int id = 0; id < ncfg; ++id) {
     auto step = [&]() -> gc_status {
+      if (is_pair(d, id)) {
+        GC_TRY(pair_step<T>(ctx, g, rhs, id, bufs[cur], bufs[cur ^ 1], yB, a, b, co));
+        cur ^= 1;
+        return GC_OK;
+      }
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS1, psi, psi, yA, a, b, co, id));
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS2, yA, psi, yB, a, b, co, id));
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS3, yB, psi, yA, a, b, co, id));
@@ -916,7 +1198,12 @@ extern "C" {
 
 int gc_ch_config_count(const gc_grid* g) {
   if (!g || check_grid(g) != GC_OK || !ch_tiled(dims_of(g))) return 0;
-  return g->nz == 1 ? kN2 : kN3;
+  return g->nz == 1 ? kN2 : kN3 + kNP;
+}
+
+int gc_ch_config_fused(const gc_grid* g, int id) {
+  if (!g || id < 0 || id >= gc_ch_config_count(g)) return -1;
+  return is_pair(dims_of(g), id) ? 1 : 0;
 }
 
 gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc) {
@@ -924,6 +1211,13 @@ gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc
   const int n = gc_ch_config_count(g);
   if (id < 0 || id >= n || !ox || !oy || !zc)
     return set_error(GC_ERR_INVALID, "gridcalc: config id %d out of range [0, %d)", id, n);
+  if (is_pair(dims_of(g), id)) {
+    const PairTile c = kChPairCfgs[id - kN3];
+    *ox = c.ox;
+    *oy = c.oy;
+    *zc = c.zc;
+    return GC_OK;
+  }
   const TileCfg c = g->nz == 1 ? kCh2Cfgs[id] : kCh3Cfgs[id];
   *ox = c.ox;
   *oy = c.oy;

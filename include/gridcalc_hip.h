@@ -188,11 +188,16 @@ gc_status gc_diffuse_field(gc_context* ctx, const gc_grid* g, gc_dtype dt, void*
 int gc_ch_config_count(const gc_grid* g);
 /* Tile shape of candidate `id`. */
 gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc);
+/* 1 if 3-D candidate `id` runs RK4 as two kernels of two stages each (S1+S2, S3+S4: 9
+ * words of HBM traffic per site and step instead of 17; same results bit for bit), 0 for
+ * one kernel per stage, -1 for an invalid id.  Euler and single right-hand-side
+ * evaluations always use one-stage kernels. */
+int gc_ch_config_fused(const gc_grid* g, int id);
 /* Forces candidate `id` for (extents, dtype); -1 restores the built-in default. */
 gc_status gc_ch_set_config(const gc_grid* g, gc_dtype dt, int id);
 /* Candidate currently used for (extents, dtype). */
 gc_status gc_ch_get_config(const gc_grid* g, gc_dtype dt, int* id);
-/* Times one RK4 step (4 stage kernels) of every candidate on scratch data, keeps the
+/* Times one RK4 step (4 or 2 kernels) of every candidate on scratch data, keeps the
  * fastest for (extents, dtype) and returns its id; if ms is not NULL it receives the
  * measured ms per step of each candidate (gc_ch_config_count entries). */
 gc_status gc_ch_tune(gc_context* ctx, const gc_grid* g, gc_dtype dt, int* best, float* ms);
