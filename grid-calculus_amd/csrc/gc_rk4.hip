@@ -523,9 +523,11 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int st
 //
 // Iteration p of the z-march stores stencil-input plane z0-4+p, forms mu_A on plane z0-5+p
 // (tile+3), k_A and y' on plane z0-6+p (tile+2), mu_B on plane z0-7+p (tile+1) and the
-// outputs on plane z0-8+p.  Each quantity lives in a 3-plane LDS ring; four barriers per
-// iteration separate the passes, and every ring slot written in a pass was last read before
-// the preceding barrier.
+// outputs on plane z0-8+p.  Each of y, mu_A, y', mu_B lives in a 3-plane LDS ring; four
+// barriers per iteration separate the passes, and every ring slot written in a pass was last
+// read before the preceding barrier.  k_A on the tile is computed by the thread that owns
+// the output site, which keeps it in registers for the two iterations until PAIR 1 needs
+// it; the 2-site ring around the tile is spread over the block.
 
 template <int OX, int OY, int NT>
 struct PairCfg {
@@ -537,8 +539,29 @@ struct PairCfg {
   static constexpr int L2 = (N2 + NT - 1) / NT, L1 = (N1 + NT - 1) / NT;
   static constexpr int OR = OX * OY / NT;  // output sites per thread
   static constexpr int ROWS = NT / OX;
+  static constexpr int NR = 4 * W2 + 4 * OY;  // tile+2 sites outside the tile
+  static constexpr int LR = (NR + NT - 1) / NT;
   static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
 };
+
+// Ring index r -> (mx, my) in tile+2 coordinates: two rows above, two below, two columns
+// either side.
+template <class C, int OY>
+__device__ __forceinline__ void ring_site(int r, int& mx, int& my) {
+  if (r < 2 * C::W2) {
+    mx = r % C::W2;
+    my = r / C::W2;
+  } else if (r < 4 * C::W2) {
+    r -= 2 * C::W2;
+    mx = r % C::W2;
+    my = OY + 2 + r / C::W2;
+  } else {
+    r -= 4 * C::W2;
+    const int col = r & 3;
+    my = 2 + (r >> 2);
+    mx = col < 2 ? col : C::W2 - 4 + col;
+  }
+}
 
 template <class T, int OX, int OY, int ZC, int NT, int PAIR>
 __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restrict__ y,
@@ -551,7 +574,6 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
   __shared__ T mas[3][C::N3];
   __shared__ T y2s[3][C::N2];
   __shared__ T mbs[3][C::N1];
-  __shared__ T kq[PAIR == 1 ? 3 : 1][PAIR == 1 ? OX * OY : 1];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
@@ -569,25 +591,26 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     yoff[s] = idx < C::N4 ? (unsigned)(wrap_any(x0 + px - 4, d.nx) + d.nx * wrap_any(y0 + py - 4, d.ny))
                           : 0u;
   }
-  // PAIR 1: psi at the tile+2 sites of stage A
-  unsigned eoff[PAIR == 1 ? C::L2 : 1];
+  const int ox = t % OX, oy0 = t / OX;
+  // PAIR 1: psi at the ring sites of stage A (the tile sites use ooff)
+  unsigned roff[PAIR == 1 ? C::LR : 1];
   if constexpr (PAIR == 1) {
 #pragma unroll
-    for (int s = 0; s < C::L2; ++s) {
-      const int idx = t + s * NT;
-      const int px = idx % C::W2, py = idx / C::W2;
-      eoff[s] = idx < C::N2 ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
-                            : 0u;
+    for (int s = 0; s < C::LR; ++s) {
+      const int r = t + s * NT;
+      int mx = 0, my = 0;
+      if (r < C::NR) ring_site<C, OY>(r, mx, my);
+      roff[s] = r < C::NR ? (unsigned)(wrap_any(x0 + mx - 2, d.nx) + d.nx * wrap_any(y0 + my - 2, d.ny))
+                          : 0u;
     }
   }
-  const int ox = t % OX, oy0 = t / OX;
-  unsigned ooff[C::OR];
-  bool oin[C::OR];
+  unsigned ooff[C::OR];  // tile sites, wrapped (a tile site past the grid edge is the halo
+  bool oin[C::OR];       // of a site inside it); oin: the site is an output
 #pragma unroll
   for (int q = 0; q < C::OR; ++q) {
     const int gy = y0 + oy0 + q * C::ROWS;
     oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
-    ooff[q] = oin[q] ? (unsigned)(x0 + ox + d.nx * gy) : 0u;
+    ooff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
   }
 
   T yreg[C::L4];
@@ -597,7 +620,10 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     for (int s = 0; s < C::L4; ++s)
       if (t + s * NT < C::N4) yreg[s] = yp[yoff[s]];
   };
-  T pe[PAIR == 1 ? C::L2 : 1];
+  T pe[PAIR == 1 ? C::LR : 1];    // PAIR 1: psi at this thread's ring sites
+  T pt[PAIR == 1 ? C::OR : 1];    // PAIR 1: psi at this thread's tile sites, plane of k_A
+  // PAIR 1: k_A (= k3) at the tile sites of the last three k_A planes, newest first
+  T k0r[PAIR == 1 ? C::OR : 1] = {}, k1r[PAIR == 1 ? C::OR : 1] = {}, k2r[PAIR == 1 ? C::OR : 1] = {};
   T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
 
   load_y(z0 - 4);
@@ -612,8 +638,10 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       if (p >= 4) {
         const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 6 + p, d.nz);
 #pragma unroll
-        for (int s = 0; s < C::L2; ++s)
-          if (t + s * NT < C::N2) pe[s] = pp[eoff[s]];
+        for (int s = 0; s < C::LR; ++s)
+          if (t + s * NT < C::NR) pe[s] = pp[roff[s]];
+#pragma unroll
+        for (int q = 0; q < C::OR; ++q) pt[q] = pp[ooff[q]];
       }
     }
     if (p >= 8) {
@@ -647,25 +675,38 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     if (p >= 4) {  // k_A and y' on plane z0-6+p, tile+2
       const int zA = z0 - 6 + p;
       const bool own = zA >= z0 && zA < z0 + nout;
+      auto kA = [&](int mx, int my) {
+        const int o = (my + 1) * C::W3 + (mx + 1);
+        return lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+      };
+      // tile sites: the output owner
 #pragma unroll
-      for (int s = 0; s < C::L2; ++s) {
-        const int idx = t + s * NT;
-        if (idx < C::N2) {
-          const int mx = idx % C::W2, my = idx / C::W2;
-          const int o = (my + 1) * C::W3 + (mx + 1);
-          const T kv = lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
-          const int lx = mx - 2, ly = my - 2;  // tile coordinates
-          const bool in_tile = lx >= 0 && lx < OX && ly >= 0 && ly < OY;
-          if constexpr (PAIR == 0) {
-            const T base = ys[s2][(my + 2) * C::W4 + (mx + 2)];
-            y2s[s0][idx] = base + co.half_dt * kv;
-            if (own && in_tile && x0 + lx < d.nx && y0 + ly < d.ny)
-              __builtin_nontemporal_store(
-                  kv, a + (size_t)d.plane * (size_t)zA + (size_t)(x0 + lx) + (size_t)d.nx * (y0 + ly));
-          } else {
-            y2s[s0][idx] = pe[s] + co.dt * kv;
-            if (in_tile) kq[s0][ly * OX + lx] = kv;
-          }
+      for (int q = 0; q < C::OR; ++q) {
+        const int mx = ox + 2, my = oy0 + q * C::ROWS + 2;
+        const T kv = kA(mx, my);
+        const int e = my * C::W2 + mx;
+        if constexpr (PAIR == 0) {
+          y2s[s0][e] = ys[s2][(my + 2) * C::W4 + (mx + 2)] + co.half_dt * kv;
+          if (own && oin[q])
+            __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + ooff[q]);
+        } else {
+          y2s[s0][e] = pt[q] + co.dt * kv;
+          k2r[q] = k1r[q];
+          k1r[q] = k0r[q];
+          k0r[q] = kv;
+        }
+      }
+      // the 2-site ring around the tile
+#pragma unroll
+      for (int s = 0; s < C::LR; ++s) {
+        const int r = t + s * NT;
+        if (r < C::NR) {
+          int mx, my;
+          ring_site<C, OY>(r, mx, my);
+          const T kv = kA(mx, my);
+          const int e = my * C::W2 + mx;
+          if constexpr (PAIR == 0) y2s[s0][e] = ys[s2][(my + 2) * C::W4 + (mx + 2)] + co.half_dt * kv;
+          else y2s[s0][e] = pe[s] + co.dt * kv;
         }
       }
     }
@@ -696,8 +737,7 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
             __builtin_nontemporal_store(kv, b + kb + ooff[q]);
             __builtin_nontemporal_store(pv[q] + co.half_dt * kv, out + kb + ooff[q]);
           } else {
-            const T k3 = kq[s2][oy * OX + ox];
-            const T bsum = bv[q] + k3;
+            const T bsum = bv[q] + k2r[q];  // k3 of this plane (k_A pass of iteration p-2)
             __builtin_nontemporal_store(pv[q] + (co.sixth_dt * (av[q] + kv) + co.third_dt * bsum),
                                         out + kb + ooff[q]);
           }
@@ -722,20 +762,27 @@ constexpr TileCfg kCh2Cfgs[] = {{64, 4, 1},  {64, 8, 1},   {64, 16, 1},
                                 {64, 32, 1}, {128, 8, 1}, {128, 16, 1}};
 constexpr int kN3 = sizeof(kCh3Cfgs) / sizeof(kCh3Cfgs[0]);
 constexpr int kN2 = sizeof(kCh2Cfgs) / sizeof(kCh2Cfgs[0]);
-// defaults: the measured best on 256^3 (FP32: 64x16x16, FP64: 64x8x32) and 2-D 1024^2
-constexpr int kDefault3F32 = 6, kDefault3F64 = 4, kDefault2 = 2;
+// one-stage defaults: the measured best one-stage shapes on 256^3 (FP32: 64x16x16, FP64:
+// 64x8x32) and 2-D 1024^2; used for Euler and single right-hand-side evaluations
+constexpr int kStage3F32 = 6, kStage3F64 = 4, kDefault2 = 2;
 
 // Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
 inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
 
-// 3-D two-stage (fused pair) shapes: ids kN3 .. kN3 + kNP - 1; block size NT threads.
+// 3-D two-stage (fused pair) shapes: ids kN3 .. kN3 + kNP - 1; block size NT threads.  The
+// 128 x 16 and 64 x 32 tiles need more LDS than a workgroup has in FP64; FP64 runs
+// 64 x 16 x ZC there.
 struct PairTile {
   int ox, oy, zc, nt;
 };
-constexpr PairTile kChPairCfgs[] = {{64, 8, 32, 512}, {64, 8, 64, 512}, {32, 16, 32, 512},
-                                    {32, 16, 64, 512}};
+constexpr PairTile kChPairCfgs[] = {{64, 8, 32, 512},    {64, 16, 16, 1024},  {64, 16, 32, 1024},
+                                    {64, 16, 64, 1024},  {128, 16, 16, 1024}, {128, 16, 32, 1024},
+                                    {128, 16, 64, 1024}, {64, 32, 16, 1024},  {64, 32, 32, 1024}};
 constexpr int kNP = sizeof(kChPairCfgs) / sizeof(kChPairCfgs[0]);
 inline bool is_pair(const Dims& d, int id) { return d.nz > 1 && id >= kN3; }
+// 3-D RK4 defaults: the measured best on 256^3 (FP32: two-stage 64x32x32, FP64: two-stage
+// 64x16x64)
+constexpr int kDefault3F32 = kN3 + 8, kDefault3F64 = kN3 + 3;
 
 #define GC_STAGE_SWITCH(LAUNCH)      \
   switch (stage) {                   \
@@ -820,11 +867,32 @@ void launch_pair3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int pair,
 template <class T>
 void launch_pair_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf, int pair,
                      const T* y, const T* psi, T* out, T* a, T* b, const StageCoef<T>& co) {
+  constexpr bool f32 = sizeof(T) == 4;
   switch (id - kN3) {
     case 0: launch_pair3<T, 64, 8, 32, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
-    case 1: launch_pair3<T, 64, 8, 64, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
-    case 2: launch_pair3<T, 32, 16, 32, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
-    default: launch_pair3<T, 32, 16, 64, 512>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 1: launch_pair3<T, 64, 16, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 2: launch_pair3<T, 64, 16, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 3: launch_pair3<T, 64, 16, 64, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 4:
+      if constexpr (f32) launch_pair3<T, 128, 16, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    case 5:
+      if constexpr (f32) launch_pair3<T, 128, 16, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    case 6:
+      if constexpr (f32) launch_pair3<T, 128, 16, 64, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 64, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    case 7:
+      if constexpr (f32) launch_pair3<T, 64, 32, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    default:
+      if constexpr (f32) launch_pair3<T, 64, 32, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
   }
 }
 
@@ -853,7 +921,7 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
   const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
   if (ch_tiled(d)) {
     int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
-    if (is_pair(d, id)) id = sizeof(T) == 8 ? kDefault3F64 : kDefault3F32;  // one stage only
+    if (is_pair(d, id)) id = sizeof(T) == 8 ? kStage3F64 : kStage3F32;  // one stage only
     launch_ch_cfg<T>(ctx, id, d, cf, stage, y, psi, ynext, a, b, co);
   } else {
     const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;

@@ -186,7 +186,8 @@ gc_status gc_diffuse_field(gc_context* ctx, const gc_grid* g, gc_dtype dt, void*
 
 /* Number of candidate shapes for this grid's kernel family (0 if untiled). */
 int gc_ch_config_count(const gc_grid* g);
-/* Tile shape of candidate `id`. */
+/* Tile shape of candidate `id` (as run in FP32; two-stage candidates whose FP32 tile needs
+ * more LDS than a workgroup has in FP64 run 64 x 16 x zc in FP64). */
 gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc);
 /* 1 if 3-D candidate `id` runs RK4 as two kernels of two stages each (S1+S2, S3+S4: 9
  * words of HBM traffic per site and step instead of 17; same results bit for bit), 0 for
