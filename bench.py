@@ -16,6 +16,7 @@ prefix of the same workload.
   python bench.py --workload sweep [--n 512]      # operator GB/s (3-D operator sweep)
   python bench.py --workload ensemble             # 64 uneven 2-D members, graph-replayed
   python bench.py --workload learned [--n-ic 4]   # learned corrector vs plain RK4 to T=20
+  python bench.py --workload tune [--n 256]       # autotuner: ms/step of every tile shape
 """
 from __future__ import annotations
 
@@ -45,7 +46,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
-                                                          "learned"])
+                                                          "learned", "tune"])
     p.add_argument("--n-ic", type=int, default=4)
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--diag-every", type=int, default=500)
@@ -56,13 +57,13 @@ def parse():
     return p.parse_args()
 
 
-def profiled_traffic(name):
+def profiled_traffic(name, key="traffic_bytes_per_step"):
     """Bytes per fused step measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by the
     device-copy calibration) for this workload, as recorded in profiles/<name>."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_step ="):
+            if line.startswith(key + " ="):
                 return float(line.split("=")[1])
     except OSError:
         pass
@@ -86,8 +87,8 @@ def run_timed(gc, ctx, g, d, steps, diag_every, kappa, dt):
 
 
 def step_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
-    """ms per fused RK4 step: HIP events on the context's stream around `steps` steps and
-    nothing else (4 stage-kernel launches per step)."""
+    """ms per fused RK4 step: HIP events on the context's stream around `steps` (even) steps
+    and nothing else (2 two-stage or 4 one-stage kernel launches per step)."""
     d = gc.DeviceArray.from_numpy(ctx, c0.astype(dtype))
     gc.ch_integrate(ctx, g, d, 1.0, kappa, dt, 2, gc.RK4)
     ctx.sync()
@@ -113,6 +114,9 @@ def ch3d(a):
                 best, ms = gc.ch_tune(ctx, g, dtype)
             tiles[name] = gc.ch_config_shape(g, gc.ch_get_config(g, dtype))
         extra["tile_ox_oy_zc"] = tiles
+        fused = gc.ch_config_fused(g, gc.ch_get_config(g, gc.F32))
+        extra["two_stage_kernels"] = {"f32": fused,
+                                      "f64": gc.ch_config_fused(g, gc.ch_get_config(g, gc.F64))}
 
         d32 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
         if a.warmup:
@@ -142,7 +146,10 @@ def ch3d(a):
                 "ms_per_step": ms64 / a.steps,
                 "step_kernels_ms": sms64,
                 "hbm_GBs_alg": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9,
-                "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS}
+                "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "traffic": profiled_traffic(
+                    f"ch3d_rk4_{'pair' if extra['two_stage_kernels']['f64'] else 'stage'}_kernels_{n}_f32.md",
+                    "traffic_bytes_per_step_f64")}
 
     if not a.no_learned:
         extra["learned"] = learned(a)
@@ -184,8 +191,10 @@ def ch3d(a):
                    "grid": [n, n, n], "dt": dt, "kappa": kappa, "M": 1.0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(f"ch3d_rk4_stage_kernels_{n}_f32.md"),
-                     "kernel": "k_ch_stage3d<float,...> x4 (one fused RK4 step = 4 stage launches)",
+                     "traffic": profiled_traffic(f"ch3d_rk4_{'pair' if fused else 'stage'}_kernels_{n}_f32.md"),
+                     "kernel": (f"k_ch_pair3d<float,{tiles['f32'][0]},{tiles['f32'][1]},{tiles['f32'][2]},...> "
+                                "x2 (one RK4 step = S1+S2 and S3+S4 launches)") if fused else
+                               "k_ch_stage3d<float,...> x4 (one RK4 step = 4 stage launches)",
                      "step_kernels_ms": sms, "alg_bytes_per_step": bytes_step},
         "cpu_baseline": cpu,
         "extra": extra,
@@ -358,6 +367,24 @@ def learned(a):
                          "pairs": "snapshot m -> m+1 of 2000 FP64 GPU RK4 trajectories"}}
 
 
+def tune(a):
+    """The tile autotuner's measurements for the 3-D workload's grid (default 256^3): ms per
+    RK4 step of every candidate shape, one-stage and two-stage kernels, FP32 and FP64."""
+    gc = entry.load_package()
+    n = a.n or 256
+    g = gc.Grid.make(n, n, n, 1.0)
+    rows = []
+    with gc.Context() as ctx:
+        for dtype, name, w in ((gc.F32, "f32", 4), (gc.F64, "f64", 8)):
+            best, ms = gc.ch_tune(ctx, g, dtype)
+            for i, m in enumerate(ms):
+                rows.append({"dtype": name, "id": i, "tile_ox_oy_zc": gc.ch_config_shape(g, i),
+                             "two_stage": gc.ch_config_fused(g, i), "ms_per_step": m,
+                             "hbm_frac_16W": WORDS_PER_SITE_STEP * w * n ** 3 / (m / 1e3) / 1e9
+                             / HBM_PEAK_GBS, "best": i == best})
+    return {"metric": "RK4 ms per step by tile shape", "grid": [n, n, n], "rows": rows}
+
+
 def copy(a):
     """Plain device-to-device copy of a known size (256 MiB): the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -385,7 +412,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble,
-           "learned": learned}[a.workload](a)
+           "learned": learned, "tune": tune}[a.workload](a)
     print(json.dumps(out))
 
 

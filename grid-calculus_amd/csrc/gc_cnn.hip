@@ -439,12 +439,13 @@ __global__ __launch_bounds__(256) void k_sum_f32(const float* __restrict__ x, lo
   if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-__global__ void k_sum_final(const double* __restrict__ part, int n, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += part[i];
-    out[0] = s;
-  }
+// One wave: lane l sums part[l], part[l + 64], ... in order, then a fixed xor tree.
+__global__ __launch_bounds__(64) void k_sum_final(const double* __restrict__ part, int n,
+                                                  double* __restrict__ out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 64) s += part[i];
+  for (int st = 32; st >= 1; st >>= 1) s += __shfl_xor(s, st);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 template <class TS, class TD>
@@ -861,17 +862,30 @@ __global__ __launch_bounds__(256) void k_mass_fe(const T* __restrict__ c, int nx
   }
 }
 
-__global__ void k_mass_fe_final(const Comp2* __restrict__ part, int nb, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
-    for (int b = 0; b < nb; ++b) {
-      mc += part[b].mc;
-      fc += part[b].fc;
-      neu(ms, mc, part[b].ms);
-      neu(fs, fc, part[b].fs);
-    }
-    out[0] = ms + mc;
-    out[1] = fs + fc;
+// One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
+// a fixed xor tree (sums with TwoSum, compensations added).
+__global__ __launch_bounds__(64) void k_mass_fe_final(const Comp2* __restrict__ part, int nb,
+                                                      double* __restrict__ out) {
+  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    mc += part[b].mc;
+    fc += part[b].fc;
+    neu(ms, mc, part[b].ms);
+    neu(fs, fc, part[b].fs);
+  }
+  Comp2 v{ms, mc, fs, fc};
+  for (int st = 1; st < 64; st <<= 1) {
+    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
+    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
+    double s1 = lo.ms, c1 = lo.mc + hi.mc;
+    neu(s1, c1, hi.ms);
+    double s2 = lo.fs, c2 = lo.fc + hi.fc;
+    neu(s2, c2, hi.fs);
+    v = Comp2{s1, c1, s2, c2};
+  }
+  if (threadIdx.x == 0) {
+    out[0] = v.ms + v.mc;
+    out[1] = v.fs + v.fc;
   }
 }
 
