@@ -304,8 +304,9 @@ def learned(a):
     learned path advances the FP64 state with the trained network and falls back to plain
     RK4 for a macro step whose proposal drifts the mass by more than 1e-6 or raises the free
     energy.  Error: relative L2 against the plain FP64 GPU RK4 result (bit-identical to the
-    CPU oracle's RK4; the parity tests pin that).  Times are wall-clock, device-synchronised,
-    after one untimed run of each path; n_ic initial states, median time, max error."""
+    CPU oracle's RK4; the parity tests pin that).  The plain path's tile shape is autotuned
+    first.  Times are wall-clock, device-synchronised, after one untimed run of each path;
+    n_ic initial states, median time, max error."""
     gc = entry.load_package()
     from gridcalc_amd import corrector as cr  # noqa: E402
     params, meta = cr.load_params(256, 256)
@@ -319,6 +320,10 @@ def learned(a):
     res = {"plain_fp64": [], "plain_fp32": [], "learned_fp64": []}
     errs, errs32, fallbacks = [], [], []
     with gc.Context() as ctx:
+        tiles = {}
+        for dtype, name in ((gc.F64, "f64"), (gc.F32, "f32")):  # the plain path at its best
+            best, _ = gc.ch_tune(ctx, g, dtype)
+            tiles[name] = gc.ch_config_shape(g, best)[:2]
         net = gc.Corrector(ctx, H, W, 1)
         net.set_params(params, meta["scale"])
 
@@ -361,7 +366,7 @@ def learned(a):
             "speedup_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64"],
             "speedup_vs_plain_fp32": med["plain_fp32"] / med["learned_fp64"],
             "rel_l2_learned_vs_fp64": max(errs), "rel_l2_plain_fp32_vs_fp64": max(errs32),
-            "fallbacks": fallbacks, "target_rel_l2": 1e-3,
+            "fallbacks": fallbacks, "target_rel_l2": 1e-3, "plain_tile_ox_oy": tiles,
             "target_met": bool(max(errs) <= 1e-3 and med["plain_fp64"] / med["learned_fp64"] >= 5),
             "training": {"seconds": meta.get("seconds"), "final_loss": meta["history"][-1],
                          "pairs": "snapshot m -> m+1 of 2000 FP64 GPU RK4 trajectories"}}

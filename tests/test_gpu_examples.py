@@ -1,0 +1,101 @@
+"""The reference's example programs (examples/cahn_hilliard.cpp, heterogeneous_diffusion.cpp,
+elastic_energy.cpp), compiled unchanged against this build's header API
+(grid-calculus_amd/cpp, target ref-examples), run on the GPU: their VTK snapshots (written
+with 17 significant digits) are checked against the oracle from the snapshot they start from.
+
+The binaries are built where the reference checkout exists; elsewhere the tests skip."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "grid-calculus_amd", "cpp", "bin", "ref_examples")
+
+
+def _exe(name):
+    p = os.path.join(BIN, name)
+    if not os.path.exists(p):
+        pytest.skip(f"{name} not built (needs the reference's example sources at build time)")
+    return p
+
+
+def _run(args, timeout=120):
+    print("running:", " ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def _vtk(path):
+    """(nx, ny, nz), spacing, data (nz, ny, nx) of a legacy ASCII STRUCTURED_POINTS file."""
+    lines = open(path).read().split("\n")
+    dims = spacing = None
+    for i, ln in enumerate(lines):
+        if ln.startswith("DIMENSIONS"):
+            dims = tuple(int(v) for v in ln.split()[1:])
+        elif ln.startswith("SPACING"):
+            spacing = tuple(float(v) for v in ln.split()[1:])
+        elif ln.startswith("LOOKUP_TABLE"):
+            vals = np.array([float(v) for v in lines[i + 1:] if v.strip()])
+            break
+    nx, ny, nz = dims
+    return dims, spacing, vals.reshape(nz, ny, nx)
+
+
+def test_cahn_hilliard_example(orc, tmp_path):
+    exe = _exe("cahn_hilliard")
+    out = tmp_path / "ch"
+    stdout = _run([exe, "--n-x", "16", "--dt", "0.01", "--n-steps", "40", "--snapshot-every", "20",
+                   "--kappa", "1", "--mobility", "1", "--seed", "7", "--out-dir", str(out)])
+    dims, h, c0 = _vtk(out / "psi_000000.vtk")
+    _, _, c20 = _vtk(out / "psi_000020.vtk")
+    _, _, c40 = _vtk(out / "psi_000040.vtk")
+    og = orc.Grid.make(*dims, h)
+    w20 = orc.ch_integrate(og, c0, 1.0, 1.0, 0.01, 20, orc.RK4)
+    w40 = orc.ch_integrate(og, w20, 1.0, 1.0, 0.01, 20, orc.RK4)
+    assert np.array_equal(c20, w20)
+    assert np.array_equal(c40, w40)
+    # the printed free energy at step 40 (printf %+.6e)
+    line = [ln for ln in stdout.splitlines() if ln.startswith("step=    40")][0]
+    F = float(line.split("F=")[1].split()[0])
+    assert abs(F - orc.free_energy(og, w40, 1.0)) <= 5e-7 * abs(F)
+
+
+@pytest.mark.parametrize("integ", ["rk4", "euler"])
+def test_heterogeneous_diffusion_example(orc, tmp_path, integ):
+    exe = _exe("heterogeneous_diffusion")
+    out = tmp_path / integ
+    n, dt, d0, eps = 16, 0.002, 0.1, 0.05
+    _run([exe, "--n-x", str(n), "--dt", str(dt), "--n-steps", "30", "--snapshot-every", "30",
+          "--d-mean", str(d0), "--d-amplitude", str(eps), "--d-axis", "y", "--integrator", integ,
+          "--out-dir", str(out)])
+    dims, h, c0 = _vtk(out / "psi_000000.vtk")
+    _, _, c30 = _vtk(out / "psi_000030.vtk")
+    og = orc.Grid.make(*dims, h)
+    # D = D0 + eps cos(2 pi y / L) sampled at y = j h (the example's D field; cos evaluated by
+    # numpy here, so agreement is to rounding rather than bit for bit)
+    L = dims[1] * h[1]
+    y = np.arange(dims[1]) * h[1]
+    D = np.broadcast_to((d0 + eps * np.cos(2.0 * 3.14159265358979323846 / L * y))[None, :, None],
+                        c0.shape).copy()
+    want = orc.diffuse_field(og, c0, D, dt, 30, orc.RK4 if integ == "rk4" else orc.EULER)
+    assert np.abs(c30 - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_elastic_energy_example(tmp_path):
+    """The strain energy of the example's single-wavenumber displacement (k = 1 on [0, 2 pi]^3)
+    from the order-2 Vec3 gradient: the central difference scales the mode by sin(h)/h, so
+    the energy's relative error against the closed form is exactly 1 - (sin(h)/h)^2 --
+    5.036e-2 at N = 16 and 1.279e-2 at N = 32 (printed to 4 digits)."""
+    exe = _exe("elastic_energy")
+    for n in (16, 32):
+        stdout = _run([exe, "--n-x", str(n), "--out-dir", str(tmp_path / f"e{n}")])
+        line = [ln for ln in stdout.splitlines() if "rel_err" in ln][0]
+        err = float(line.split("rel_err =")[1].split()[0])
+        h = 2.0 * np.pi / n
+        want = 1.0 - (np.sin(h) / h) ** 2
+        assert abs(err - want) <= 1e-3 * want, (n, err, want)
