@@ -158,10 +158,7 @@ def ch3d(a):
     if not a.no_cpu:
         orc = entry.load_oracle()
         orc.lib()
-        try:
-            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-        except (AttributeError, OSError):
-            pass
+        _pin_one_core()
         og = orc.Grid.make(n, n, n, 1.0)
         t0 = time.perf_counter()
         orc.ch_integrate(og, c0, 1.0, kappa, dt, a.cpu_steps, orc.RK4)
@@ -210,7 +207,12 @@ def sweep(a):
     g = gc.Grid.make(n, n, n, h)
     u = gc.inputs.sweep_field(n, h, seed=7)
     rows = []
-    words = {"laplacian": 2, "biharmonic": 2, "gradient": 4, "divergence": 4, "cell_laplacian": 3}
+    words = {"laplacian": 2, "biharmonic": 2, "gradient": 4, "divergence": 4, "cell_laplacian": 3,
+             "gradient_vec": 12, "divergence_mat": 12}
+    ops = [("laplacian", 2), ("laplacian", 4), ("gradient", 2), ("gradient", 4),
+           ("divergence", 2), ("divergence", 4), ("biharmonic", 2), ("biharmonic", 4),
+           ("cell_laplacian", 2), ("gradient_vec", 2), ("gradient_vec", 4),
+           ("divergence_mat", 2), ("divergence_mat", 4)]
     # three independent components for the vector field; D ~ lognormal(0, 1)
     vv = np.empty((n, n, n, 3), np.float64)
     for c in range(3):
@@ -222,12 +224,19 @@ def sweep(a):
             d_v = gc.DeviceArray.from_numpy(ctx, vv.astype(dtype))
             d_D = gc.DeviceArray.from_numpy(ctx, D.astype(dtype))
             outs = {"laplacian": gc.DeviceArray((n, n, n), dtype),
-                    "gradient": gc.DeviceArray((n, n, n, 3), dtype)}
-            for name, order in [("laplacian", 2), ("laplacian", 4), ("gradient", 2), ("gradient", 4),
-                                ("divergence", 2), ("divergence", 4), ("biharmonic", 2),
-                                ("biharmonic", 4), ("cell_laplacian", 2)]:
+                    "gradient": gc.DeviceArray((n, n, n, 3), dtype),
+                    "mat": gc.DeviceArray((n, n, n, 9), dtype)}
+            # the tensor input of divergence_mat: the order-2 gradient of the vector field
+            gc.gradient_vec(ctx, g, d_v, 2, out=outs["mat"])
+            d_m = gc.DeviceArray((n, n, n, 9), dtype)
+            d_m.copy_from(ctx, outs["mat"])
+            for name, order in ops:
                 def run():
-                    if name == "laplacian":
+                    if name == "gradient_vec":
+                        gc.gradient_vec(ctx, g, d_v, order, out=outs["mat"])
+                    elif name == "divergence_mat":
+                        gc.divergence_mat(ctx, g, d_m, order, out=outs["gradient"])
+                    elif name == "laplacian":
                         gc.laplacian(ctx, g, d_u, order, out=outs["laplacian"])
                     elif name == "biharmonic":
                         gc.biharmonic(ctx, g, d_u, order, out=outs["laplacian"])
@@ -248,9 +257,44 @@ def sweep(a):
                 rows.append({"op": name, "order": order, "dtype": "f64" if w == 8 else "f32",
                              "ms": ms, "GBs_alg": gbs, "hbm_frac": gbs / HBM_PEAK_GBS})
                 print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
-            for x in (d_u, d_v, d_D, *outs.values()):
+            for x in (d_u, d_v, d_D, d_m, *outs.values()):
                 x.free()
-    return {"metric": "operator GB/s (algorithmic bytes)", "n": n, "rows": rows}
+    cpu = None
+    if not a.no_cpu:  # the oracle (FP64, 1 thread) once per operator on the same input
+        orc = entry.load_oracle()
+        orc.lib()
+        _pin_one_core()
+        og = orc.Grid.make(n, n, n, h)
+        cpu = {}
+        mat = orc.gradient_vec(og, vv, 2)
+        for name, order in ops:
+            t0 = time.perf_counter()
+            if name == "gradient_vec":
+                orc.gradient_vec(og, vv, order)
+            elif name == "divergence_mat":
+                orc.divergence_mat(og, mat, order)
+            elif name == "cell_laplacian":
+                orc.cell_laplacian(og, u, D)
+            elif name == "divergence":
+                orc.divergence(og, vv, order)
+            else:
+                getattr(orc, name)(og, u, order)
+            el = time.perf_counter() - t0
+            gpu = [r for r in rows if r["op"] == name and r["order"] == order and r["dtype"] == "f64"][0]
+            gpu["cpu_ms"] = el * 1e3
+            gpu["speedup_vs_cpu"] = el * 1e3 / gpu["ms"]
+            cpu[f"{name}{order}"] = el * 1e3
+        cpu = {"ms_per_op": cpu, "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (C restatement, FP64, gcc -O2, 1 thread), each operator once "
+                         f"on the same {n}^3 input"}
+    return {"metric": "operator GB/s (algorithmic bytes)", "n": n, "rows": rows, "cpu_baseline": cpu}
+
+
+def _pin_one_core():
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
 
 
 def ensemble_members(gc, n=64, seed=4):
@@ -289,7 +333,21 @@ def ensemble(a):
         sites = ens.sites()
         ens.close()
     value = sites * steps / (ms / 1e3)
+    cpu = None
+    if not a.no_cpu:  # the oracle's RK4 (FP64, 1 thread), 2 steps of every member
+        orc = entry.load_oracle()
+        orc.lib()
+        _pin_one_core()
+        t0 = time.perf_counter()
+        for m, c in zip(members, ics):
+            og = orc.Grid.make(m["nx"], m["ny"], 1, 1.0)
+            orc.ch_integrate(og, c, 1.0, m["kappa"], dt, 2, orc.RK4)
+        el = time.perf_counter() - t0
+        cpu = {"value": sites * 2 / el, "unit": "site-updates/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle RK4 (FP64, gcc -O2, 1 thread): the first 2 steps of all 64 "
+                         f"members ({el:.1f} s)"}
     return {"metric": "ensemble RK4 site-updates/s", "value": value, "unit": "site-updates/s",
+            "cpu_baseline": cpu,
             "dtype": "f32", "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps,
             "sites": sites, "members": len(members), "tile_ox_oy": tile,
             "tune_ms_per_step": tune_ms,
@@ -317,8 +375,8 @@ def learned(a):
     n_steps = 2000
     g = gc.Grid.make(W, H, 1)
     seeds = [991001 + i for i in range(a.n_ic)]
-    res = {"plain_fp64": [], "plain_fp32": [], "learned_fp64": []}
-    errs, errs32, fallbacks = [], [], []
+    res = {"plain_fp64": [], "plain_fp32": [], "learned_fp64": [], "learned_fp64_warm1": []}
+    errs, errs32, errs_w, fallbacks = [], [], [], []
     with gc.Context() as ctx:
         tiles = {}
         for dtype, name in ((gc.F64, "f64"), (gc.F32, "f32")):  # the plain path at its best
@@ -353,19 +411,46 @@ def learned(a):
             t, nf = timed(lambda: net.integrate(g, lrn, 1.0, 1.0, dt, sub, n_steps // sub, 1e-6))
             res["learned_fp64"].append(t)
             fallbacks.append(nf)
+            # variant: the first macro step (the white-noise initial state, far from the smooth
+            # states the network is trained to advance) by plain RK4, the rest learned
+            wrm = gc.DeviceArray.from_numpy(ctx, c0)
+
+            def warm():
+                gc.ch_integrate(ctx, g, wrm, 1.0, 1.0, dt, sub, gc.RK4)
+                return net.integrate(g, wrm, 1.0, 1.0, dt, sub, n_steps // sub - 1, 1e-6)
+            t, _ = timed(warm)
+            res["learned_fp64_warm1"].append(t)
             r, l, q = ref.numpy(ctx), lrn.numpy(ctx), f32.numpy(ctx).astype(np.float64)
+            w = wrm.numpy(ctx)
             errs.append(float(np.linalg.norm(l - r) / np.linalg.norm(r)))
             errs32.append(float(np.linalg.norm(q - r) / np.linalg.norm(r)))
-            for arr in (ref, f32, lrn):
+            errs_w.append(float(np.linalg.norm(w - r) / np.linalg.norm(r)))
+            for arr in (ref, f32, lrn, wrm):
                 arr.free()
         net.close()
     med = {k: float(np.median(v)) for k, v in res.items()}
+    cpu = None
+    if not a.no_cpu:  # the oracle's RK4 (FP64, 1 thread) on a prefix, scaled to T
+        orc = entry.load_oracle()
+        orc.lib()
+        _pin_one_core()
+        og = orc.Grid.make(W, H, 1, 1.0)
+        c0 = cr.initial_state(H, W, seeds[0]).astype(np.float64)
+        t0 = time.perf_counter()
+        orc.ch_integrate(og, c0, 1.0, 1.0, dt, 160, orc.RK4)
+        el = time.perf_counter() - t0
+        cpu = {"value": el * n_steps / 160, "unit": "s to T=20", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle RK4 (FP64, gcc -O2, 1 thread): the first 160 of 2000 steps "
+                         f"({el:.2f} s), scaled"}
     return {"metric": "learned vs plain time-to-T", "T": n_steps * dt, "grid": [H, W],
+            "cpu_baseline": cpu,
             "macro_steps": n_steps // sub, "n_ic": len(seeds),
             "seconds": med,
             "speedup_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64"],
             "speedup_vs_plain_fp32": med["plain_fp32"] / med["learned_fp64"],
             "rel_l2_learned_vs_fp64": max(errs), "rel_l2_plain_fp32_vs_fp64": max(errs32),
+            "rel_l2_learned_warm1_vs_fp64": max(errs_w),
+            "speedup_warm1_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64_warm1"],
             "fallbacks": fallbacks, "target_rel_l2": 1e-3, "plain_tile_ox_oy": tiles,
             "target_met": bool(max(errs) <= 1e-3 and med["plain_fp64"] / med["learned_fp64"] >= 5),
             "training": {"seconds": meta.get("seconds"), "final_loss": meta["history"][-1],

@@ -6,136 +6,16 @@
 // right; this file is compiled with -ffp-contract=off, so double-precision results
 // equal the CPU path bit for bit.
 //
-// Layout: one thread per site.  A 64 x 4 thread block covers a 64 x 4 tile of one
-// xy-plane; block ids are remapped so each XCD owns a contiguous range of tiles
-// (neighbouring rows and planes meet in the same L2).  Neighbours come through the
-// L1/L2 caches: the operators move 2-4 words per site and are bounded by HBM.
+// Two engines, both marching through chunks of z-planes per block with block ids remapped so
+// each XCD owns a contiguous range of tiles (neighbouring tiles meet in the same L2):
+// scalar-input operators keep a ring of input planes in LDS; vector- and tensor-field
+// operators keep the components differentiated along z in per-thread register queues and
+// stage only the current plane (x or y halo) in LDS.  The operators move 2-12 words per site
+// and are bounded by HBM.
 #include "gc_common.hpp"
 
 namespace gc {
 namespace {
-
-constexpr int TX = 64, TY = 4;
-
-struct Tile {
-  int i, j, k;
-  bool valid;
-};
-
-// Maps the (remapped) linear block id to a site.  Grid is 1-D: nbx*nby*nz blocks.
-__device__ __forceinline__ Tile tile_site(const Dims& d, int nbx, int nby) {
-  const unsigned nb = gridDim.x;
-  const unsigned b = xcd_remap(blockIdx.x, nb);
-  const int bx = (int)(b % (unsigned)nbx);
-  const unsigned r = b / (unsigned)nbx;
-  const int by = (int)(r % (unsigned)nby);
-  const int bz = (int)(r / (unsigned)nby);
-  Tile t;
-  t.i = bx * TX + (int)(threadIdx.x % TX);
-  t.j = by * TY + (int)(threadIdx.x / TX);
-  t.k = bz;
-  t.valid = t.i < d.nx && t.j < d.ny;
-  return t;
-}
-
-struct Launch {
-  dim3 grid, block;
-  int nbx, nby;
-};
-inline Launch launch_for(const Dims& d) {
-  Launch L;
-  L.nbx = (d.nx + TX - 1) / TX;
-  L.nby = (d.ny + TY - 1) / TY;
-  L.grid = dim3((unsigned)(L.nbx * L.nby * d.nz));
-  L.block = dim3(TX * TY);
-  return L;
-}
-
-// Neighbour address helpers.
-struct Nb {
-  const Dims& d;
-  int i, j, k;
-  __device__ __forceinline__ size_t at(int di, int dj, int dk) const {
-    return (size_t)wrap(i + di, d.nx) + (size_t)d.nx * (size_t)wrap(j + dj, d.ny) +
-           (size_t)d.plane * (size_t)wrap(k + dk, d.nz);
-  }
-  __device__ __forceinline__ size_t c() const {
-    return (size_t)i + (size_t)d.nx * (size_t)j + (size_t)d.plane * (size_t)k;
-  }
-};
-
-// diff/Gradient.hpp:113-134 (Vec3 -> Mat3, column-major: M(comp, axis) at comp + 3*axis)
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_gradient_vec(const T* __restrict__ v,
-                                                      T* __restrict__ out, Dims d, int nbx,
-                                                      int nby, T ihx, T ihy, T ihz) {
-  using S = D1<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T* o9 = out + 9 * nb.c();
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    T dx = 0, dy = 0, dz = 0;
-#pragma unroll
-    for (int m = 0; m < S::N; ++m) {
-      const int o = m - S::R;
-      const T w = T(S::w(m));
-      dx += w * v[3 * nb.at(o, 0, 0) + comp];
-      dy += w * v[3 * nb.at(0, o, 0) + comp];
-      dz += w * v[3 * nb.at(0, 0, o) + comp];
-    }
-    o9[comp + 0] = dx * ihx;
-    o9[comp + 3] = dy * ihy;
-    o9[comp + 6] = dz * ihz;
-  }
-}
-
-// diff/Divergence.hpp:56-69 (Vec3 -> scalar)
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_divergence(const T* __restrict__ v, T* __restrict__ out,
-                                                    Dims d, int nbx, int nby, T ihx, T ihy,
-                                                    T ihz) {
-  using S = D1<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T ax = 0, ay = 0, az = 0;
-#pragma unroll
-  for (int m = 0; m < S::N; ++m) {
-    const int o = m - S::R;
-    const T w = T(S::w(m));
-    ax += w * v[3 * nb.at(o, 0, 0) + 0];
-    ay += w * v[3 * nb.at(0, o, 0) + 1];
-    az += w * v[3 * nb.at(0, 0, o) + 2];
-  }
-  out[nb.c()] = ax * ihx + ay * ihy + az * ihz;
-}
-
-// diff/Divergence.hpp:113-131 (Mat3 -> Vec3)
-template <class T, int O>
-__global__ __launch_bounds__(256) void k_divergence_mat(const T* __restrict__ m9,
-                                                        T* __restrict__ out, Dims d, int nbx,
-                                                        int nby, T ihx, T ihy, T ihz) {
-  using S = D1<O>;
-  const Tile t = tile_site(d, nbx, nby);
-  if (!t.valid) return;
-  const Nb nb{d, t.i, t.j, t.k};
-  T* o3 = out + 3 * nb.c();
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    T dx = 0, dy = 0, dz = 0;
-#pragma unroll
-    for (int m = 0; m < S::N; ++m) {
-      const int o = m - S::R;
-      const T w = T(S::w(m));
-      dx += w * m9[9 * nb.at(o, 0, 0) + comp + 0];
-      dy += w * m9[9 * nb.at(0, o, 0) + comp + 3];
-      dz += w * m9[9 * nb.at(0, 0, o) + comp + 6];
-    }
-    o3[comp] = dx * ihx + dy * ihy + dz * ihz;
-  }
-}
 
 // ---- z-marching LDS engine for the scalar-input operators --------------------------------
 //
@@ -362,6 +242,257 @@ void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, co
                      0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen);
 }
 
+// ---- vector- and tensor-field operators: z-queue engine -----------------------------------
+//
+// The inputs are AoS (CIN values per site).  Each derivative direction reads only some of the
+// components: those differentiated along x are staged per plane in LDS with an x halo, those
+// along y with a y halo, and those along z are kept by the thread that owns the column in a
+// register queue of the 2R+1 planes z-R..z+R.  A block owns a 64 x 8 tile (two columns per
+// thread) and marches a chunk of planes; the next plane's values are fetched while the
+// current plane is formed.  Accumulation order is the reference's, as in the kernels above.
+
+template <int R, int NXc, int NYc, int NZc>
+struct VecTile {
+  static constexpr int OX = 64, OY = 8, NT = 256, COLS = OX * OY / NT;  // 2 columns per thread
+  static constexpr int PXW = OX + 2 * R;                                // x-staged plane width
+  static constexpr int PYH = OY + 2 * R;                                // y-staged plane height
+  static constexpr int NXH = 2 * R * OY, NYH = 2 * R * OX;              // halo sites
+  static constexpr int LXH = (NXH + NT - 1) / NT, LYH = (NYH + NT - 1) / NT;
+};
+
+template <class T, int O, class Op>
+__global__ __launch_bounds__(256) void k_vec_zq(Op op, const T* __restrict__ in,
+                                                T* __restrict__ out, Dims d, int ntx, int nty,
+                                                int zlen) {
+  constexpr int R = D1<O>::R, NXc = Op::NXc, NYc = Op::NYc, NZc = Op::NZc, CIN = Op::CIN;
+  using V = VecTile<R, NXc, NYc, NZc>;
+  __shared__ T px[NXc > 0 ? NXc : 1][V::OY * V::PXW];
+  __shared__ T py[NYc > 0 ? NYc : 1][V::PYH * V::OX];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * V::OX, y0 = ty * V::OY, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+  const int ox = t % V::OX, oy0 = t / V::OX;  // columns (ox, oy0) and (ox, oy0 + 4)
+
+  unsigned coff[V::COLS];  // own columns, wrapped (a column past the grid edge is a halo)
+  bool cin[V::COLS];
+#pragma unroll
+  for (int c = 0; c < V::COLS; ++c) {
+    const int gy = y0 + oy0 + c * (V::NT / V::OX);
+    cin[c] = x0 + ox < d.nx && gy < d.ny;
+    coff[c] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
+  }
+  // x halo: columns -R..-1 and OX..OX+R-1 of each row; y halo: rows -R..-1, OY..OY+R-1
+  unsigned xhoff[V::LXH], yhoff[V::LYH];
+  int xhslot[V::LXH], yhslot[V::LYH];
+#pragma unroll
+  for (int s = 0; s < V::LXH; ++s) {
+    const int h = t + s * V::NT;
+    const int row = h / (2 * R), k = h % (2 * R);
+    const int lx = k < R ? k : V::OX + k;  // plane column (0..PXW-1)
+    xhslot[s] = h < V::NXH ? row * V::PXW + lx : -1;
+    xhoff[s] = h < V::NXH ? (unsigned)(wrap_any(x0 + lx - R, d.nx) + d.nx * wrap_any(y0 + row, d.ny)) : 0u;
+  }
+#pragma unroll
+  for (int s = 0; s < V::LYH; ++s) {
+    const int h = t + s * V::NT;
+    const int col = h % V::OX, k = h / V::OX;
+    const int ly = k < R ? k : V::OY + k;  // plane row (0..PYH-1)
+    yhslot[s] = h < V::NYH ? ly * V::OX + col : -1;
+    yhoff[s] = h < V::NYH ? (unsigned)(wrap_any(x0 + col, d.nx) + d.nx * wrap_any(y0 + ly - R, d.ny)) : 0u;
+  }
+
+  T own[V::COLS][NXc + NYc > 0 ? NXc + NYc : 1];  // own columns, x- and y-staged comps
+  T xh[V::LXH][NXc > 0 ? NXc : 1], yh[V::LYH][NYc > 0 ? NYc : 1];
+  T znext[V::COLS][NZc > 0 ? NZc : 1];             // z-staged comps of plane z+R
+  T q[V::COLS][NZc > 0 ? NZc : 1][2 * R + 1];       // z queue, planes z-R..z+R
+
+  auto load_xy = [&](int plane) {
+    const T* ip = in + (size_t)CIN * (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c) {
+#pragma unroll
+      for (int i = 0; i < NXc; ++i) own[c][i] = ip[(size_t)CIN * coff[c] + Op::xc(i)];
+#pragma unroll
+      for (int i = 0; i < NYc; ++i) own[c][NXc + i] = ip[(size_t)CIN * coff[c] + Op::yc(i)];
+    }
+#pragma unroll
+    for (int s = 0; s < V::LXH; ++s)
+      if (xhslot[s] >= 0)
+#pragma unroll
+        for (int i = 0; i < NXc; ++i) xh[s][i] = ip[(size_t)CIN * xhoff[s] + Op::xc(i)];
+#pragma unroll
+    for (int s = 0; s < V::LYH; ++s)
+      if (yhslot[s] >= 0)
+#pragma unroll
+        for (int i = 0; i < NYc; ++i) yh[s][i] = ip[(size_t)CIN * yhoff[s] + Op::yc(i)];
+  };
+  auto load_z = [&](int plane, T (&dst)[V::COLS][NZc > 0 ? NZc : 1]) {
+    const T* ip = in + (size_t)CIN * (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c)
+#pragma unroll
+      for (int i = 0; i < NZc; ++i) dst[c][i] = ip[(size_t)CIN * coff[c] + Op::zc(i)];
+  };
+
+  // queue holds planes z0-R .. z0+R-1 in slots 0..2R-1 before the first iteration
+  for (int j = 0; j < 2 * R; ++j) {
+    T tmp[V::COLS][NZc > 0 ? NZc : 1];
+    load_z(z0 - R + j, tmp);
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c)
+#pragma unroll
+      for (int i = 0; i < NZc; ++i) {
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) q[c][i][m] = q[c][i][m + 1];
+        q[c][i][2 * R - 1] = tmp[c][i];
+      }
+  }
+  load_xy(z0);
+  load_z(z0 + R, znext);
+  for (int k = 0; k < nout; ++k) {
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c) {
+      const int oy = oy0 + c * (V::NT / V::OX);
+#pragma unroll
+      for (int i = 0; i < NXc; ++i) px[i][oy * V::PXW + ox + R] = own[c][i];
+#pragma unroll
+      for (int i = 0; i < NYc; ++i) py[i][(oy + R) * V::OX + ox] = own[c][NXc + i];
+#pragma unroll
+      for (int i = 0; i < NZc; ++i) q[c][i][2 * R] = znext[c][i];
+    }
+#pragma unroll
+    for (int s = 0; s < V::LXH; ++s)
+      if (xhslot[s] >= 0)
+#pragma unroll
+        for (int i = 0; i < NXc; ++i) px[i][xhslot[s]] = xh[s][i];
+#pragma unroll
+    for (int s = 0; s < V::LYH; ++s)
+      if (yhslot[s] >= 0)
+#pragma unroll
+        for (int i = 0; i < NYc; ++i) py[i][yhslot[s]] = yh[s][i];
+    __syncthreads();
+    if (k + 1 < nout) {
+      load_xy(z0 + k + 1);
+      load_z(z0 + k + 1 + R, znext);
+    }
+    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c) {
+      if (cin[c]) {
+        const int oy = oy0 + c * (V::NT / V::OX);
+        auto X = [&](int i, int o) { return px[i][oy * V::PXW + ox + R + o]; };
+        auto Y = [&](int i, int o) { return py[i][(oy + R + o) * V::OX + ox]; };
+        auto Z = [&](int i, int o) { return q[c][i][R + o]; };
+        op.template apply<O>(X, Y, Z, out, zo + coff[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < V::COLS; ++c)
+#pragma unroll
+      for (int i = 0; i < NZc; ++i)
+#pragma unroll
+        for (int m = 0; m < 2 * R; ++m) q[c][i][m] = q[c][i][m + 1];
+    __syncthreads();
+  }
+}
+
+// diff/Divergence.hpp:56-69 (Vec3 -> scalar): V_x along x, V_y along y, V_z along z
+template <class T>
+struct OpDivVec {
+  static constexpr int CIN = 3, NXc = 1, NYc = 1, NZc = 1;
+  __host__ __device__ static constexpr int xc(int) { return 0; }
+  __host__ __device__ static constexpr int yc(int) { return 1; }
+  __host__ __device__ static constexpr int zc(int) { return 2; }
+  T ihx, ihy, ihz;
+  template <int O, class FX, class FY, class FZ>
+  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+    using S = D1<O>;
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int m = 0; m < S::N; ++m) {
+      const int o = m - S::R;
+      const T w = T(S::w(m));
+      ax += w * X(0, o);
+      ay += w * Y(0, o);
+      az += w * Z(0, o);
+    }
+    out[c] = ax * ihx + ay * ihy + az * ihz;
+  }
+};
+
+// diff/Divergence.hpp:113-131 (Mat3 -> Vec3): column a of M along axis a
+template <class T>
+struct OpDivMat {
+  static constexpr int CIN = 9, NXc = 3, NYc = 3, NZc = 3;
+  __host__ __device__ static constexpr int xc(int i) { return i; }
+  __host__ __device__ static constexpr int yc(int i) { return 3 + i; }
+  __host__ __device__ static constexpr int zc(int i) { return 6 + i; }
+  T ihx, ihy, ihz;
+  template <int O, class FX, class FY, class FZ>
+  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+    using S = D1<O>;
+    T* o3 = out + 3 * c;
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+      for (int m = 0; m < S::N; ++m) {
+        const int o = m - S::R;
+        const T w = T(S::w(m));
+        dx += w * X(comp, o);
+        dy += w * Y(comp, o);
+        dz += w * Z(comp, o);
+      }
+      o3[comp] = dx * ihx + dy * ihy + dz * ihz;
+    }
+  }
+};
+
+// diff/Gradient.hpp:113-134 (Vec3 -> Mat3, M(comp, axis) at comp + 3*axis): every component
+// along every axis
+template <class T>
+struct OpGradVec {
+  static constexpr int CIN = 3, NXc = 3, NYc = 3, NZc = 3;
+  __host__ __device__ static constexpr int xc(int i) { return i; }
+  __host__ __device__ static constexpr int yc(int i) { return i; }
+  __host__ __device__ static constexpr int zc(int i) { return i; }
+  T ihx, ihy, ihz;
+  template <int O, class FX, class FY, class FZ>
+  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+    using S = D1<O>;
+    T* o9 = out + 9 * c;
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+      for (int m = 0; m < S::N; ++m) {
+        const int o = m - S::R;
+        const T w = T(S::w(m));
+        dx += w * X(comp, o);
+        dy += w * Y(comp, o);
+        dz += w * Z(comp, o);
+      }
+      o9[comp + 0] = dx * ihx;
+      o9[comp + 3] = dy * ihy;
+      o9[comp + 6] = dz * ihz;
+    }
+  }
+};
+
+constexpr int kVLen = 32;  // planes per block
+
+template <class T, int O, class Op>
+void vec_launch(gc_context* ctx, const Dims& d, const Op& op, const T* in, T* out) {
+  const int ntx = (d.nx + 63) / 64, nty = (d.ny + 7) / 8, nzc = (d.nz + kVLen - 1) / kVLen;
+  hipLaunchKernelGGL((k_vec_zq<T, O, Op>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
+                     ctx->stream, op, in, out, d, ntx, nty, kVLen);
+}
+
 template <class T>
 struct Inv {
   T x, y, z;
@@ -414,7 +545,6 @@ gc_status common_checks(gc_context* ctx, const gc_grid* g, gc_dtype dt, int orde
 template <class T, int O>
 void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void* in, void* out) {
   const Dims d = dims_of(g);
-  const Launch L = launch_for(d);
   const Inv<double> ih = inv_h(g);
   const T x = T(ih.x), y = T(ih.y), z = T(ih.z);
   switch (which) {
@@ -422,16 +552,13 @@ void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void
       zmarch_launch<T>(ctx, d, OpGrad<T, O>{x, y, z}, (const T*)in, nullptr, (T*)out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_gradient_vec<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
-                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      vec_launch<T, O>(ctx, d, OpGradVec<T>{x, y, z}, (const T*)in, (T*)out);
       break;
     case 2:
-      hipLaunchKernelGGL((k_divergence<T, O>), L.grid, L.block, 0, ctx->stream, (const T*)in,
-                         (T*)out, d, L.nbx, L.nby, x, y, z);
+      vec_launch<T, O>(ctx, d, OpDivVec<T>{x, y, z}, (const T*)in, (T*)out);
       break;
     default:
-      hipLaunchKernelGGL((k_divergence_mat<T, O>), L.grid, L.block, 0, ctx->stream,
-                         (const T*)in, (T*)out, d, L.nbx, L.nby, x, y, z);
+      vec_launch<T, O>(ctx, d, OpDivMat<T>{x, y, z}, (const T*)in, (T*)out);
       break;
   }
 }
