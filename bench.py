@@ -17,6 +17,7 @@ prefix of the same workload.
   python bench.py --workload ensemble             # 64 uneven 2-D members, graph-replayed
   python bench.py --workload learned [--n-ic 4]   # learned corrector vs plain RK4 to T=20
   python bench.py --workload tune [--n 256]       # autotuner: ms/step of every tile shape
+  python bench.py --workload diffuse [--n 256]    # constant- and heterogeneous-D diffusion RK4
 """
 from __future__ import annotations
 
@@ -46,7 +47,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
-                                                          "learned", "tune"])
+                                                          "learned", "tune", "diffuse"])
     p.add_argument("--n-ic", type=int, default=4)
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--diag-every", type=int, default=500)
@@ -457,6 +458,60 @@ def learned(a):
                          "pairs": "snapshot m -> m+1 of 2000 FP64 GPU RK4 trajectories"}}
 
 
+def diffuse(a):
+    """Heterogeneous-D diffusion (fvm::cellLaplacian right-hand side) and constant-D diffusion,
+    3-D 256^3, h = 2 pi / 256, RK4: ms per step and the roofline on 16 W per site per step (the
+    stage pipeline) + 4 W for reading D once per stage (heterogeneous D).  D ~ 0.1 + 0.05
+    cos(x); psi ~ U(-1, 1); dt at 0.4 of the CFL bound.  Both FP64 and FP32."""
+    gc = entry.load_package()
+    n = a.n or 256
+    h = 2 * np.pi / n
+    g = gc.Grid.make(n, n, n, h)
+    psi = gc.inputs.symmetric(61, n ** 3).reshape(n, n, n)
+    x = np.arange(n) * h
+    Dh = np.broadcast_to(0.1 + 0.05 * np.cos(x)[None, None, :], (n, n, n)).copy()
+    dt = 0.4 * 0.6963 * h * h / (3 * 0.15)
+    steps = min(a.steps, 200)
+    rows = []
+    with gc.Context() as ctx:
+        for dtype, w in ((np.float64, 8), (np.float32, 4)):
+            d_p = gc.DeviceArray.from_numpy(ctx, psi.astype(dtype))
+            d_D = gc.DeviceArray.from_numpy(ctx, Dh.astype(dtype))
+            for kind in ("field", "const"):
+                def run(k):
+                    if kind == "field":
+                        gc.diffuse_field(ctx, g, d_p, d_D, dt, k, gc.RK4)
+                    else:
+                        gc.diffuse_const(ctx, g, d_p, 0.15, dt, k, gc.RK4)
+                run(2)
+                ctx.sync()
+                ctx.timer_start()
+                run(steps)
+                ms = ctx.timer_stop() / steps
+                words = 20 if kind == "field" else 16
+                gbs = words * w * n ** 3 / (ms / 1e3) / 1e9
+                rows.append({"D": kind, "dtype": "f64" if w == 8 else "f32", "ms_per_step": ms,
+                             "site_updates_per_s": n ** 3 / (ms / 1e3), "GBs_alg": gbs,
+                             "hbm_frac": gbs / HBM_PEAK_GBS})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+            d_p.free()
+            d_D.free()
+    cpu = None
+    if not a.no_cpu:
+        orc = entry.load_oracle()
+        orc.lib()
+        _pin_one_core()
+        og = orc.Grid.make(n, n, n, h)
+        t0 = time.perf_counter()
+        orc.diffuse_field(og, psi, Dh, dt, 2, orc.RK4)
+        el = time.perf_counter() - t0
+        cpu = {"value": n ** 3 * 2 / el, "unit": "site-updates/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle heterogeneous-D RK4 (FP64, gcc -O2, 1 thread): 2 steps of "
+                         f"the same {n}^3 input ({el:.1f} s)"}
+    return {"metric": "diffusion RK4 ms per step", "grid": [n, n, n], "steps": steps, "rows": rows,
+            "cpu_baseline": cpu}
+
+
 def tune(a):
     """The tile autotuner's measurements for the 3-D workload's grid (default 256^3): ms per
     RK4 step of every candidate shape, one-stage and two-stage kernels, FP32 and FP64."""
@@ -502,7 +557,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble,
-           "learned": learned, "tune": tune}[a.workload](a)
+           "learned": learned, "tune": tune, "diffuse": diffuse}[a.workload](a)
     print(json.dumps(out))
 
 

@@ -495,8 +495,9 @@ __global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, const T* __res
 }
 
 template <class T, class Rhs>
-gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int stage, const T* y,
-                       T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+gc_status launch_stage_untiled(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int stage,
+                               const T* y, T* psi, T* ynext, T* a, T* b,
+                               const StageCoef<T>& co) {
   const Dims d = dims_of(g);
   const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;
   const dim3 grid((unsigned)(nbx * nby * d.nz));
@@ -504,6 +505,164 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const Rhs& rhs, int st
                      ynext, a, b, d, nbx, nby, co);
   GC_LAUNCHED();
   return GC_OK;
+}
+
+// ---- LDS-tiled diffusion stage kernel (3-D) ----------------------------------------------
+//
+// The constant-D (laplacian<2>) and heterogeneous-D (fvm::cellLaplacian) right-hand sides
+// have radius 1: a block keeps the planes z-1, z, z+1 of the stage input (and of D) for its
+// 64 x 8 tile + 1-site halo in a 3-plane LDS ring and marches a chunk of planes; the next
+// plane and the stage operands are fetched while the current plane is formed.  Arithmetic as
+// LapRhs / FvmRhs.
+
+template <class T, int OX, int OY, int ZC, int STAGE, bool FVM>
+__global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfield, T ihx2,
+                                                      T ihy2, T ihz2, const T* __restrict__ y,
+                                                      T* psi, T* __restrict__ ynext,
+                                                      T* __restrict__ a, T* __restrict__ b,
+                                                      Dims d, int ntx, int nty,
+                                                      StageCoef<T> co) {
+  constexpr int NT = 256, PW = OX + 2, PN = PW * (OY + 2), PL = (PN + NT - 1) / NT;
+  constexpr int ROWS = NT / OX, OR = OX * OY / NT;
+  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
+  __shared__ T ys[3][PN];
+  __shared__ T ds[FVM ? 3 : 1][FVM ? PN : 1];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned r = bid / (unsigned)ntx;
+  const int ty = (int)(r % (unsigned)nty);
+  const int zc = (int)(r / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
+  const int nout = min(ZC, d.nz - z0);
+
+  unsigned poff[PL];
+#pragma unroll
+  for (int s = 0; s < PL; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % PW, py = idx / PW;
+    poff[s] = idx < PN ? (unsigned)(wrap_any(x0 + px - 1, d.nx) + d.nx * wrap_any(y0 + py - 1, d.ny)) : 0u;
+  }
+  const int ox = t % OX, oy0 = t / OX;
+  unsigned ooff[OR];
+  bool oin[OR];
+#pragma unroll
+  for (int q = 0; q < OR; ++q) {
+    const int gy = y0 + oy0 + q * ROWS;
+    oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
+    ooff[q] = oin[q] ? (unsigned)(x0 + ox + d.nx * gy) : 0u;
+  }
+  T yreg[PL], dreg[FVM ? PL : 1];
+  auto load = [&](int plane) {
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+      if (t + s * NT < PN) {
+        yreg[s] = y[zo + poff[s]];
+        if constexpr (FVM) dreg[s] = Dfield[zo + poff[s]];
+      }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+      if (t + s * NT < PN) {
+        ys[slot][t + s * NT] = yreg[s];
+        if constexpr (FVM) ds[slot][t + s * NT] = dreg[s];
+      }
+  };
+  T pv[OR] = {}, av[OR] = {}, bv[OR] = {};
+
+  // iteration p stores plane z0-1+p in slot p%3 and forms output plane z0+p-2
+  load(z0 - 1);
+  const int last = nout + 1;
+  for (int p = 0; p <= last; ++p) {
+    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // slots of p, p-1, p-2
+    store(s0);
+    if (p < last) load(z0 + p);
+    if (p >= 2) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 + p - 2);
+#pragma unroll
+      for (int q = 0; q < OR; ++q)
+        if (oin[q]) stage_in<STAGE>(psi + kb, a + kb, b + kb, ooff[q], pv[q], av[q], bv[q]);
+    }
+    __syncthreads();
+    if (p >= 2) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 + p - 2);
+#pragma unroll
+      for (int q = 0; q < OR; ++q) {
+        if (oin[q]) {
+          const int o = (oy0 + q * ROWS + 1) * PW + (ox + 1);
+          const T* yc = ys[s1];
+          const T* ylo = ys[s2];
+          const T* yhi = ys[s0];
+          T kv;
+          if constexpr (FVM) {
+            const T* dc = ds[s1];
+            const T pc = yc[o], Dc = dc[o];
+            const T dxp = FvmRhs<T>::hm(Dc, dc[o + 1]), dxm = FvmRhs<T>::hm(dc[o - 1], Dc);
+            const T dyp = FvmRhs<T>::hm(Dc, dc[o + PW]), dym = FvmRhs<T>::hm(dc[o - PW], Dc);
+            const T dzp = FvmRhs<T>::hm(Dc, ds[s0][o]), dzm = FvmRhs<T>::hm(ds[s2][o], Dc);
+            const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
+            const T ry = (dyp * (yc[o + PW] - pc) - dym * (pc - yc[o - PW])) * ihy2;
+            const T rz = (dzp * (yhi[o] - pc) - dzm * (pc - ylo[o])) * ihz2;
+            kv = rx + ry + rz;
+          } else {
+            kv = lap_lds(yc, ylo, yhi, o, PW, ihx2, ihy2, ihz2);
+          }
+          if constexpr (base_from_y<STAGE>()) pv[q] = yc[o];
+          stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi + kb, ynext + kb, a + kb,
+                           b + kb, co);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+#define GC_STAGE_SWITCH(LAUNCH)      \
+  switch (stage) {                   \
+    case kRhsOnly: LAUNCH(kRhsOnly); break; \
+    case kS1: LAUNCH(kS1); break;    \
+    case kS2: LAUNCH(kS2); break;    \
+    case kS3: LAUNCH(kS3); break;    \
+    case kS4: LAUNCH(kS4); break;    \
+    default: LAUNCH(kEuler); break;  \
+  }
+
+constexpr int kDiffOX = 64, kDiffOY = 8, kDiffZC = 32;
+
+// Grids with Nz >= 4 run the tiled kernel; flatter grids the untiled one.
+template <class T, bool FVM>
+gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, T ihx2, T ihy2, T ihz2,
+                            int stage, const T* y, T* psi, T* ynext, T* a, T* b,
+                            const StageCoef<T>& co) {
+  const Dims d = dims_of(g);
+  const int ntx = (d.nx + kDiffOX - 1) / kDiffOX, nty = (d.ny + kDiffOY - 1) / kDiffOY;
+  const int nzc = (d.nz + kDiffZC - 1) / kDiffZC;
+  const dim3 grid((unsigned)(ntx * nty * nzc));
+#define GC_LD(S)                                                                               \
+  hipLaunchKernelGGL((k_diff_stage3d<T, kDiffOX, kDiffOY, kDiffZC, S, FVM>), grid, dim3(256), 0, \
+                     ctx->stream, D, ihx2, ihy2, ihz2, y, psi, ynext, a, b, d, ntx, nty, co)
+  GC_STAGE_SWITCH(GC_LD)
+#undef GC_LD
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+template <class T>
+gc_status launch_stage(gc_context* ctx, const gc_grid* g, const LapRhs<T>& rhs, int stage,
+                       const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  if (g->nz < 4) return launch_stage_untiled<T>(ctx, g, rhs, stage, y, psi, ynext, a, b, co);
+  return launch_diff_stage<T, false>(ctx, g, nullptr, rhs.ihx2, rhs.ihy2, rhs.ihz2, stage, y, psi,
+                                     ynext, a, b, co);
+}
+
+template <class T>
+gc_status launch_stage(gc_context* ctx, const gc_grid* g, const FvmRhs<T>& rhs, int stage,
+                       const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
+  if (g->nz < 4) return launch_stage_untiled<T>(ctx, g, rhs, stage, y, psi, ynext, a, b, co);
+  return launch_diff_stage<T, true>(ctx, g, rhs.D, rhs.ihx2, rhs.ihy2, rhs.ihz2, stage, y, psi,
+                                    ynext, a, b, co);
 }
 
 // ---- two RK4 stages per pass (3-D) ---------------------------------------------------------
@@ -784,15 +943,6 @@ inline bool is_pair(const Dims& d, int id) { return d.nz > 1 && id >= kN3; }
 // 64x16x64)
 constexpr int kDefault3F32 = kN3 + 8, kDefault3F64 = kN3 + 3;
 
-#define GC_STAGE_SWITCH(LAUNCH)      \
-  switch (stage) {                   \
-    case kRhsOnly: LAUNCH(kRhsOnly); break; \
-    case kS1: LAUNCH(kS1); break;    \
-    case kS2: LAUNCH(kS2); break;    \
-    case kS3: LAUNCH(kS3); break;    \
-    case kS4: LAUNCH(kS4); break;    \
-    default: LAUNCH(kEuler); break;  \
-  }
 
 template <class T, int OX, int OY, int ZC>
 void launch_ch3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int stage, const T* y,

@@ -68,7 +68,7 @@ def test_ch_zero_rhs_constant_field(gc, ctx):
 
 
 @pytest.mark.parametrize("integ", [0, 1])
-@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 24, 1)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 24, 1), (70, 9, 40), (8, 12, 5)])
 def test_diffuse_const_bit_exact(gc, orc, ctx, integ, shape):
     h = 2 * np.pi / shape[0]
     g, og = _grids(gc, orc, *shape, h=h)
@@ -80,7 +80,7 @@ def test_diffuse_const_bit_exact(gc, orc, ctx, integ, shape):
 
 
 @pytest.mark.parametrize("integ", [0, 1])
-@pytest.mark.parametrize("shape", [(16, 16, 16), (40, 8, 1)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (40, 8, 1), (70, 9, 40), (8, 12, 5)])
 def test_diffuse_field_bit_exact(gc, orc, ctx, integ, shape):
     h = 2 * np.pi / shape[0]
     g, og = _grids(gc, orc, *shape, h=h)
