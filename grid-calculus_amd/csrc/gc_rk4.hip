@@ -619,6 +619,247 @@ __global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfie
   }
 }
 
+// Two diffusion stages per pass (PAIR 0 = S1+S2, PAIR 1 = S3+S4), the scheme of the
+// Cahn-Hilliard two-stage kernels below with radius-1 right-hand sides: the first stage on
+// the tile + 1-site halo (its stage value stays in LDS), the second on the tile; stage input
+// and D read with a 2-site halo.  Per site and step 9 W (+ 2 W of D) instead of 17 W (+ 4 W).
+// Iteration p stores input plane z0-2+p, forms k_A on plane z0-3+p and the outputs on plane
+// z0-4+p; rings: input 3 planes, D 4 planes (the second stage reaches one plane further
+// back), y' 3 planes; three barriers per iteration.
+
+template <int OX, int OY, int NT>
+struct DPairCfg {
+  static constexpr int W2 = OX + 4, N2 = W2 * (OY + 4);  // stage input, D
+  static constexpr int W1 = OX + 2, N1 = W1 * (OY + 2);  // k_A, y'
+  static constexpr int L2 = (N2 + NT - 1) / NT;
+  static constexpr int NR = 2 * W1 + 2 * OY;              // tile+1 sites outside the tile
+  static constexpr int LR = (NR + NT - 1) / NT;
+  static constexpr int OR = OX * OY / NT, ROWS = NT / OX;
+  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
+};
+
+// Ring index r -> (mx, my) in tile+1 coordinates: the row above, the row below, the columns
+// either side.
+template <class C, int OY>
+__device__ __forceinline__ void ring1_site(int r, int& mx, int& my) {
+  if (r < C::W1) {
+    mx = r;
+    my = 0;
+  } else if (r < 2 * C::W1) {
+    mx = r - C::W1;
+    my = OY + 1;
+  } else {
+    r -= 2 * C::W1;
+    my = 1 + (r >> 1);
+    mx = (r & 1) ? C::W1 - 1 : 0;
+  }
+}
+
+// right-hand side at LDS offset o (row width W) from y planes (lo, c, hi) and D planes
+template <class T, bool FVM>
+__device__ __forceinline__ T diff_rhs(const T* yc, const T* ylo, const T* yhi, const T* dc,
+                                      const T* dlo, const T* dhi, int o, int W, T ihx2, T ihy2,
+                                      T ihz2) {
+  if constexpr (FVM) {
+    const T pc = yc[o], Dc = dc[o];
+    const T dxp = FvmRhs<T>::hm(Dc, dc[o + 1]), dxm = FvmRhs<T>::hm(dc[o - 1], Dc);
+    const T dyp = FvmRhs<T>::hm(Dc, dc[o + W]), dym = FvmRhs<T>::hm(dc[o - W], Dc);
+    const T dzp = FvmRhs<T>::hm(Dc, dhi[o]), dzm = FvmRhs<T>::hm(dlo[o], Dc);
+    const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
+    const T ry = (dyp * (yc[o + W] - pc) - dym * (pc - yc[o - W])) * ihy2;
+    const T rz = (dzp * (yhi[o] - pc) - dzm * (pc - ylo[o])) * ihz2;
+    return rx + ry + rz;
+  } else {
+    return lap_lds(yc, ylo, yhi, o, W, ihx2, ihy2, ihz2);
+  }
+}
+
+template <class T, int OX, int OY, int ZC, int NT, int PAIR, bool FVM>
+__global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield, T ihx2, T ihy2,
+                                                    T ihz2, const T* __restrict__ y,
+                                                    const T* __restrict__ psi,
+                                                    T* __restrict__ out, T* __restrict__ a,
+                                                    T* __restrict__ b, Dims d, int ntx, int nty,
+                                                    StageCoef<T> co) {
+  using C = DPairCfg<OX, OY, NT>;
+  constexpr int ND = FVM ? 4 : 1;
+  __shared__ T ys[3][C::N2];
+  __shared__ T ds[ND][FVM ? C::N2 : 1];
+  __shared__ T y2s[3][C::N1];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned r = bid / (unsigned)ntx;
+  const int ty = (int)(r % (unsigned)nty);
+  const int zc = (int)(r / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
+  const int nout = min(ZC, d.nz - z0);
+
+  unsigned yoff[C::L2];
+#pragma unroll
+  for (int s = 0; s < C::L2; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % C::W2, py = idx / C::W2;
+    yoff[s] = idx < C::N2 ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
+                          : 0u;
+  }
+  const int ox = t % OX, oy0 = t / OX;
+  unsigned ooff[C::OR];  // tile sites, wrapped; oin: the site is an output
+  bool oin[C::OR];
+#pragma unroll
+  for (int q = 0; q < C::OR; ++q) {
+    const int gy = y0 + oy0 + q * C::ROWS;
+    oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
+    ooff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
+  }
+  unsigned roff[PAIR == 1 ? C::LR : 1];  // PAIR 1: psi at the ring sites of stage A
+  if constexpr (PAIR == 1) {
+#pragma unroll
+    for (int s = 0; s < C::LR; ++s) {
+      const int rr = t + s * NT;
+      int mx = 0, my = 0;
+      if (rr < C::NR) ring1_site<C, OY>(rr, mx, my);
+      roff[s] = rr < C::NR ? (unsigned)(wrap_any(x0 + mx - 1, d.nx) + d.nx * wrap_any(y0 + my - 1, d.ny))
+                           : 0u;
+    }
+  }
+
+  T yreg[C::L2], dreg[FVM ? C::L2 : 1];
+  auto load = [&](int plane) {
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < C::L2; ++s)
+      if (t + s * NT < C::N2) {
+        yreg[s] = y[zo + yoff[s]];
+        if constexpr (FVM) dreg[s] = Dfield[zo + yoff[s]];
+      }
+  };
+  T pe[PAIR == 1 ? C::LR : 1];  // PAIR 1: psi at the ring sites, plane of k_A
+  T pt[PAIR == 1 ? C::OR : 1];  // PAIR 1: psi at the tile sites, plane of k_A
+  T k0r[PAIR == 1 ? C::OR : 1] = {}, k1r[PAIR == 1 ? C::OR : 1] = {};  // k_A, newest first
+  T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+
+  load(z0 - 2);
+  const int last = nout + 3;
+  for (int p = 0; p <= last; ++p) {
+    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // y/y' slots of p, p-1, p-2
+    const int d0 = p % ND, d1 = (p + ND - 1) % ND, d2 = (p + ND - 2) % ND, d3 = (p + ND - 3) % ND;
+#pragma unroll
+    for (int s = 0; s < C::L2; ++s)
+      if (t + s * NT < C::N2) {
+        ys[s0][t + s * NT] = yreg[s];
+        if constexpr (FVM) ds[d0][t + s * NT] = dreg[s];
+      }
+    if (p < last) load(z0 - 1 + p);
+    if constexpr (PAIR == 1) {
+      if (p >= 2) {
+        const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 3 + p, d.nz);
+#pragma unroll
+        for (int s = 0; s < C::LR; ++s)
+          if (t + s * NT < C::NR) pe[s] = pp[roff[s]];
+#pragma unroll
+        for (int q = 0; q < C::OR; ++q) pt[q] = pp[ooff[q]];
+      }
+      if (p >= 4) {
+        const size_t kb = (size_t)d.plane * (size_t)(z0 - 4 + p);
+#pragma unroll
+        for (int q = 0; q < C::OR; ++q) {
+          if (oin[q]) {
+            pv[q] = __builtin_nontemporal_load(psi + kb + ooff[q]);
+            av[q] = __builtin_nontemporal_load(a + kb + ooff[q]);
+            bv[q] = __builtin_nontemporal_load(b + kb + ooff[q]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 2) {  // k_A and y' on plane z0-3+p, tile+1
+      const int zA = z0 - 3 + p;
+      const bool own = zA >= z0 && zA < z0 + nout;
+      auto kA = [&](int mx, int my) {
+        const int o = (my + 1) * C::W2 + (mx + 1);
+        return diff_rhs<T, FVM>(ys[s1], ys[s2], ys[s0], FVM ? ds[d1] : nullptr,
+                                FVM ? ds[d2] : nullptr, FVM ? ds[d0] : nullptr, o, C::W2, ihx2,
+                                ihy2, ihz2);
+      };
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q) {  // tile sites: the output owner
+        const int mx = ox + 1, my = oy0 + q * C::ROWS + 1;
+        const T kv = kA(mx, my);
+        const int e = my * C::W1 + mx;
+        if constexpr (PAIR == 0) {
+          y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
+          if (own && oin[q])
+            __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + ooff[q]);
+        } else {
+          y2s[s0][e] = pt[q] + co.dt * kv;
+          k1r[q] = k0r[q];
+          k0r[q] = kv;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < C::LR; ++s) {  // the 1-site ring around the tile
+        const int rr = t + s * NT;
+        if (rr < C::NR) {
+          int mx, my;
+          ring1_site<C, OY>(rr, mx, my);
+          const T kv = kA(mx, my);
+          const int e = my * C::W1 + mx;
+          if constexpr (PAIR == 0) y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
+          else y2s[s0][e] = pe[s] + co.dt * kv;
+        }
+      }
+      if constexpr (PAIR == 0) {  // base psi of the output plane z0-4+p (input plane p-2)
+        if (p >= 4) {
+#pragma unroll
+          for (int q = 0; q < C::OR; ++q)
+            pv[q] = ys[s2][(oy0 + q * C::ROWS + 2) * C::W2 + (ox + 2)];
+        }
+      }
+    }
+    __syncthreads();
+    if (p >= 4) {  // outputs on plane z0-4+p from y' planes p-2..p and D planes p-3..p-1
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 4 + p);
+#pragma unroll
+      for (int q = 0; q < C::OR; ++q) {
+        if (oin[q]) {
+          const int oy = oy0 + q * C::ROWS;
+          const int o1 = (oy + 1) * C::W1 + (ox + 1);
+          T kv;
+          if constexpr (FVM) {
+            // D on the tile+1 region of planes p-3 (lo), p-2 (centre), p-1 (hi), stored with the
+            // 2-site halo: offset (oy+2)*W2 + (ox+2)
+            const int o2 = (oy + 2) * C::W2 + (ox + 2);
+            const T* yc = y2s[s1];
+            const T* ylo = y2s[s2];
+            const T* yhi = y2s[s0];
+            const T* dc = ds[d2];
+            const T pc = yc[o1], Dc = dc[o2];
+            const T dxp = FvmRhs<T>::hm(Dc, dc[o2 + 1]), dxm = FvmRhs<T>::hm(dc[o2 - 1], Dc);
+            const T dyp = FvmRhs<T>::hm(Dc, dc[o2 + C::W2]), dym = FvmRhs<T>::hm(dc[o2 - C::W2], Dc);
+            const T dzp = FvmRhs<T>::hm(Dc, ds[d1][o2]), dzm = FvmRhs<T>::hm(ds[d3][o2], Dc);
+            const T rx = (dxp * (yc[o1 + 1] - pc) - dxm * (pc - yc[o1 - 1])) * ihx2;
+            const T ry = (dyp * (yc[o1 + C::W1] - pc) - dym * (pc - yc[o1 - C::W1])) * ihy2;
+            const T rz = (dzp * (yhi[o1] - pc) - dzm * (pc - ylo[o1])) * ihz2;
+            kv = rx + ry + rz;
+          } else {
+            kv = lap_lds(y2s[s1], y2s[s2], y2s[s0], o1, C::W1, ihx2, ihy2, ihz2);
+          }
+          if constexpr (PAIR == 0) {
+            __builtin_nontemporal_store(kv, b + kb + ooff[q]);
+            __builtin_nontemporal_store(pv[q] + co.half_dt * kv, out + kb + ooff[q]);
+          } else {
+            const T bsum = bv[q] + k1r[q];  // k3 of this plane (k_A pass of iteration p-1)
+            __builtin_nontemporal_store(pv[q] + (co.sixth_dt * (av[q] + kv) + co.third_dt * bsum),
+                                        out + kb + ooff[q]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 #define GC_STAGE_SWITCH(LAUNCH)      \
   switch (stage) {                   \
     case kRhsOnly: LAUNCH(kRhsOnly); break; \
@@ -647,6 +888,40 @@ gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, T ihx
 #undef GC_LD
   GC_LAUNCHED();
   return GC_OK;
+}
+
+// Two-stage diffusion kernels: 64 x 16 tiles, 1024 threads, 32-plane chunks; grids with
+// Nz >= 8.
+constexpr int kDpOX = 64, kDpOY = 16, kDpZC = 32, kDpNT = 1024;
+
+template <class T, bool FVM>
+gc_status diff_pair_step(gc_context* ctx, const gc_grid* g, const T* D, T ihx2, T ihy2, T ihz2,
+                         const T* psi_in, T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
+  const Dims d = dims_of(g);
+  const int ntx = (d.nx + kDpOX - 1) / kDpOX, nty = (d.ny + kDpOY - 1) / kDpOY;
+  const int nzc = (d.nz + kDpZC - 1) / kDpZC;
+  const dim3 grid((unsigned)(ntx * nty * nzc));
+  hipLaunchKernelGGL((k_diff_pair3d<T, kDpOX, kDpOY, kDpZC, kDpNT, 0, FVM>), grid, dim3(kDpNT), 0,
+                     ctx->stream, D, ihx2, ihy2, ihz2, psi_in, psi_in, y3, a, b, d, ntx, nty, co);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL((k_diff_pair3d<T, kDpOX, kDpOY, kDpZC, kDpNT, 1, FVM>), grid, dim3(kDpNT), 0,
+                     ctx->stream, D, ihx2, ihy2, ihz2, y3, psi_in, psi_out, a, b, d, ntx, nty, co);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+template <class T>
+gc_status pair_step(gc_context* ctx, const gc_grid* g, const LapRhs<T>& rhs, const T* psi_in,
+                    T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
+  return diff_pair_step<T, false>(ctx, g, nullptr, rhs.ihx2, rhs.ihy2, rhs.ihz2, psi_in, psi_out,
+                                  y3, a, b, co);
+}
+
+template <class T>
+gc_status pair_step(gc_context* ctx, const gc_grid* g, const FvmRhs<T>& rhs, const T* psi_in,
+                    T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
+  return diff_pair_step<T, true>(ctx, g, rhs.D, rhs.ihx2, rhs.ihy2, rhs.ihz2, psi_in, psi_out, y3,
+                                 a, b, co);
 }
 
 template <class T>
@@ -1131,6 +1406,18 @@ gc_status run_steps(gc_context* ctx, const gc_grid* g, const Rhs& rhs, T* psi, d
           GC_HIP(hipMemcpyAsync(psi, cur, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
         return GC_OK;
       }
+    } else if (g->nz >= 8) {  // diffusion: two stages per pass
+      T* cur = psi;
+      T* other = yA;
+      for (int s = 0; s < n_steps; ++s) {
+        GC_TRY(pair_step<T>(ctx, g, rhs, cur, other, yB, a, b, co));
+        T* tmp = cur;
+        cur = other;
+        other = tmp;
+      }
+      if (cur != psi)
+        GC_HIP(hipMemcpyAsync(psi, cur, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+      return GC_OK;
     }
     for (int s = 0; s < n_steps; ++s) {
       GC_TRY(launch_stage<T>(ctx, g, rhs, kS1, psi, psi, yA, a, b, co));
