@@ -155,6 +155,7 @@ def lib():
         "gc_cnn_get_params": ([vp, vp], st),
         "gc_cnn_get_grads": ([vp, vp], st),
         "gc_cnn_forward": ([vp, C.c_int, vp, C.c_int, vp], st),
+        "gc_cnn_forward_layers": ([vp, C.c_int, vp, C.c_int, vp], st),
         "gc_cnn_loss_grad": ([vp, C.c_int, vp, vp, ip, C.c_int, dp], st),
         "gc_cnn_adam": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], st),
         "gc_cnn_predict": ([vp, C.c_int, vp, vp, C.c_int], st),
@@ -575,9 +576,11 @@ class Corrector:
         _check(lib().gc_cnn_get_grads(self._h, g.ctypes.data_as(C.c_void_p)))
         return g
 
-    def forward(self, c, B):
-        """Raw network output for the first B images of device array c -> numpy (B, H, W)."""
-        _check(lib().gc_cnn_forward(self._h, c.gc_dtype, c.ptr, B, self.out.ptr))
+    def forward(self, c, B, layers=False):
+        """Raw network output for the first B images of device array c -> numpy (B, H, W);
+        layers=True runs the per-layer kernels instead of the fused inference kernel."""
+        fn = lib().gc_cnn_forward_layers if layers else lib().gc_cnn_forward
+        _check(fn(self._h, c.gc_dtype, c.ptr, B, self.out.ptr))
         return self.out.numpy(self.ctx)[:B]
 
     def loss_grad(self, c, target, idx=None, B=None):

@@ -46,8 +46,23 @@ static_assert(kNParams == 37601, "parameter count");
 
 constexpr int kTR = 8, kTC = 32, kSP = 40;  // conv tile rows, cols, LDS site stride (bf16)
 
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
+// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x) for
+// x >= 0 (|error| <= 1.5e-7), odd extension for x < 0.
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = 1.0f / fmaf(0.3275911f, ax, 1.0f);
+  float poly = fmaf(t, 1.061405429f, -1.453152027f);
+  poly = fmaf(t, poly, 1.421413741f);
+  poly = fmaf(t, poly, -0.284496736f);
+  poly = fmaf(t, poly, 0.254829592f);
+  poly *= t;
+  const float y = 1.0f - poly * __expf(-ax * ax);
+  return copysignf(y, x);
+}
+
 __device__ __forceinline__ float gelu_f(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+  return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f));
 }
 __device__ __forceinline__ float gelu_grad_f(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
@@ -149,6 +164,138 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
         out[o] = (__bf16)(v * gelu_grad_f(zprev[o]));
       }
     }
+  }
+}
+
+// ---- whole-network inference in one kernel -------------------------------------------------
+// A block owns an 8 x 32 output tile and evaluates all six layers on it, each on the tile
+// plus the halo the remaining layers need (layer l on a (6-l)-site halo): the state region
+// (halo 6) is staged in LDS as FP32, activations ping-pong between two LDS buffers (site
+// stride kSP), and nothing but the output leaves the block.  Every value is formed with the
+// per-layer kernels' arithmetic (same FMA order, same MFMA operand fragments and K order), so
+// the output is bit-identical to theirs; the halo layers cost about 3x the matrix work of the
+// tile, which at one image per launch is cheaper than five more launches.
+constexpr int kFR = kTR, kFC = kTC;                       // output tile
+constexpr int kFH0 = kFR + 12, kFW0 = kFC + 12;           // state region (halo 6)
+constexpr int kFNA = (kFR + 10) * (kFC + 10);             // layer-1 region (largest)
+constexpr int kFNB = (kFR + 8) * (kFC + 8);               // layer-2 region
+
+constexpr int kFNT = 512;  // threads of the fused kernel (8 waves)
+
+template <class TS>
+__global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
+                                                   const float* __restrict__ w1,
+                                                   const float* __restrict__ b1,
+                                                   const __bf16* __restrict__ wmid,
+                                                   const float* __restrict__ p,
+                                                   const float* __restrict__ w6,
+                                                   const float* __restrict__ b6,
+                                                   float* __restrict__ out,
+                                                   double* __restrict__ part, int H, int W,
+                                                   int tiles_x, int tiles_y) {
+  __shared__ float cs[kFH0 * kFW0];
+  __shared__ double psum[kFR * kFC / 64];
+  __shared__ float w1s[288 + 32], w6s[288];
+  __shared__ __attribute__((aligned(16))) __bf16 bufA[kFNA * kSP];
+  __shared__ __attribute__((aligned(16))) __bf16 bufB[kFNB * kSP];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tx = blockIdx.x % tiles_x;
+  const int r1 = blockIdx.x / tiles_x;
+  const int ty = r1 % tiles_y, b = r1 / tiles_y;
+  const int x0 = tx * kFC, y0 = ty * kFR;
+  const TS* img = c + (size_t)b * H * W;
+  for (int q = t; q < kFH0 * kFW0; q += kFNT) {
+    const int ly = q / kFW0, lx = q % kFW0;
+    cs[q] = (float)img[(size_t)wrap1(y0 + ly - 6, H) * W + wrap1(x0 + lx - 6, W)];
+  }
+  for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
+  for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
+  __syncthreads();
+  // layer 1 on the halo-5 region (kFR + 10) x (kFC + 10)
+  {
+    constexpr int RW = kFC + 10, RN = kFNA;
+    for (int q = t; q < RN; q += kFNT) {
+      const int ry = q / RW, rx = q % RW;
+      float xv[9];
+#pragma unroll
+      for (int tp = 0; tp < 9; ++tp) xv[tp] = cs[(ry + tp / 3) * kFW0 + rx + tp % 3];
+#pragma unroll 1
+      for (int co = 0; co < kC; co += 8) {
+        bf16x8 av;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int tp = 0; tp < 9; ++tp) acc = fmaf(w1s[(co + u) * 9 + tp], xv[tp], acc);
+          av[u] = (__bf16)gelu_f(acc + w1s[288 + co + u]);
+        }
+        *(bf16x8*)(bufA + q * kSP + co) = av;
+      }
+    }
+  }
+  __syncthreads();
+  // layers 2..5 on the matrix cores: output region halo h = 4..1, input one site wider
+  const int j = lane & 31, kh = (lane >> 5) * 8;
+#pragma unroll 1
+  for (int l = 0; l < 4; ++l) {
+    const int h = 4 - l;
+    const int Wo = kFC + 2 * h, Ho = kFR + 2 * h, Wi = Wo + 2;
+    const __bf16* in = (l & 1) ? bufB : bufA;
+    __bf16* ob = (l & 1) ? bufA : bufB;
+    bf16x8 wreg[18];
+    const __bf16* wt = wmid + (size_t)l * kMidW;
+#pragma unroll
+    for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
+    const float bj = p[kOffMid + l * kMidStride + kMidW + j];
+    // a wave takes the two 32-site halves of one output row (independent accumulators,
+    // so the two MFMA chains overlap)
+    for (int r = wave; r < Ho; r += kFNT / 64) {
+      f32x16 acc0 = {}, acc1 = {};
+      const int xs1 = Wo - 32;
+#pragma unroll
+      for (int kc = 0; kc < 18; ++kc) {
+        const int tap = kc >> 1, dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int row = (r + 1 + dy) * Wi + j + 1 + dx;
+        const bf16x8 a0 = *(const bf16x8*)(in + row * kSP + (kc & 1) * 16 + kh);
+        const bf16x8 a1 = *(const bf16x8*)(in + (row + xs1) * kSP + (kc & 1) * 16 + kh);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, wreg[kc], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, wreg[kc], acc1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int xi = (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+        ob[(r * Wo + xi) * kSP + j] = (__bf16)gelu_f(acc0[q] + bj);
+        if (xs1 + xi >= 32) ob[(r * Wo + xs1 + xi) * kSP + j] = (__bf16)gelu_f(acc1[q] + bj);
+      }
+    }
+    __syncthreads();
+  }
+  // layer 6 on the tile from the layer-5 region (halo 1, in bufA)
+  if (t < kFR * kFC) {
+    constexpr int Wi = kFC + 2;
+    const int ry = t / kFC, rx = t % kFC;
+    float acc = 0.0f;
+    for (int tp = 0; tp < 9; ++tp) {
+      const int site = (ry + tp / 3) * Wi + rx + tp % 3;
+#pragma unroll
+      for (int ch = 0; ch < kC; ch += 8) {
+        const bf16x8 v = *(const bf16x8*)(bufA + site * kSP + ch);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fmaf(w6s[tp * kC + ch + u], (float)v[u], acc);
+      }
+    }
+    const float o = acc + b6[0];
+    out[(size_t)b * H * W + (size_t)(y0 + ry) * W + x0 + rx] = o;
+    // the tile's sum of the output (for the mean removal), in a fixed order
+    double sd = (double)o;
+    for (int st = 32; st >= 1; st >>= 1) sd += __shfl_xor(sd, st);
+    if ((t & 63) == 0) psum[t >> 6] = sd;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double sd = psum[0];
+    for (int w = 1; w < kFR * kFC / 64; ++w) sd += psum[w];
+    part[blockIdx.x] = sd;
   }
 }
 
@@ -428,16 +575,6 @@ __global__ __launch_bounds__(256) void k_apply(const TS* __restrict__ c, TS* __r
   cn[s] = (TS)((double)c[s] + scale * ((double)out[s] - mean));
 }
 
-__global__ __launch_bounds__(256) void k_sum_f32(const float* __restrict__ x, long n,
-                                                 double* __restrict__ part) {
-  __shared__ double sh[4];
-  double s = 0.0;
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += x[i];
-  for (int st = 32; st >= 1; st >>= 1) s += __shfl_xor(s, st);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-}
 
 // One wave: lane l sums part[l], part[l + 64], ... in order, then a fixed xor tree.
 __global__ __launch_bounds__(64) void k_sum_final(const double* __restrict__ part, int n,
@@ -470,6 +607,7 @@ struct gc_cnn {
   size_t part_floats = 0;
   double* dpart = nullptr;
   int* idx = nullptr;
+  double* dmean = nullptr;  // per-image sum of the network output (mean removal)
   void* hist = nullptr;     // learned integration: proposals of one chunk
   double* diag = nullptr;   // learned integration: (mass, free energy) per proposal
   int adam_t = 0;
@@ -531,6 +669,18 @@ gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
   }
   hipLaunchKernelGGL(k_conv6, dim3(g1), dim3(256), 0, st, n->a[4], n->p + kOffW6, n->p + kOffB6, n->out,
                      n->H, n->W, B);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// Inference: the whole network in one kernel per 8 x 32 tile (k_fwd_fused); output to n->out.
+template <class TS>
+gc_status forward_fused(gc_cnn* n, const TS* c, int B) {
+  if (!n->packed) GC_TRY(pack(n));
+  const int tx = n->W / kTC, ty = n->H / kTR;
+  hipLaunchKernelGGL((k_fwd_fused<TS>), dim3((unsigned)(tx * ty * B)), dim3(kFNT), 0,
+                     n->ctx->stream, c, n->p + kOffW1, n->p + kOffB1, n->wmid, n->p,
+                     n->p + kOffW6, n->p + kOffB6, n->out, n->dpart, n->H, n->W, tx, ty);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -640,6 +790,7 @@ gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** o
   A((void**)&n->part, n->part_floats * sizeof(float));
   A((void**)&n->dpart, ((ns + 255) / 256 + 1024) * sizeof(double));
   A((void**)&n->idx, batch_max * sizeof(int));
+  A((void**)&n->dmean, batch_max * sizeof(double));
   if (st == GC_OK && (hipMemsetAsync(n->p, 0, np4, ctx->stream) != hipSuccess ||
                       hipMemsetAsync(n->m, 0, np4, ctx->stream) != hipSuccess ||
                       hipMemsetAsync(n->v, 0, np4, ctx->stream) != hipSuccess))
@@ -656,7 +807,7 @@ gc_status gc_cnn_destroy(gc_cnn* n) {
   if (!n) return GC_OK;
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->wmid, n->wmidT, n->dzA, n->dzB,
-                  n->out, n->dout, n->part, n->dpart, n->idx, n->hist, n->diag};
+                  n->out, n->dout, n->part, n->dpart, n->idx, n->dmean, n->hist, n->diag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 5; ++l) {
@@ -706,6 +857,18 @@ gc_status gc_cnn_forward(gc_cnn* n, gc_dtype dt, const void* c, int B, float* ou
   GC_TRY(check_cnn(n, B));
   GC_TRY(check_dtype(dt));
   if (!c) return set_error(GC_ERR_INVALID, "gridcalc: null input");
+  if (dt == GC_F64) GC_TRY(forward_fused<double>(n, (const double*)c, B));
+  else GC_TRY(forward_fused<float>(n, (const float*)c, B));
+  if (out_dev)
+    GC_HIP(hipMemcpyAsync(out_dev, n->out, (size_t)B * n->H * n->W * sizeof(float),
+                          hipMemcpyDeviceToDevice, n->ctx->stream));
+  return GC_OK;
+}
+
+gc_status gc_cnn_forward_layers(gc_cnn* n, gc_dtype dt, const void* c, int B, float* out_dev) {
+  GC_TRY(check_cnn(n, B));
+  GC_TRY(check_dtype(dt));
+  if (!c) return set_error(GC_ERR_INVALID, "gridcalc: null input");
   if (dt == GC_F64) GC_TRY(forward<double>(n, (const double*)c, nullptr, B, false));
   else GC_TRY(forward<float>(n, (const float*)c, nullptr, B, false));
   if (out_dev)
@@ -749,21 +912,20 @@ gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int
   if (!c || !c_new) return set_error(GC_ERR_INVALID, "gridcalc: null state");
   const long HW = (long)n->H * n->W;
   hipStream_t st = n->ctx->stream;
-  if (dt == GC_F64) GC_TRY(forward<double>(n, (const double*)c, nullptr, B, false));
-  else GC_TRY(forward<float>(n, (const float*)c, nullptr, B, false));
+  if (dt == GC_F64) GC_TRY(forward_fused<double>(n, (const double*)c, B));
+  else GC_TRY(forward_fused<float>(n, (const float*)c, B));
+  const int T = (n->W / kTC) * (n->H / kTR);  // tiles (partial sums) per image
   for (int b = 0; b < B; ++b) {
-    const unsigned nb = 256;
-    hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, st, n->out + b * HW, HW, n->dpart);
-    GC_LAUNCHED();
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart, (int)nb, n->dpart + nb);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart + (size_t)b * T, T,
+                       n->dmean + b);
     GC_LAUNCHED();
     const unsigned g1 = (unsigned)((HW + 255) / 256);
     if (dt == GC_F64)
       hipLaunchKernelGGL((k_apply<double>), dim3(g1), dim3(256), 0, st, (const double*)c + b * HW,
-                         (double*)c_new + b * HW, n->out + b * HW, n->dpart + nb, n->scale, HW);
+                         (double*)c_new + b * HW, n->out + b * HW, n->dmean + b, n->scale, HW);
     else
       hipLaunchKernelGGL((k_apply<float>), dim3(g1), dim3(256), 0, st, (const float*)c + b * HW,
-                         (float*)c_new + b * HW, n->out + b * HW, n->dpart + nb, n->scale, HW);
+                         (float*)c_new + b * HW, n->out + b * HW, n->dmean + b, n->scale, HW);
     GC_LAUNCHED();
   }
   return GC_OK;
@@ -864,6 +1026,63 @@ __global__ __launch_bounds__(256) void k_mass_fe(const T* __restrict__ c, int nx
 
 // One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
 // a fixed xor tree (sums with TwoSum, compensations added).
+// The residual update of k_apply fused with k_mass_fe: each thread forms the new state at its
+// site and at the four neighbours (the same expression as k_apply, so the same values),
+// writes its own site and accumulates the mass and the density of the new state.
+template <class T>
+__global__ __launch_bounds__(256) void k_apply_mass_fe(const T* __restrict__ c, T* __restrict__ cn,
+                                                       const float* __restrict__ out,
+                                                       const double* __restrict__ sum_out,
+                                                       double scale, int nx, int ny, double ihx,
+                                                       double ihy, double kappa,
+                                                       Comp2* __restrict__ part) {
+  __shared__ Comp2 sh[4];
+  const long n = (long)nx * ny;
+  const double mean = sum_out[0] / (double)n;
+  auto nv = [&](size_t q) { return (T)((double)c[q] + scale * ((double)out[q] - mean)); };
+  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+  for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256) {
+    const int i = (int)(s % nx), j = (int)(s / nx);
+    const T pT = nv((size_t)s);
+    cn[s] = pT;
+    const double p = (double)pT;
+    double dx = 0.0, dy = 0.0;
+    dx += -0.5 * (double)nv((size_t)j * nx + wrap1(i - 1, nx));
+    dy += -0.5 * (double)nv((size_t)wrap1(j - 1, ny) * nx + i);
+    dx += 0.0 * p;
+    dy += 0.0 * p;
+    dx += 0.5 * (double)nv((size_t)j * nx + wrap1(i + 1, nx));
+    dy += 0.5 * (double)nv((size_t)wrap1(j + 1, ny) * nx + i);
+    const double gx = dx * ihx, gy = dy * ihy, gz = 0.0;
+    const double sq = gx * gx + gy * gy + gz * gz;
+    neu(ms, mc, p);
+    neu(fs, fc, -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * sq);
+  }
+  Comp2 v{ms, mc, fs, fc};
+  for (int st = 1; st < 64; st <<= 1) {
+    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
+    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
+    double s1 = lo.ms, c1 = lo.mc + hi.mc;
+    neu(s1, c1, hi.ms);
+    double s2 = lo.fs, c2 = lo.fc + hi.fc;
+    neu(s2, c2, hi.fs);
+    v = Comp2{s1, c1, s2, c2};
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Comp2 r = sh[0];
+    for (int w = 1; w < 4; ++w) {
+      double c1 = r.mc + sh[w].mc, c2 = r.fc + sh[w].fc;
+      neu(r.ms, c1, sh[w].ms);
+      neu(r.fs, c2, sh[w].fs);
+      r.mc = c1;
+      r.fc = c2;
+    }
+    part[blockIdx.x] = r;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_mass_fe_final(const Comp2* __restrict__ part, int nb,
                                                       double* __restrict__ out) {
   double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
@@ -903,6 +1122,31 @@ gc_status mass_fe_async(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c,
   else
     hipLaunchKernelGGL((k_mass_fe<float>), dim3(kMFBlocks), dim3(256), 0, st, (const float*)c,
                        g->nx, g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_mass_fe_final, dim3(1), dim3(64), 0, st, part, kMFBlocks, out_dev);
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// One learned proposal with its diagnostics: c_new = c + scale*(out - mean(out)), and
+// (sum of c_new, sum of its density) into out_dev[0..1].
+gc_status predict_diag(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c, void* c_new,
+                       double kappa, double* out_dev) {
+  hipStream_t st = n->ctx->stream;
+  if (dt == GC_F64) GC_TRY(forward_fused<double>(n, (const double*)c, 1));
+  else GC_TRY(forward_fused<float>(n, (const float*)c, 1));
+  const int T = (n->W / kTC) * (n->H / kTR);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart, T, n->dmean);
+  GC_LAUNCHED();
+  Comp2* part = (Comp2*)n->dpart;
+  if (dt == GC_F64)
+    hipLaunchKernelGGL((k_apply_mass_fe<double>), dim3(kMFBlocks), dim3(256), 0, st,
+                       (const double*)c, (double*)c_new, n->out, n->dmean, n->scale, g->nx, g->ny,
+                       1.0 / g->hx, 1.0 / g->hy, kappa, part);
+  else
+    hipLaunchKernelGGL((k_apply_mass_fe<float>), dim3(kMFBlocks), dim3(256), 0, st,
+                       (const float*)c, (float*)c_new, n->out, n->dmean, n->scale, g->nx, g->ny,
+                       1.0 / g->hx, 1.0 / g->hy, kappa, part);
   GC_LAUNCHED();
   hipLaunchKernelGGL(k_mass_fe_final, dim3(1), dim3(64), 0, st, part, kMFBlocks, out_dev);
   GC_LAUNCHED();
@@ -955,8 +1199,7 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
     const void* src = psi;
     for (int k = 1; k <= K; ++k) {
       void* dst = hist + (size_t)k * bytes;
-      GC_TRY(gc_cnn_predict(n, dt, src, dst, 1));
-      GC_TRY(mass_fe_async(n, g, dt, dst, p->kappa, n->diag + 2 * k));
+      GC_TRY(predict_diag(n, g, dt, src, dst, p->kappa, n->diag + 2 * k));
       src = dst;
     }
     GC_TRY(fetch(n, h + 2, n->diag + 2, 2 * K));

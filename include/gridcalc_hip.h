@@ -266,8 +266,11 @@ gc_status gc_cnn_get_params(gc_cnn* n, float* host);
 /* Gradient of the last gc_cnn_loss_grad call (host FP32, param-count values). */
 gc_status gc_cnn_get_grads(gc_cnn* n, float* host);
 /* Network output `out` (before scale and mean removal) for B device images of dtype dt;
- * out_dev (device FP32, B*H*W) may be NULL. */
+ * out_dev (device FP32, B*H*W) may be NULL.  One kernel runs all six layers per 8 x 32
+ * tile (inference); gc_cnn_forward_layers runs the per-layer kernels of the training path.
+ * Both give the same bits. */
 gc_status gc_cnn_forward(gc_cnn* n, gc_dtype dt, const void* c, int B, float* out_dev);
+gc_status gc_cnn_forward_layers(gc_cnn* n, gc_dtype dt, const void* c, int B, float* out_dev);
 /* One training evaluation: images c[idx[b]] and targets target[idx[b]] (device arrays of
  * dtype dt; idx is a host array of B indices, or NULL for 0..B-1).  Loss is the mean of
  * (out - (target - c)/scale)^2; its gradient goes to the parameter gradient buffer.  If

@@ -129,3 +129,20 @@ def test_training_reduces_loss(gc, cnn, ctx):
     l1 = net.loss_grad(d_c, d_t, B=B)
     assert l1 < 0.1 * l0, (l0, l1)
     net.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(2, 16, 64), (1, 256, 256)])
+def test_fused_inference_equals_layer_kernels(gc, cnn, ctx, dtype, shape):
+    """The one-kernel inference path (all six layers per 8 x 32 tile, halos recomputed) gives
+    the per-layer kernels' output bit for bit."""
+    B, H, W = shape
+    p = _params(cnn, 13)
+    c = np.random.default_rng(4).uniform(-1, 1, (B, H, W))
+    net = gc.Corrector(ctx, H, W, B)
+    net.set_params(p, 0.1)
+    d_c = gc.DeviceArray.from_numpy(ctx, c.astype(dtype))
+    fused = net.forward(d_c, B)
+    layers = net.forward(d_c, B, layers=True)
+    assert np.array_equal(fused, layers)
+    net.close()
