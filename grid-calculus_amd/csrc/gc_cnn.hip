@@ -562,28 +562,36 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf1
   }
 }
 
+// Sum of the per-tile partial sums part[0..nparts): wave 0 of the block, lane l summing
+// part[l], part[l + 64], ... in order, then a fixed xor tree (every block gets the same bits);
+// the result is stored to *dst (LDS) before the barrier.
+__device__ __forceinline__ void block_sum_parts(const double* __restrict__ part, int nparts,
+                                                double* dst) {
+  if (threadIdx.x < 64) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 64) v += part[i];
+    for (int st = 32; st >= 1; st >>= 1) v += __shfl_xor(v, st);
+    if (threadIdx.x == 0) *dst = v;
+  }
+  __syncthreads();
+}
+
 // Residual update with the mean of the increment removed (mass conservation):
 // c_new = c + scale * (out - sum(out)/N).
 template <class TS>
 __global__ __launch_bounds__(256) void k_apply(const TS* __restrict__ c, TS* __restrict__ cn,
                                                const float* __restrict__ out,
-                                               const double* __restrict__ sum_out, double scale,
-                                               long n) {
+                                               const double* __restrict__ part, int nparts,
+                                               double scale, long n) {
+  __shared__ double sum;
+  block_sum_parts(part, nparts, &sum);
   const long s = (long)blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
-  const double mean = sum_out[0] / (double)n;
+  const double mean = sum / (double)n;
   cn[s] = (TS)((double)c[s] + scale * ((double)out[s] - mean));
 }
 
 
-// One wave: lane l sums part[l], part[l + 64], ... in order, then a fixed xor tree.
-__global__ __launch_bounds__(64) void k_sum_final(const double* __restrict__ part, int n,
-                                                  double* __restrict__ out) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 64) s += part[i];
-  for (int st = 32; st >= 1; st >>= 1) s += __shfl_xor(s, st);
-  if (threadIdx.x == 0) out[0] = s;
-}
 
 template <class TS, class TD>
 __global__ __launch_bounds__(256) void k_cast(const TS* __restrict__ a, TD* __restrict__ b, long n) {
@@ -607,7 +615,7 @@ struct gc_cnn {
   size_t part_floats = 0;
   double* dpart = nullptr;
   int* idx = nullptr;
-  double* dmean = nullptr;  // per-image sum of the network output (mean removal)
+  unsigned* done = nullptr;  // learned integration: blocks finished (last-block reduction)
   void* hist = nullptr;     // learned integration: proposals of one chunk
   double* diag = nullptr;   // learned integration: (mass, free energy) per proposal
   int adam_t = 0;
@@ -790,8 +798,9 @@ gc_status gc_cnn_create(gc_context* ctx, int H, int W, int batch_max, gc_cnn** o
   A((void**)&n->part, n->part_floats * sizeof(float));
   A((void**)&n->dpart, ((ns + 255) / 256 + 1024) * sizeof(double));
   A((void**)&n->idx, batch_max * sizeof(int));
-  A((void**)&n->dmean, batch_max * sizeof(double));
+  A((void**)&n->done, sizeof(unsigned));
   if (st == GC_OK && (hipMemsetAsync(n->p, 0, np4, ctx->stream) != hipSuccess ||
+                      hipMemsetAsync(n->done, 0, sizeof(unsigned), ctx->stream) != hipSuccess ||
                       hipMemsetAsync(n->m, 0, np4, ctx->stream) != hipSuccess ||
                       hipMemsetAsync(n->v, 0, np4, ctx->stream) != hipSuccess))
     st = set_error(GC_ERR_DEVICE, "gridcalc: CNN initialisation failed");
@@ -807,7 +816,7 @@ gc_status gc_cnn_destroy(gc_cnn* n) {
   if (!n) return GC_OK;
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->wmid, n->wmidT, n->dzA, n->dzB,
-                  n->out, n->dout, n->part, n->dpart, n->idx, n->dmean, n->hist, n->diag};
+                  n->out, n->dout, n->part, n->dpart, n->idx, n->done, n->hist, n->diag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 5; ++l) {
@@ -916,16 +925,15 @@ gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int
   else GC_TRY(forward_fused<float>(n, (const float*)c, B));
   const int T = (n->W / kTC) * (n->H / kTR);  // tiles (partial sums) per image
   for (int b = 0; b < B; ++b) {
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart + (size_t)b * T, T,
-                       n->dmean + b);
-    GC_LAUNCHED();
     const unsigned g1 = (unsigned)((HW + 255) / 256);
     if (dt == GC_F64)
       hipLaunchKernelGGL((k_apply<double>), dim3(g1), dim3(256), 0, st, (const double*)c + b * HW,
-                         (double*)c_new + b * HW, n->out + b * HW, n->dmean + b, n->scale, HW);
+                         (double*)c_new + b * HW, n->out + b * HW, n->dpart + (size_t)b * T, T,
+                         n->scale, HW);
     else
       hipLaunchKernelGGL((k_apply<float>), dim3(g1), dim3(256), 0, st, (const float*)c + b * HW,
-                         (float*)c_new + b * HW, n->out + b * HW, n->dmean + b, n->scale, HW);
+                         (float*)c_new + b * HW, n->out + b * HW, n->dpart + (size_t)b * T, T,
+                         n->scale, HW);
     GC_LAUNCHED();
   }
   return GC_OK;
@@ -1026,19 +1034,51 @@ __global__ __launch_bounds__(256) void k_mass_fe(const T* __restrict__ c, int nx
 
 // One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
 // a fixed xor tree (sums with TwoSum, compensations added).
+// One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
+// a fixed xor tree (sums with TwoSum, compensations added); lane 0 writes the two results.
+__device__ __forceinline__ void mass_fe_reduce(const Comp2* __restrict__ part, int nb,
+                                               double* __restrict__ out) {
+  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    mc += part[b].mc;
+    fc += part[b].fc;
+    neu(ms, mc, part[b].ms);
+    neu(fs, fc, part[b].fs);
+  }
+  Comp2 v{ms, mc, fs, fc};
+  for (int st = 1; st < 64; st <<= 1) {
+    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
+    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
+    double s1 = lo.ms, c1 = lo.mc + hi.mc;
+    neu(s1, c1, hi.ms);
+    double s2 = lo.fs, c2 = lo.fc + hi.fc;
+    neu(s2, c2, hi.fs);
+    v = Comp2{s1, c1, s2, c2};
+  }
+  if (threadIdx.x == 0) {
+    out[0] = v.ms + v.mc;
+    out[1] = v.fs + v.fc;
+  }
+}
+
 // The residual update of k_apply fused with k_mass_fe: each thread forms the new state at its
 // site and at the four neighbours (the same expression as k_apply, so the same values),
 // writes its own site and accumulates the mass and the density of the new state.
 template <class T>
 __global__ __launch_bounds__(256) void k_apply_mass_fe(const T* __restrict__ c, T* __restrict__ cn,
                                                        const float* __restrict__ out,
-                                                       const double* __restrict__ sum_out,
-                                                       double scale, int nx, int ny, double ihx,
-                                                       double ihy, double kappa,
-                                                       Comp2* __restrict__ part) {
+                                                       const double* __restrict__ osum,
+                                                       int nparts, double scale, int nx, int ny,
+                                                       double ihx, double ihy, double kappa,
+                                                       Comp2* __restrict__ part,
+                                                       unsigned* __restrict__ done,
+                                                       double* __restrict__ result) {
   __shared__ Comp2 sh[4];
+  __shared__ double sum;
+  __shared__ bool last;
+  block_sum_parts(osum, nparts, &sum);
   const long n = (long)nx * ny;
-  const double mean = sum_out[0] / (double)n;
+  const double mean = sum / (double)n;
   auto nv = [&](size_t q) { return (T)((double)c[q] + scale * ((double)out[q] - mean)); };
   double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
   for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256) {
@@ -1080,32 +1120,20 @@ __global__ __launch_bounds__(256) void k_apply_mass_fe(const T* __restrict__ c, 
       r.fc = c2;
     }
     part[blockIdx.x] = r;
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) {  // every partial is written: reduce them here instead of in another launch
+    __threadfence();
+    if (threadIdx.x < 64) mass_fe_reduce(part, (int)gridDim.x, result);
+    if (threadIdx.x == 0) *done = 0u;  // ready for the next launch
   }
 }
 
 __global__ __launch_bounds__(64) void k_mass_fe_final(const Comp2* __restrict__ part, int nb,
                                                       double* __restrict__ out) {
-  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 64) {
-    mc += part[b].mc;
-    fc += part[b].fc;
-    neu(ms, mc, part[b].ms);
-    neu(fs, fc, part[b].fs);
-  }
-  Comp2 v{ms, mc, fs, fc};
-  for (int st = 1; st < 64; st <<= 1) {
-    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
-    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
-    double s1 = lo.ms, c1 = lo.mc + hi.mc;
-    neu(s1, c1, hi.ms);
-    double s2 = lo.fs, c2 = lo.fc + hi.fc;
-    neu(s2, c2, hi.fs);
-    v = Comp2{s1, c1, s2, c2};
-  }
-  if (threadIdx.x == 0) {
-    out[0] = v.ms + v.mc;
-    out[1] = v.fs + v.fc;
-  }
+  mass_fe_reduce(part, nb, out);
 }
 
 constexpr int kMFBlocks = 128;
@@ -1136,19 +1164,15 @@ gc_status predict_diag(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c, 
   if (dt == GC_F64) GC_TRY(forward_fused<double>(n, (const double*)c, 1));
   else GC_TRY(forward_fused<float>(n, (const float*)c, 1));
   const int T = (n->W / kTC) * (n->H / kTR);
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, st, n->dpart, T, n->dmean);
-  GC_LAUNCHED();
-  Comp2* part = (Comp2*)n->dpart;
+  Comp2* part = (Comp2*)(n->dpart + T);  // after the tile sums the kernel reads
   if (dt == GC_F64)
     hipLaunchKernelGGL((k_apply_mass_fe<double>), dim3(kMFBlocks), dim3(256), 0, st,
-                       (const double*)c, (double*)c_new, n->out, n->dmean, n->scale, g->nx, g->ny,
-                       1.0 / g->hx, 1.0 / g->hy, kappa, part);
+                       (const double*)c, (double*)c_new, n->out, n->dpart, T, n->scale, g->nx,
+                       g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part, n->done, out_dev);
   else
     hipLaunchKernelGGL((k_apply_mass_fe<float>), dim3(kMFBlocks), dim3(256), 0, st,
-                       (const float*)c, (float*)c_new, n->out, n->dmean, n->scale, g->nx, g->ny,
-                       1.0 / g->hx, 1.0 / g->hy, kappa, part);
-  GC_LAUNCHED();
-  hipLaunchKernelGGL(k_mass_fe_final, dim3(1), dim3(64), 0, st, part, kMFBlocks, out_dev);
+                       (const float*)c, (float*)c_new, n->out, n->dpart, T, n->scale, g->nx,
+                       g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part, n->done, out_dev);
   GC_LAUNCHED();
   return GC_OK;
 }
