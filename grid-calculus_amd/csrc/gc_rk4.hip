@@ -1047,6 +1047,31 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     ooff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
   }
 
+  // LDS offsets of this thread's sites in every pass (the same on every plane)
+  int oA[C::L3];  // mu_A: centre in the y plane (tile+4 layout); -1: no site
+#pragma unroll
+  for (int s = 0; s < C::L3; ++s) {
+    const int idx = t + s * NT;
+    oA[s] = idx < C::N3 ? (idx / C::W3 + 1) * C::W4 + idx % C::W3 + 1 : -1;
+  }
+  // k_A ring: centre in the mu_A plane, index in the y' plane, base in the y plane
+  int rk[C::LR], re[C::LR], rb[C::LR];
+#pragma unroll
+  for (int s = 0; s < C::LR; ++s) {
+    const int rr = t + s * NT;
+    int mx = 0, my = 0;
+    if (rr < C::NR) ring_site<C, OY>(rr, mx, my);
+    rk[s] = rr < C::NR ? (my + 1) * C::W3 + (mx + 1) : -1;
+    re[s] = my * C::W2 + mx;
+    rb[s] = (my + 2) * C::W4 + (mx + 2);
+  }
+  int oB[C::L1];  // mu_B: centre in the y' plane; -1: no site
+#pragma unroll
+  for (int s = 0; s < C::L1; ++s) {
+    const int idx = t + s * NT;
+    oB[s] = idx < C::N1 ? (idx / C::W1 + 1) * C::W2 + idx % C::W1 + 1 : -1;
+  }
+
   T yreg[C::L4];
   auto load_y = [&](int plane) {
     const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
@@ -1095,13 +1120,11 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     if (p >= 2) {  // mu_A on plane z0-5+p, tile+3
 #pragma unroll
       for (int s = 0; s < C::L3; ++s) {
-        const int idx = t + s * NT;
-        if (idx < C::N3) {
-          const int mx = idx % C::W3, my = idx / C::W3;
-          const int o = (my + 1) * C::W4 + (mx + 1);
+        const int o = oA[s];
+        if (o >= 0) {
           const T l = lap_lds(ys[s1], ys[s2], ys[s0], o, C::W4, cf.ihx2, cf.ihy2, cf.ihz2);
           const T yy = ys[s1][o];
-          mas[s0][idx] = -yy + yy * yy * yy - cf.kappa * l;
+          mas[s0][t + s * NT] = -yy + yy * yy * yy - cf.kappa * l;
         }
       }
     }
@@ -1133,13 +1156,11 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       // the 2-site ring around the tile
 #pragma unroll
       for (int s = 0; s < C::LR; ++s) {
-        const int r = t + s * NT;
-        if (r < C::NR) {
-          int mx, my;
-          ring_site<C, OY>(r, mx, my);
-          const T kv = kA(mx, my);
-          const int e = my * C::W2 + mx;
-          if constexpr (PAIR == 0) y2s[s0][e] = ys[s2][(my + 2) * C::W4 + (mx + 2)] + co.half_dt * kv;
+        const int o = rk[s];
+        if (o >= 0) {
+          const T kv = lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const int e = re[s];
+          if constexpr (PAIR == 0) y2s[s0][e] = ys[s2][rb[s]] + co.half_dt * kv;
           else y2s[s0][e] = pe[s] + co.dt * kv;
         }
       }
@@ -1148,13 +1169,11 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
     if (p >= 6) {  // mu_B on plane z0-7+p, tile+1
 #pragma unroll
       for (int s = 0; s < C::L1; ++s) {
-        const int idx = t + s * NT;
-        if (idx < C::N1) {
-          const int mx = idx % C::W1, my = idx / C::W1;
-          const int o = (my + 1) * C::W2 + (mx + 1);
+        const int o = oB[s];
+        if (o >= 0) {
           const T l = lap_lds(y2s[s1], y2s[s2], y2s[s0], o, C::W2, cf.ihx2, cf.ihy2, cf.ihz2);
           const T yy = y2s[s1][o];
-          mbs[s0][idx] = -yy + yy * yy * yy - cf.kappa * l;
+          mbs[s0][t + s * NT] = -yy + yy * yy * yy - cf.kappa * l;
         }
       }
     }
