@@ -353,6 +353,11 @@ def ensemble(a):
             "sites": sites, "members": len(members), "tile_ox_oy": tile,
             "tune_ms_per_step": tune_ms,
             "state_GBs": 2 * 4 * sites / (ms / steps / 1e3) / 1e9,
+            # the C4 algorithmic figure: 16 words (64 B in FP32) per site and RK4 step
+            "roofline": {"bound": "hbm", "achieved": WORDS_PER_SITE_STEP * 4 * sites / (ms / steps / 1e3) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": WORDS_PER_SITE_STEP * 4 * sites / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "note": "the kernel moves 2 words per site per step (temporal blocking)"},
             "max_rel_mass_drift": max(abs(m1 - m0) / max(abs(m0), 1e-300) for m0, m1 in zip(mass0, mass)),
             "F_increases": sum(1 for f0, f1 in zip(F0, F) if f1 > f0)}
 

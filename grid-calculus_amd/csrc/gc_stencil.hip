@@ -242,6 +242,136 @@ void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, co
                      0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen);
 }
 
+// ---- biharmonic: ring + register queues --------------------------------------------------
+//
+// The biharmonic reads 21 axis values and 75 mixed values per site.  On the ring engine every
+// one is an LDS read; here the values along z -- the centre column over z-R..z+R and the
+// columns (x+-a, y), (x, y+-a) over z-R2..z+R2 that the xz and yz mixed terms use -- stay in
+// per-thread register queues (one column per thread), so a plane costs the in-plane window
+// plus one new value per queue.  Same ring fill, same summation order as OpBih.
+template <class T, int O>
+__global__ __launch_bounds__(256) void k_bih_zq(OpBih<T, O> op, const T* __restrict__ f,
+                                                T* __restrict__ out, Dims d, int ntx, int nty,
+                                                int zlen) {
+  using S2 = D2<O>;
+  using S4 = D4<O>;
+  constexpr int R = S4::R, R2 = S2::R, NQ = 2 * R2 + 1;
+  using Z = ZTile<R, 1, 64, 4>;
+  static_assert(Z::OR == 1, "one column per thread");
+  __shared__ T ring[Z::NP * Z::PN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * 64, y0 = ty * 4, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+
+  unsigned poff[Z::PL];
+  bool pin[Z::PL];
+#pragma unroll
+  for (int s = 0; s < Z::PL; ++s) {
+    const int idx = t + s * Z::NT;
+    pin[s] = idx < Z::PN;
+    const int px = idx % Z::PW, py = idx / Z::PW;
+    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
+                     : 0u;
+  }
+  T reg[Z::PL];
+  auto load = [&](int plane) {
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < Z::PL; ++s)
+      if (pin[s]) reg[s] = f[zo + poff[s]];
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int s = 0; s < Z::PL; ++s)
+      if (pin[s]) ring[slot * Z::PN + t + s * Z::NT] = reg[s];
+  };
+  for (int i = 0; i < 2 * R; ++i) {
+    load(z0 - R + i);
+    store(i);
+  }
+  load(z0 + R);
+  const int ox = t % 64, oy = t / 64;
+  const int base = (oy + R) * Z::PW + (ox + R);
+  const bool valid = x0 + ox < d.nx && y0 + oy < d.ny;
+  T qc[2 * R + 1];      // centre column, planes z-R..z+R
+  T qx[2 * R2][NQ];     // columns x-R2..x+R2 (without x), planes z-R2..z+R2
+  T qy[2 * R2][NQ];     // columns y-R2..y+R2 (without y)
+  auto side = [](int a) { return a < 0 ? a + R2 : a + R2 - 1; };  // offset -> queue index
+  for (int k = 0; k < nout; ++k) {
+    store((k + 2 * R) % Z::NP);
+    __syncthreads();
+    if (k + 1 < nout) load(z0 + k + 1 + R);
+    int slot[Z::NP];
+#pragma unroll
+    for (int j = 0; j < Z::NP; ++j) slot[j] = (k + j) % Z::NP;  // plane z-R+j
+    auto F = [&](int dx, int dy, int dz) {
+      return ring[slot[dz + R] * Z::PN + base + dy * Z::PW + dx];
+    };
+    if (k == 0) {  // fill the queues
+#pragma unroll
+      for (int m = 0; m < 2 * R + 1; ++m) qc[m] = F(0, 0, m - R);
+#pragma unroll
+      for (int a = -R2; a <= R2; ++a) {
+        if (a == 0) continue;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+          qx[side(a)][m] = F(a, 0, m - R2);
+          qy[side(a)][m] = F(0, a, m - R2);
+        }
+      }
+    } else {  // shift by one plane, read the newest plane
+#pragma unroll
+      for (int m = 0; m < 2 * R; ++m) qc[m] = qc[m + 1];
+      qc[2 * R] = F(0, 0, R);
+#pragma unroll
+      for (int a = -R2; a <= R2; ++a) {
+        if (a == 0) continue;
+#pragma unroll
+        for (int m = 0; m < NQ - 1; ++m) {
+          qx[side(a)][m] = qx[side(a)][m + 1];
+          qy[side(a)][m] = qy[side(a)][m + 1];
+        }
+        qx[side(a)][NQ - 1] = F(a, 0, R2);
+        qy[side(a)][NQ - 1] = F(0, a, R2);
+      }
+    }
+    if (valid) {
+      T dx4 = 0, dy4 = 0, dz4 = 0;
+#pragma unroll
+      for (int m = 0; m < S4::N; ++m) {
+        const int o = m - S4::R;
+        const T w = T(S4::w(m));
+        dx4 += w * F(o, 0, 0);
+        dy4 += w * F(0, o, 0);
+        dz4 += w * qc[R + o];
+      }
+      T dxy = 0, dxz = 0, dyz = 0;
+#pragma unroll
+      for (int mb = 0; mb < S2::N; ++mb) {
+        const int ob = mb - S2::R;
+        const double wb = S2::w(mb);
+#pragma unroll
+        for (int ma = 0; ma < S2::N; ++ma) {
+          const int oa = ma - S2::R;
+          const T wab = T(S2::w(ma) * wb);
+          dxy += wab * F(oa, ob, 0);
+          dxz += wab * (oa == 0 ? qc[R + ob] : qx[side(oa)][R2 + ob]);
+          dyz += wab * (oa == 0 ? qc[R + ob] : qy[side(oa)][R2 + ob]);
+        }
+      }
+      out[(size_t)d.plane * (size_t)(z0 + k) + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] =
+          dx4 * op.ihx4 + dy4 * op.ihy4 + dz4 * op.ihz4 +
+          T(2) * (dxy * op.ihx2 * op.ihy2 + dxz * op.ihx2 * op.ihz2 + dyz * op.ihy2 * op.ihz2);
+    }
+    __syncthreads();
+  }
+}
+
 // ---- vector- and tensor-field operators: z-queue engine -----------------------------------
 //
 // The inputs are AoS (CIN values per site).  Each derivative direction reads only some of the
@@ -583,10 +713,10 @@ void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
   const Dims d = dims_of(g);
   const Inv<double> i2 = inv_h2(g);
   // inv_h4 = inv_h2 * inv_h2 in double (Biharmonic.hpp:62-64), then cast
-  zmarch_launch<T>(ctx, d,
-                   OpBih<T, O>{T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x), T(i2.y * i2.y),
-                               T(i2.z * i2.z)},
-                   (const T*)in, nullptr, (T*)out);
+  const OpBih<T, O> op{T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x), T(i2.y * i2.y), T(i2.z * i2.z)};
+  const int ntx = (d.nx + 63) / 64, nty = (d.ny + 3) / 4, nzc = (d.nz + kZLen - 1) / kZLen;
+  hipLaunchKernelGGL((k_bih_zq<T, O>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
+                     ctx->stream, op, (const T*)in, (T*)out, d, ntx, nty, kZLen);
 }
 
 }  // namespace
