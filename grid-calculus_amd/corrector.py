@@ -27,7 +27,7 @@ from . import (F32, F64, RK4, Context, Corrector, DeviceArray, Ensemble, Grid, N
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000, n_steps=2000,
-              batch_traj=100, minibatch=16, iters_per_batch=5000, lr_max=2e-3, lr_min=2e-5,
+              batch_traj=100, minibatch=16, iters_per_batch=9000, lr_max=2e-3, lr_min=2e-5,
               seed=12345)
 
 

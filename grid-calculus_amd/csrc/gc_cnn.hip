@@ -9,11 +9,11 @@
 // The 32 -> 32 layers take BF16 weights and activations; the 1 -> 32 and 32 -> 1 layers
 // take the state and their weights in FP32; every accumulation is FP32.  The four 32 -> 32 layers
 // carry 98 % of the FLOPs and run as implicit GEMMs on the matrix cores
-// (v_mfma_f32_32x32x16_bf16): M = 32 sites of a row, N = 32 output channels, K = 9 taps x
-// 32 input channels (18 MFMAs per 32 x 32 output tile).  Backward-data is the same
-// convolution with the taps flipped and the weight matrix transposed; backward-weights
-// is a GEMM over sites whose operands are staged channel-major in LDS.  The 1 -> 32 and
-// 32 -> 1 layers (288 MACs per site) run on the vector ALUs.
+// (v_mfma_f32_32x32x16_bf16): M = 32 output channels, N = 32 sites, K = 9 taps x 32 input
+// channels (18 MFMAs per 32 x 32 output tile; a lane ends with 16 channels of one site).
+// Backward-data is the same convolution with the taps flipped and the weight matrix
+// transposed; backward-weights is a GEMM over sites whose operands are staged channel-major
+// in LDS.  The 1 -> 32 and 32 -> 1 layers (288 MACs per site) run on the vector ALUs.
 //
 // Parameters are one FP32 vector (37601 values): W1[co][tap], b1, {W_l[co][tap][ci], b_l}
 // for l = 2..5, W6[tap][ci], b6.  Activations are NHWC (32 channels contiguous per site).
@@ -35,6 +35,7 @@ namespace {
 using namespace gc;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kC = 32;            // channels
@@ -46,23 +47,21 @@ static_assert(kNParams == 37601, "parameter count");
 
 constexpr int kTR = 8, kTC = 32, kSP = 40;  // conv tile rows, cols, LDS site stride (bf16)
 
-// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
-// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x) for
-// x >= 0 (|error| <= 1.5e-7), odd extension for x < 0.
-__device__ __forceinline__ float erf_as(float x) {
-  const float ax = fabsf(x);
-  const float t = 1.0f / fmaf(0.3275911f, ax, 1.0f);
-  float poly = fmaf(t, 1.061405429f, -1.453152027f);
-  poly = fmaf(t, poly, 1.421413741f);
-  poly = fmaf(t, poly, -0.284496736f);
-  poly = fmaf(t, poly, 0.254829592f);
-  poly *= t;
-  const float y = 1.0f - poly * __expf(-ax * ax);
-  return copysignf(y, x);
-}
-
+// GELU(z) = z Phi(z) = 0.5 z (1 + erf(z / sqrt 2)) with erf by Abramowitz & Stegun, Handbook of
+// Mathematical Functions, 7.1.26: erf(x) = 1 - P(t) exp(-x^2), t = 1 / (1 + p x), x >= 0,
+// P(t) = a1 t + ... + a5 t^5, |error| <= 1.5e-7.  With x = |z| / sqrt 2 and
+// h = 0.5 z P(t) exp(-z^2 / 2), GELU(z) = z - h for z >= 0 and h for z < 0; the 1 / sqrt 2 is
+// folded into p, the 0.5 into the a_i, and log2(e) into the exponent (hardware exp2).
 __device__ __forceinline__ float gelu_f(float z) {
-  return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f));
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(z), 1.0f));
+  float P = fmaf(t, 0.5f * 1.061405429f, 0.5f * -1.453152027f);
+  P = fmaf(t, P, 0.5f * 1.421413741f);
+  P = fmaf(t, P, 0.5f * -0.284496736f);
+  P = fmaf(t, P, 0.5f * 0.254829592f);
+  P *= t;
+  const float e = __builtin_amdgcn_exp2f(z * (-0.5f * 1.44269504088896341f) * z);
+  const float h = z * (P * e);
+  return z >= 0.0f ? z - h : h;
 }
 __device__ __forceinline__ float gelu_grad_f(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
@@ -148,21 +147,31 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
       const int tap = kc >> 1, dy = tap / 3 - 1, dx = tap % 3 - 1;
       const int site = (r + 1 + dy) * (kTC + 2) + (j + 1 + dx);
       const bf16x8 a = *(const bf16x8*)(xs + site * kSP + (kc & 1) * 16 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wreg[kc], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a, acc, 0, 0, 0);
     }
-    const size_t row = img + (size_t)(y0 + r) * W + x0;
+    // lane: site x0 + j of the row, channels 8 g + 4 (lane / 32) + i in acc[4 g + i]
+    const size_t o0 = (img + (size_t)(y0 + r) * W + x0 + j) * kC + (lane >> 5) * 4;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int xi = (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-      const size_t o = (row + xi) * kC + j;
-      float v = acc[q];
+    for (int g = 0; g < 4; ++g) {
+      const size_t o = o0 + g * 8;
+      float v[4];
+      bf16x4 ov;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[4 * g + i];
       if (MODE == 0) {
-        v += bias[j];
-        if (zout) zout[o] = v;
-        out[o] = (__bf16)gelu_f(v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += bias[g * 8 + (lane >> 5) * 4 + i];
+        if (zout) *(float4*)(zout + o) = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(v[i]);
       } else {
-        out[o] = (__bf16)(v * gelu_grad_f(zprev[o]));
+        const float4 zp = *(const float4*)(zprev + o);
+        ov[0] = (__bf16)(v[0] * gelu_grad_f(zp.x));
+        ov[1] = (__bf16)(v[1] * gelu_grad_f(zp.y));
+        ov[2] = (__bf16)(v[2] * gelu_grad_f(zp.z));
+        ov[3] = (__bf16)(v[3] * gelu_grad_f(zp.w));
       }
+      *(bf16x4*)(out + o) = ov;
     }
   }
 }
@@ -173,8 +182,9 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
 // (halo 6) is staged in LDS as FP32, activations ping-pong between two LDS buffers (site
 // stride kSP), and nothing but the output leaves the block.  Every value is formed with the
 // per-layer kernels' arithmetic (same FMA order, same MFMA operand fragments and K order), so
-// the output is bit-identical to theirs; the halo layers cost about 3x the matrix work of the
-// tile, which at one image per launch is cheaper than five more launches.
+// the output is bit-identical to theirs.  The halos cost 1.9x the tile's matrix work in
+// layers 2..5 (1984 sites in 32-site chunks for 1024) and 3x its layer-1 work, which at one
+// image per launch is cheaper than five more launches.
 constexpr int kFR = kTR, kFC = kTC;                       // output tile
 constexpr int kFH0 = kFR + 12, kFW0 = kFC + 12;           // state region (halo 6)
 constexpr int kFNA = (kFR + 10) * (kFC + 10);             // layer-1 region (largest)
@@ -234,39 +244,53 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
     }
   }
   __syncthreads();
-  // layers 2..5 on the matrix cores: output region halo h = 4..1, input one site wider
-  const int j = lane & 31, kh = (lane >> 5) * 8;
+  // layers 2..5 on the matrix cores: output region halo h = 4..1, input one site wider.  The
+  // output region's sites, row-major, are cut into chunks of 32; a wave takes chunks c and
+  // c + 8 (independent accumulators, so the two MFMA chains overlap), c = wave, wave + 16, ...
+  const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
 #pragma unroll 1
   for (int l = 0; l < 4; ++l) {
     const int h = 4 - l;
-    const int Wo = kFC + 2 * h, Ho = kFR + 2 * h, Wi = Wo + 2;
+    const int Wo = kFC + 2 * h, Ho = kFR + 2 * h, Wi = Wo + 2, NS = Wo * Ho;
+    const int nch = (NS + 31) / 32;
     const __bf16* in = (l & 1) ? bufB : bufA;
     __bf16* ob = (l & 1) ? bufA : bufB;
     bf16x8 wreg[18];
     const __bf16* wt = wmid + (size_t)l * kMidW;
 #pragma unroll
     for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
-    const float bj = p[kOffMid + l * kMidStride + kMidW + j];
-    // a wave takes the two 32-site halves of one output row (independent accumulators,
-    // so the two MFMA chains overlap)
-    for (int r = wave; r < Ho; r += kFNT / 64) {
+    float bj[16];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) bj[4 * g + i] = p[kOffMid + l * kMidStride + kMidW + g * 8 + cq + i];
+    auto store = [&](const f32x16& acc, int s) {
+      if (s < NS) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          bf16x4 ov;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
+          *(bf16x4*)(ob + s * kSP + g * 8 + cq) = ov;
+        }
+      }
+    };
+    for (int c = wave; c < nch; c += 16) {
+      const int c1 = c + 8;
+      const int s0 = min(c * 32 + j, NS - 1), s1 = min(c1 * 32 + j, NS - 1);
+      const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1, i1 = (s1 / Wo + 1) * Wi + s1 % Wo + 1;
       f32x16 acc0 = {}, acc1 = {};
-      const int xs1 = Wo - 32;
 #pragma unroll
       for (int kc = 0; kc < 18; ++kc) {
-        const int tap = kc >> 1, dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const int row = (r + 1 + dy) * Wi + j + 1 + dx;
-        const bf16x8 a0 = *(const bf16x8*)(in + row * kSP + (kc & 1) * 16 + kh);
-        const bf16x8 a1 = *(const bf16x8*)(in + (row + xs1) * kSP + (kc & 1) * 16 + kh);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, wreg[kc], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, wreg[kc], acc1, 0, 0, 0);
+        const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
+        const int ko = (kc & 1) * 16 + kh;
+        const bf16x8 a0 = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
+        const bf16x8 a1 = *(const bf16x8*)(in + (i1 + off) * kSP + ko);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a1, acc1, 0, 0, 0);
       }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int xi = (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-        ob[(r * Wo + xi) * kSP + j] = (__bf16)gelu_f(acc0[q] + bj);
-        if (xs1 + xi >= 32) ob[(r * Wo + xs1 + xi) * kSP + j] = (__bf16)gelu_f(acc1[q] + bj);
-      }
+      store(acc0, c * 32 + j);
+      if (c1 < nch) store(acc1, c1 * 32 + j);
     }
     __syncthreads();
   }

@@ -624,12 +624,14 @@ __global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfie
 // the tile + 1-site halo (its stage value stays in LDS), the second on the tile; stage input
 // and D read with a 2-site halo.  Per site and step 9 W (+ 2 W of D) instead of 17 W (+ 4 W).
 // Iteration p stores input plane z0-2+p, forms k_A on plane z0-3+p and the outputs on plane
-// z0-4+p; rings: input 3 planes, D 4 planes (the second stage reaches one plane further
-// back), y' 3 planes; three barriers per iteration.
+// z0-4+p; rings: input and y' 3 planes each.  Heterogeneous D: the harmonic mean of each face
+// depends on D only, so it is formed once per plane and used by both cells and both stages
+// (x and y faces in 3-plane LDS rings, z faces in per-thread queues along the columns; D
+// itself in a 2-plane ring): about 4 FP64 divisions per site and step pair instead of 13.
 
 template <int OX, int OY, int NT>
 struct DPairCfg {
-  static constexpr int W2 = OX + 4, N2 = W2 * (OY + 4);  // stage input, D
+  static constexpr int W2 = OX + 4, H2 = OY + 4, N2 = W2 * H2;  // stage input, D
   static constexpr int W1 = OX + 2, N1 = W1 * (OY + 2);  // k_A, y'
   static constexpr int L2 = (N2 + NT - 1) / NT;
   static constexpr int NR = 2 * W1 + 2 * OY;              // tile+1 sites outside the tile
@@ -682,9 +684,10 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
                                                     T* __restrict__ b, Dims d, int ntx, int nty,
                                                     StageCoef<T> co) {
   using C = DPairCfg<OX, OY, NT>;
-  constexpr int ND = FVM ? 4 : 1;
+  constexpr int ND = FVM ? 2 : 1, NF = FVM ? 3 : 1, NFN = FVM ? C::N2 : 1;
   __shared__ T ys[3][C::N2];
   __shared__ T ds[ND][FVM ? C::N2 : 1];
+  __shared__ T fx[NF][NFN], fy[NF][NFN];  // FVM: face harmonic means of the last 3 planes
   __shared__ T y2s[3][C::N1];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -712,16 +715,19 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
     oin[q] = (x0 + ox < d.nx) && (gy < d.ny);
     ooff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
   }
+  // the 1-site ring around the tile: E1 coordinates
+  int rmx[C::LR], rmy[C::LR];
   unsigned roff[PAIR == 1 ? C::LR : 1];  // PAIR 1: psi at the ring sites of stage A
-  if constexpr (PAIR == 1) {
 #pragma unroll
-    for (int s = 0; s < C::LR; ++s) {
-      const int rr = t + s * NT;
-      int mx = 0, my = 0;
-      if (rr < C::NR) ring1_site<C, OY>(rr, mx, my);
+  for (int s = 0; s < C::LR; ++s) {
+    const int rr = t + s * NT;
+    int mx = 0, my = 0;
+    if (rr < C::NR) ring1_site<C, OY>(rr, mx, my);
+    rmx[s] = rr < C::NR ? mx : -1;
+    rmy[s] = my;
+    if constexpr (PAIR == 1)
       roff[s] = rr < C::NR ? (unsigned)(wrap_any(x0 + mx - 1, d.nx) + d.nx * wrap_any(y0 + my - 1, d.ny))
                            : 0u;
-    }
   }
 
   T yreg[C::L2], dreg[FVM ? C::L2 : 1];
@@ -738,12 +744,16 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
   T pt[PAIR == 1 ? C::OR : 1];  // PAIR 1: psi at the tile sites, plane of k_A
   T k0r[PAIR == 1 ? C::OR : 1] = {}, k1r[PAIR == 1 ? C::OR : 1] = {};  // k_A, newest first
   T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+  // FVM: z-face means along this thread's columns: tile sites keep the last three planes'
+  // (newest first), ring sites the last one
+  T fzt0[FVM ? C::OR : 1] = {}, fzt1[FVM ? C::OR : 1] = {}, fzt2[FVM ? C::OR : 1] = {};
+  T fzr[FVM ? C::LR : 1] = {};
 
   load(z0 - 2);
   const int last = nout + 3;
   for (int p = 0; p <= last; ++p) {
-    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // y/y' slots of p, p-1, p-2
-    const int d0 = p % ND, d1 = (p + ND - 1) % ND, d2 = (p + ND - 2) % ND, d3 = (p + ND - 3) % ND;
+    const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // slots of p, p-1, p-2
+    const int d0 = p % ND, d1 = (p + ND - 1) % ND;               // D slots of p, p-1
 #pragma unroll
     for (int s = 0; s < C::L2; ++s)
       if (t + s * NT < C::N2) {
@@ -773,19 +783,66 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
       }
     }
     __syncthreads();
+    if constexpr (FVM) {
+      // x- and y-face means of the plane just stored (fvm/CellLaplacian.hpp:48-50): the face
+      // between a site and its +x (+y) neighbour, shared by both cells and both stages
+      const T* dp = ds[d0];
+#pragma unroll
+      for (int s = 0; s < C::L2; ++s) {
+        const int idx = t + s * NT;
+        if (idx < C::N2) {
+          const int mx = idx % C::W2, my = idx / C::W2;
+          if (mx < C::W2 - 1) fx[s0][idx] = FvmRhs<T>::hm(dp[idx], dp[idx + 1]);
+          if (my < C::H2 - 1) fy[s0][idx] = FvmRhs<T>::hm(dp[idx], dp[idx + C::W2]);
+        }
+      }
+      if (p == 1) {  // z-faces of plane z0-2 (under the first k_A plane)
+#pragma unroll
+        for (int q = 0; q < C::OR; ++q) {
+          const int e2 = (oy0 + q * C::ROWS + 2) * C::W2 + (ox + 2);
+          fzt0[q] = FvmRhs<T>::hm(ds[d1][e2], ds[d0][e2]);
+        }
+#pragma unroll
+        for (int s = 0; s < C::LR; ++s)
+          if (rmx[s] >= 0) {
+            const int e2 = (rmy[s] + 1) * C::W2 + (rmx[s] + 1);
+            fzr[s] = FvmRhs<T>::hm(ds[d1][e2], ds[d0][e2]);
+          }
+      }
+      __syncthreads();
+    }
     if (p >= 2) {  // k_A and y' on plane z0-3+p, tile+1
       const int zA = z0 - 3 + p;
       const bool own = zA >= z0 && zA < z0 + nout;
-      auto kA = [&](int mx, int my) {
+      // right-hand side at E1 site (mx, my); FVM: fzp / fzm are the z-face means above / below
+      auto kA = [&](int mx, int my, T fzp, T fzm) {
         const int o = (my + 1) * C::W2 + (mx + 1);
-        return diff_rhs<T, FVM>(ys[s1], ys[s2], ys[s0], FVM ? ds[d1] : nullptr,
-                                FVM ? ds[d2] : nullptr, FVM ? ds[d0] : nullptr, o, C::W2, ihx2,
-                                ihy2, ihz2);
+        if constexpr (FVM) {
+          const T* yc = ys[s1];
+          const T pc = yc[o];
+          const T dxp = fx[s1][o], dxm = fx[s1][o - 1];
+          const T dyp = fy[s1][o], dym = fy[s1][o - C::W2];
+          const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
+          const T ry = (dyp * (yc[o + C::W2] - pc) - dym * (pc - yc[o - C::W2])) * ihy2;
+          const T rz = (fzp * (ys[s0][o] - pc) - fzm * (pc - ys[s2][o])) * ihz2;
+          return rx + ry + rz;
+        } else {
+          return lap_lds(ys[s1], ys[s2], ys[s0], o, C::W2, ihx2, ihy2, ihz2);
+        }
       };
 #pragma unroll
       for (int q = 0; q < C::OR; ++q) {  // tile sites: the output owner
         const int mx = ox + 1, my = oy0 + q * C::ROWS + 1;
-        const T kv = kA(mx, my);
+        T fzp = T(0), fzm = T(0);
+        if constexpr (FVM) {
+          const int o = (my + 1) * C::W2 + (mx + 1);
+          fzp = FvmRhs<T>::hm(ds[d1][o], ds[d0][o]);  // planes zA, zA+1
+          fzm = fzt0[q];
+          fzt2[q] = fzt1[q];
+          fzt1[q] = fzt0[q];
+          fzt0[q] = fzp;
+        }
+        const T kv = kA(mx, my, fzp, fzm);
         const int e = my * C::W1 + mx;
         if constexpr (PAIR == 0) {
           y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
@@ -799,11 +856,16 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
       }
 #pragma unroll
       for (int s = 0; s < C::LR; ++s) {  // the 1-site ring around the tile
-        const int rr = t + s * NT;
-        if (rr < C::NR) {
-          int mx, my;
-          ring1_site<C, OY>(rr, mx, my);
-          const T kv = kA(mx, my);
+        if (rmx[s] >= 0) {
+          const int mx = rmx[s], my = rmy[s];
+          T fzp = T(0), fzm = T(0);
+          if constexpr (FVM) {
+            const int o = (my + 1) * C::W2 + (mx + 1);
+            fzp = FvmRhs<T>::hm(ds[d1][o], ds[d0][o]);
+            fzm = fzr[s];
+            fzr[s] = fzp;
+          }
+          const T kv = kA(mx, my, fzp, fzm);
           const int e = my * C::W1 + mx;
           if constexpr (PAIR == 0) y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
           else y2s[s0][e] = pe[s] + co.dt * kv;
@@ -818,7 +880,7 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
       }
     }
     __syncthreads();
-    if (p >= 4) {  // outputs on plane z0-4+p from y' planes p-2..p and D planes p-3..p-1
+    if (p >= 4) {  // outputs on plane z0-4+p from y' planes p-2..p
       const size_t kb = (size_t)d.plane * (size_t)(z0 - 4 + p);
 #pragma unroll
       for (int q = 0; q < C::OR; ++q) {
@@ -827,20 +889,14 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           const int o1 = (oy + 1) * C::W1 + (ox + 1);
           T kv;
           if constexpr (FVM) {
-            // D on the tile+1 region of planes p-3 (lo), p-2 (centre), p-1 (hi), stored with the
-            // 2-site halo: offset (oy+2)*W2 + (ox+2)
-            const int o2 = (oy + 2) * C::W2 + (ox + 2);
+            const int o2 = (oy + 2) * C::W2 + (ox + 2);  // the same site in the face planes
             const T* yc = y2s[s1];
-            const T* ylo = y2s[s2];
-            const T* yhi = y2s[s0];
-            const T* dc = ds[d2];
-            const T pc = yc[o1], Dc = dc[o2];
-            const T dxp = FvmRhs<T>::hm(Dc, dc[o2 + 1]), dxm = FvmRhs<T>::hm(dc[o2 - 1], Dc);
-            const T dyp = FvmRhs<T>::hm(Dc, dc[o2 + C::W2]), dym = FvmRhs<T>::hm(dc[o2 - C::W2], Dc);
-            const T dzp = FvmRhs<T>::hm(Dc, ds[d1][o2]), dzm = FvmRhs<T>::hm(ds[d3][o2], Dc);
+            const T pc = yc[o1];
+            const T dxp = fx[s2][o2], dxm = fx[s2][o2 - 1];
+            const T dyp = fy[s2][o2], dym = fy[s2][o2 - C::W2];
             const T rx = (dxp * (yc[o1 + 1] - pc) - dxm * (pc - yc[o1 - 1])) * ihx2;
             const T ry = (dyp * (yc[o1 + C::W1] - pc) - dym * (pc - yc[o1 - C::W1])) * ihy2;
-            const T rz = (dzp * (yhi[o1] - pc) - dzm * (pc - ylo[o1])) * ihz2;
+            const T rz = (fzt1[q] * (y2s[s0][o1] - pc) - fzt2[q] * (pc - y2s[s2][o1])) * ihz2;
             kv = rx + ry + rz;
           } else {
             kv = lap_lds(y2s[s1], y2s[s2], y2s[s0], o1, C::W1, ihx2, ihy2, ihz2);

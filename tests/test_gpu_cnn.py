@@ -1,5 +1,5 @@
 """GPU kernels of the learned corrector against the FP64 reference network
-(oracle/cnn_oracle.py) evaluated with the same bfloat16 roundings (input, weights,
+(oracle/cnn_oracle.py) evaluated with the same bfloat16 roundings (32 -> 32 weights,
 activations, pre-activation gradients); the remaining differences are FP32 accumulation
 order and single-precision GELU."""
 import importlib.util
