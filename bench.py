@@ -115,9 +115,12 @@ def ch3d(a):
                 best, ms = gc.ch_tune(ctx, g, dtype)
             tiles[name] = gc.ch_config_shape(g, gc.ch_get_config(g, dtype))
         extra["tile_ox_oy_zc"] = tiles
-        fused = gc.ch_config_fused(g, gc.ch_get_config(g, gc.F32))
-        extra["two_stage_kernels"] = {"f32": fused,
-                                      "f64": gc.ch_config_fused(g, gc.ch_get_config(g, gc.F64))}
+        kinds = {name: gc.ch_config_kind(g, gc.ch_get_config(g, dtype))
+                 for dtype, name in ((gc.F32, "f32"), (gc.F64, "f64"))}
+        fused = kinds["f32"] > 0
+        kname = "k_ch_pairq3d" if kinds["f32"] == 2 else "k_ch_pair3d"
+        extra["two_stage_kernels"] = {k: ["none", "plane rings", "register queues"][v]
+                                      for k, v in kinds.items()}
 
         d32 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
         if a.warmup:
@@ -149,7 +152,7 @@ def ch3d(a):
                 "hbm_GBs_alg": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9,
                 "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS,
                 "traffic": profiled_traffic(
-                    f"ch3d_rk4_{'pair' if extra['two_stage_kernels']['f64'] else 'stage'}_kernels_{n}_f32.md",
+                    f"ch3d_rk4_{'pair' if kinds['f64'] else 'stage'}_kernels_{n}_f32.md",
                     "traffic_bytes_per_step_f64")}
 
     if not a.no_learned:
@@ -190,7 +193,7 @@ def ch3d(a):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(f"ch3d_rk4_{'pair' if fused else 'stage'}_kernels_{n}_f32.md"),
-                     "kernel": (f"k_ch_pair3d<float,{tiles['f32'][0]},{tiles['f32'][1]},{tiles['f32'][2]},...> "
+                     "kernel": (f"{kname}<float,{tiles['f32'][0]},{tiles['f32'][1]},{tiles['f32'][2]},...> "
                                 "x2 (one RK4 step = S1+S2 and S3+S4 launches)") if fused else
                                "k_ch_stage3d<float,...> x4 (one RK4 step = 4 stage launches)",
                      "step_kernels_ms": sms, "alg_bytes_per_step": bytes_step},

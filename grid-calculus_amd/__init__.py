@@ -434,12 +434,18 @@ def ch_config_shape(g, i):
     return ox.value, oy.value, zc.value
 
 
-def ch_config_fused(g, i):
-    """True if 3-D candidate i runs RK4 as two two-stage kernels."""
+def ch_config_kind(g, i):
+    """0: one kernel per RK4 stage; 1: two-stage kernels with LDS plane rings; 2: two-stage
+    kernels with register queues along z."""
     r = lib().gc_ch_config_fused(C.byref(g), i)
     if r < 0:
         raise GridcalcError(1, f"gridcalc: config id {i} out of range")
-    return bool(r)
+    return r
+
+
+def ch_config_fused(g, i):
+    """True if 3-D candidate i runs RK4 as two two-stage kernels."""
+    return ch_config_kind(g, i) > 0
 
 
 def ch_set_config(g, dtype, i):

@@ -26,9 +26,15 @@ from . import (F32, F64, RK4, Context, Corrector, DeviceArray, Ensemble, Grid, N
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
+# skip_first: macro steps at the start of every trajectory left out of training (the learned
+# integrator takes them with plain RK4: the first step from the noise state is the one the
+# network predicts worst); sample_pow: training pairs are drawn with probability proportional
+# to rms(increment of their macro index)^-sample_pow (0: uniform), so that the slow late
+# increments, whose small errors accumulate over most of a run, are not drowned by the large
+# early ones in the squared loss.
 CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000, n_steps=2000,
               batch_traj=100, minibatch=16, iters_per_batch=9000, lr_max=2e-3, lr_min=2e-5,
-              seed=12345)
+              seed=12345, skip_first=0, sample_pow=0.0)
 
 
 def params_path(H, W):
@@ -100,6 +106,19 @@ def snapshot(ctx, data, n_macro, i, m):
     return out
 
 
+def increment_rms_by_macro(ctx, data, n_macro, n_traj=4):
+    """RMS of (snapshot m+1 - snapshot m) for every macro index m, over the first n_traj
+    trajectories."""
+    rms = np.zeros(n_macro)
+    for m in range(n_macro):
+        sq = 0.0
+        for i in range(n_traj):
+            d = snapshot(ctx, data, n_macro, i, m + 1).astype(np.float64) - snapshot(ctx, data, n_macro, i, m)
+            sq += float((d * d).mean())
+        rms[m] = math.sqrt(sq / n_traj)
+    return rms
+
+
 def increment_scale(ctx, data, n_macro, n_sample=64, seed=0):
     """RMS of (snapshot m+1 - snapshot m) over a sample of pairs."""
     rng = np.random.default_rng(seed)
@@ -132,6 +151,10 @@ def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
                 scale = increment_scale(ctx, data, n_macro)
                 net.set_params(init_params(cfg["seed"]), scale)
                 log(f"  increment scale {scale:.4e}")
+                ms = np.arange(cfg["skip_first"], n_macro)
+                w = increment_rms_by_macro(ctx, data, n_macro)[ms] ** -cfg["sample_pow"]
+                probs = w / w.sum()
+                log(f"  macro indices {ms[0]}..{ms[-1]}, sampling weight max/min {w.max() / w.min():.1f}")
             per_traj = n_macro + 1
             target = C.c_void_p(data.ptr.value + H * W * 4)  # snapshot m+1 of pair index m
 
@@ -142,7 +165,7 @@ def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
             losses = []
             for _ in range(cfg["iters_per_batch"]):
                 tr = rng.integers(0, cfg["batch_traj"], cfg["minibatch"])
-                m = rng.integers(0, n_macro, cfg["minibatch"])
+                m = rng.choice(ms, cfg["minibatch"], p=probs)
                 idx = (tr * per_traj + m).astype(np.int32)
                 frac = it / max(1, total_iters - 1)
                 lr = cfg["lr_min"] + 0.5 * (cfg["lr_max"] - cfg["lr_min"]) * (1 + math.cos(math.pi * frac))
@@ -193,12 +216,14 @@ def one_step_errors(net, cfg=CONFIG, n_traj=8, seed=777001, log=print):
     return math.sqrt(e_net / norm), math.sqrt(e_id / norm)
 
 
-def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6):
+def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6, warm=None):
     """Runs the learned integrator and the plain FP64 RK4 from the same U(-0.05, 0.05) state
     over n_steps steps (default cfg n_steps, i.e. T = n_steps * dt); returns a dict with the
     relative L2 and max errors of the learned result against the RK4 result, the fallback
-    count and both wall times (device synchronised)."""
+    count and both wall times (device synchronised).  The first `warm` macro steps (default
+    cfg skip_first) are plain RK4 in the learned run too."""
     ctx = net.ctx
+    warm = cfg.get("skip_first", 0) if warm is None else warm
     H, W, dt, sub = cfg["H"], cfg["W"], cfg["dt"], cfg["substeps"]
     n_steps = cfg["n_steps"] if n_steps is None else n_steps
     assert n_steps % sub == 0
@@ -212,7 +237,9 @@ def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6)
     ctx.sync()
     t_rk4 = time.perf_counter() - t0
     t0 = time.perf_counter()
-    nf = net.integrate(g, lrn, cfg["M"], cfg["kappa"], dt, sub, n_steps // sub, mass_tol)
+    if warm:
+        ch_integrate(ctx, g, lrn, cfg["M"], cfg["kappa"], dt, warm * sub, RK4)
+    nf = net.integrate(g, lrn, cfg["M"], cfg["kappa"], dt, sub, n_steps // sub - warm, mass_tol)
     ctx.sync()
     t_lrn = time.perf_counter() - t0
     r, l = ref.numpy(ctx), lrn.numpy(ctx)

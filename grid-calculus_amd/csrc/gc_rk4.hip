@@ -1256,6 +1256,192 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
   }
 }
 
+// The same two stages with the z direction in registers.  A thread owns columns (x, y) of
+// the tile+4 region (column t + s NT, row-major) and keeps, per column, the last three planes
+// of each quantity the stages form there -- y, mu_A, y', mu_B -- as register queues; LDS holds
+// one plane of each quantity, for the in-plane neighbours only.  A Laplacian then reads 4 LDS
+// values (x and y neighbours) instead of 7 and takes its centre and z neighbours from the
+// queue; the LDS addresses are loop invariant and each plane is staged once.  Iteration p
+// loads plane P_p = z0 - 4 + p and runs four phases separated by barriers:
+//   A: mu_A on P_{p-1} (columns at least 1 from the region edge)     [stores mu_B(P_{p-4})]
+//   B: k_A and y' on P_{p-2} (at least 2)                            [stores y(P_p)]
+//   C: mu_B on P_{p-3} (at least 3)                                  [stores mu_A(P_{p-1})]
+//   D: the outputs on P_{p-4} (the tile)                             [stores y'(P_{p-2})]
+// Each LDS plane is written in the phase after its last reader and read after the next
+// barrier, so one slot per quantity suffices.  The arithmetic (operands, order) is that of
+// k_ch_pair3d.
+template <class T>
+__device__ __forceinline__ T lap_q(const T* __restrict__ pl, int o, int W, T cc, T zlo, T zhi,
+                                   T ihx2, T ihy2, T ihz2) {
+  T dxx = 0, dyy = 0, dzz = 0;
+  dxx += T(1) * pl[o - 1];
+  dyy += T(1) * pl[o - W];
+  dzz += T(1) * zlo;
+  dxx += T(-2) * cc;
+  dyy += T(-2) * cc;
+  dzz += T(-2) * cc;
+  dxx += T(1) * pl[o + 1];
+  dyy += T(1) * pl[o + W];
+  dzz += T(1) * zhi;
+  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+}
+
+template <class T, int OX, int OY, int ZC, int NT, int PAIR>
+__global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __restrict__ y,
+                                                   const T* __restrict__ psi,
+                                                   T* __restrict__ out, T* __restrict__ a,
+                                                   T* __restrict__ b, Dims d, int ntx, int nty,
+                                                   StageCoef<T> co) {
+  using C = PairCfg<OX, OY, NT>;
+  constexpr int W = C::W4, H = C::H4, N = C::N4, L = C::L4;
+  __shared__ T ys[N], ms[N], ws[N], bs[N];  // y, mu_A, y', mu_B: one plane each
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned r = bid / (unsigned)ntx;
+  const int ty = (int)(r % (unsigned)nty);
+  const int zc = (int)(r / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
+  const int nout = min(ZC, d.nz - z0);
+
+  unsigned goff[L];  // the column's site in a plane (wrapped)
+  int dist[L];       // distance from the region edge; -1: no column
+  bool oin[L];       // the column is a tile site inside the grid (an output)
+#pragma unroll
+  for (int s = 0; s < L; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % W, py = idx / W;
+    if (idx < N) {
+      goff[s] = (unsigned)(wrap_any(x0 + px - 4, d.nx) + d.nx * wrap_any(y0 + py - 4, d.ny));
+      dist[s] = min(min(px, py), min(W - 1 - px, H - 1 - py));
+      oin[s] = dist[s] >= 4 && x0 + px - 4 < d.nx && y0 + py - 4 < d.ny;
+    } else {
+      goff[s] = 0u;
+      dist[s] = -1;
+      oin[s] = false;
+    }
+  }
+  // Queues of the last three planes of y, mu_A, y', mu_B (and, PAIR 1, k_A): physical slot
+  // R = p mod 3 takes the value formed in iteration p, so slot (R + 2) mod 3 holds iteration
+  // p-1's and (R + 1) mod 3 iteration p-2's.  The loop is unrolled by three so that R is a
+  // compile-time constant and the queues rotate by renaming instead of register moves.
+  T yq[3][L] = {}, mq[3][L] = {}, wq[3][L] = {}, bq[3][L] = {};
+  T kq[3][PAIR == 1 ? L : 1] = {};
+  T yn[L] = {};  // the next plane of y, in flight
+  T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
+  T pv[L] = {}, av[L] = {}, bv[L] = {};
+
+  auto load_y = [&](int plane) {
+    const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+#pragma unroll
+    for (int s = 0; s < L; ++s)
+      if (dist[s] >= 0) yn[s] = yp[goff[s]];
+  };
+  const int last = nout + 7;
+  auto iter = [&](auto rc, int p) {
+    constexpr int R0 = decltype(rc)::value, R1 = (R0 + 2) % 3, R2 = (R0 + 1) % 3;
+#pragma unroll
+    for (int s = 0; s < L; ++s) yq[R0][s] = yn[s];
+    if (p < last) load_y(z0 - 3 + p);
+    if constexpr (PAIR == 1) {
+      if (p >= 4) {
+        const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 6 + p, d.nz);
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+          if (dist[s] >= 2) pe[s] = pp[goff[s]];
+      }
+    }
+    if (p >= 8) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+#pragma unroll
+      for (int s = 0; s < L; ++s) {
+        if (oin[s]) {
+          pv[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
+          if constexpr (PAIR == 1) {
+            av[s] = __builtin_nontemporal_load(a + kb + goff[s]);
+            bv[s] = __builtin_nontemporal_load(b + kb + goff[s]);
+          }
+        }
+      }
+    }
+    // A: mu_A on P_{p-1}; store mu_B(P_{p-4}) (formed in phase C of iteration p-1)
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+      const int o = t + s * NT;
+      if (p >= 7 && dist[s] >= 3) bs[o] = bq[R1][s];
+      if (p >= 2 && dist[s] >= 1) {
+        const T yy = yq[R1][s];
+        const T l = lap_q(ys, o, W, yy, yq[R2][s], yq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2);
+        mq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
+      }
+    }
+    __syncthreads();
+    // B: k_A and y' on P_{p-2}; store y(P_p)
+    {
+      const int zA = z0 - 6 + p;
+      const bool own = zA >= z0 && zA < z0 + nout;
+#pragma unroll
+      for (int s = 0; s < L; ++s) {
+        const int o = t + s * NT;
+        if (dist[s] >= 0) ys[o] = yq[R0][s];
+        if (p >= 4 && dist[s] >= 2) {
+          const T kv =
+              lap_q(ms, o, W, mq[R1][s], mq[R2][s], mq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          if constexpr (PAIR == 0) {
+            wq[R0][s] = yq[R2][s] + co.half_dt * kv;
+            if (own && oin[s])
+              __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + goff[s]);
+          } else {
+            wq[R0][s] = pe[s] + co.dt * kv;
+            kq[R0][s] = kv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // C: mu_B on P_{p-3}; store mu_A(P_{p-1})
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+      const int o = t + s * NT;
+      if (p >= 2 && dist[s] >= 1) ms[o] = mq[R0][s];
+      if (p >= 6 && dist[s] >= 3) {
+        const T yy = wq[R1][s];
+        const T l = lap_q(ws, o, W, yy, wq[R2][s], wq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2);
+        bq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
+      }
+    }
+    __syncthreads();
+    // D: outputs on P_{p-4}; store y'(P_{p-2})
+    {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+#pragma unroll
+      for (int s = 0; s < L; ++s) {
+        const int o = t + s * NT;
+        if (p >= 4 && dist[s] >= 2) ws[o] = wq[R0][s];
+        if (p >= 8 && oin[s]) {
+          const T kv =
+              lap_q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          if constexpr (PAIR == 0) {
+            __builtin_nontemporal_store(kv, b + kb + goff[s]);
+            __builtin_nontemporal_store(pv[s] + co.half_dt * kv, out + kb + goff[s]);
+          } else {
+            const T bsum = bv[s] + kq[R2][s];  // k3 of this plane (phase B of iteration p-2)
+            __builtin_nontemporal_store(pv[s] + (co.sixth_dt * (av[s] + kv) + co.third_dt * bsum),
+                                        out + kb + goff[s]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  };
+  load_y(z0 - 4);
+  for (int p = 0; p <= last; p += 3) {
+    iter(std::integral_constant<int, 0>{}, p);
+    if (p + 1 <= last) iter(std::integral_constant<int, 1>{}, p + 1);
+    if (p + 2 <= last) iter(std::integral_constant<int, 2>{}, p + 2);
+  }
+}
+
 // ---- tile shapes of the Cahn-Hilliard stage kernels and their selection ------------------
 //
 // Candidate shapes (tile width OX, tile height OY, z-chunk ZC).  The default is used until
@@ -1278,20 +1464,24 @@ constexpr int kStage3F32 = 6, kStage3F64 = 4, kDefault2 = 2;
 // Grids with 1 < Nz < 8 use the untiled kernel (the z-march would mostly reload halos).
 inline bool ch_tiled(const Dims& d) { return d.nz == 1 || d.nz >= 8; }
 
-// 3-D two-stage (fused pair) shapes: ids kN3 .. kN3 + kNP - 1; block size NT threads.  The
-// 128 x 16 and 64 x 32 tiles need more LDS than a workgroup has in FP64; FP64 runs
-// 64 x 16 x ZC there.
+// 3-D two-stage (fused pair) shapes: ids kN3 .. kN3 + kNP - 1; block size NT threads; q:
+// register-queue kernel (k_ch_pairq3d) instead of the plane-ring kernel (k_ch_pair3d).  The
+// ring kernel's 128 x 16 and 64 x 32 tiles need more LDS than a workgroup has in FP64, and
+// the queue kernel's FP64 queues fit the registers of a 1024-thread block only for 64 x 16
+// tiles; FP64 runs 64 x 16 x ZC there.
 struct PairTile {
-  int ox, oy, zc, nt;
+  int ox, oy, zc, nt, q;
 };
-constexpr PairTile kChPairCfgs[] = {{64, 8, 32, 512},    {64, 16, 16, 1024},  {64, 16, 32, 1024},
-                                    {64, 16, 64, 1024},  {128, 16, 16, 1024}, {128, 16, 32, 1024},
-                                    {128, 16, 64, 1024}, {64, 32, 16, 1024},  {64, 32, 32, 1024}};
+constexpr PairTile kChPairCfgs[] = {
+    {64, 8, 32, 512, 0},    {64, 16, 16, 1024, 0},  {64, 16, 32, 1024, 0},  {64, 16, 64, 1024, 0},
+    {128, 16, 16, 1024, 0}, {128, 16, 32, 1024, 0}, {128, 16, 64, 1024, 0}, {64, 32, 16, 1024, 0},
+    {64, 32, 32, 1024, 0},  {64, 8, 32, 512, 1},    {64, 16, 32, 1024, 1},  {64, 16, 64, 1024, 1},
+    {64, 32, 32, 1024, 1},  {64, 32, 64, 1024, 1},  {128, 16, 32, 1024, 1}};
 constexpr int kNP = sizeof(kChPairCfgs) / sizeof(kChPairCfgs[0]);
 inline bool is_pair(const Dims& d, int id) { return d.nz > 1 && id >= kN3; }
-// 3-D RK4 defaults: the measured best on 256^3 (FP32: two-stage 64x32x32, FP64: two-stage
-// 64x16x64)
-constexpr int kDefault3F32 = kN3 + 8, kDefault3F64 = kN3 + 3;
+// 3-D RK4 defaults: the measured best on 256^3 (register-queue two-stage kernels; FP32
+// 64x32x32, FP64 64x16x64)
+constexpr int kDefault3F32 = kN3 + 12, kDefault3F64 = kN3 + 11;
 
 
 template <class T, int OX, int OY, int ZC>
@@ -1351,17 +1541,26 @@ void launch_ch_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf, 
 #undef GC_CH2
 }
 
-template <class T, int OX, int OY, int ZC, int NT>
+template <class T, int OX, int OY, int ZC, int NT, bool Q = false>
 void launch_pair3(gc_context* ctx, const Dims& d, const CHCoef<T>& cf, int pair, const T* y,
                   const T* psi, T* out, T* a, T* b, const StageCoef<T>& co) {
   const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY, nzc = (d.nz + ZC - 1) / ZC;
   const dim3 grid((unsigned)(ntx * nty * nzc));
-  if (pair == 0)
-    hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 0>), grid, dim3(NT), 0, ctx->stream, cf, y,
-                       psi, out, a, b, d, ntx, nty, co);
-  else
-    hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 1>), grid, dim3(NT), 0, ctx->stream, cf, y,
-                       psi, out, a, b, d, ntx, nty, co);
+  if constexpr (Q) {
+    if (pair == 0)
+      hipLaunchKernelGGL((k_ch_pairq3d<T, OX, OY, ZC, NT, 0>), grid, dim3(NT), 0, ctx->stream, cf,
+                         y, psi, out, a, b, d, ntx, nty, co);
+    else
+      hipLaunchKernelGGL((k_ch_pairq3d<T, OX, OY, ZC, NT, 1>), grid, dim3(NT), 0, ctx->stream, cf,
+                         y, psi, out, a, b, d, ntx, nty, co);
+  } else {
+    if (pair == 0)
+      hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 0>), grid, dim3(NT), 0, ctx->stream, cf,
+                         y, psi, out, a, b, d, ntx, nty, co);
+    else
+      hipLaunchKernelGGL((k_ch_pair3d<T, OX, OY, ZC, NT, 1>), grid, dim3(NT), 0, ctx->stream, cf,
+                         y, psi, out, a, b, d, ntx, nty, co);
+  }
 }
 
 template <class T>
@@ -1389,9 +1588,24 @@ void launch_pair_cfg(gc_context* ctx, int id, const Dims& d, const CHCoef<T>& cf
       if constexpr (f32) launch_pair3<T, 64, 32, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
       else launch_pair3<T, 64, 16, 16, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
       break;
-    default:
+    case 8:
       if constexpr (f32) launch_pair3<T, 64, 32, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
       else launch_pair3<T, 64, 16, 32, 1024>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    case 9: launch_pair3<T, 64, 8, 32, 512, true>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 10: launch_pair3<T, 64, 16, 32, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 11: launch_pair3<T, 64, 16, 64, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co); break;
+    case 12:
+      if constexpr (f32) launch_pair3<T, 64, 32, 32, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 32, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    case 13:
+      if constexpr (f32) launch_pair3<T, 64, 32, 64, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 64, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      break;
+    default:
+      if constexpr (f32) launch_pair3<T, 128, 16, 32, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
+      else launch_pair3<T, 64, 16, 32, 1024, true>(ctx, d, cf, pair, y, psi, out, a, b, co);
       break;
   }
 }
@@ -1783,7 +1997,8 @@ int gc_ch_config_count(const gc_grid* g) {
 
 int gc_ch_config_fused(const gc_grid* g, int id) {
   if (!g || id < 0 || id >= gc_ch_config_count(g)) return -1;
-  return is_pair(dims_of(g), id) ? 1 : 0;
+  if (!is_pair(dims_of(g), id)) return 0;
+  return kChPairCfgs[id - kN3].q ? 2 : 1;
 }
 
 gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc) {
