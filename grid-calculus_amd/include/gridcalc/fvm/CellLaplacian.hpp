@@ -21,22 +21,26 @@ namespace detail {
 
 inline double harmonicMean(double a, double b) noexcept { return 2.0 * a * b / (a + b); }
 
+// The reference's message for the first cell with D <= 0 (flat index `bad`).
+template <class T>
+[[noreturn]] inline void throwNonPositiveD(const core::Field<T>& D, int64_t bad) {
+  T value{};
+  hip::download(&value, D.deviceData() + bad, sizeof(T));
+  throw std::invalid_argument(
+      "fvm::cellLaplacian: D must be strictly positive at every "
+      "cell (the harmonic-mean face-D averaging is undefined "
+      "otherwise); first violation at flat index " +
+      std::to_string(static_cast<std::size_t>(bad)) + " with D=" +
+      std::to_string(static_cast<double>(value)));
+}
+
 template <class T>
 inline void requirePositiveD(const core::Field<T>& D) {
   const gc_grid g = hip::toGrid(D.getGrid());
   int64_t bad = -1;
   hip::check(gc_first_nonpositive(hip::context(), &g, hip::dtype_of<T>::value, D.deviceData(),
                                   &bad));
-  if (bad >= 0) {
-    T value{};
-    hip::download(&value, D.deviceData() + bad, sizeof(T));
-    throw std::invalid_argument(
-        "fvm::cellLaplacian: D must be strictly positive at every "
-        "cell (the harmonic-mean face-D averaging is undefined "
-        "otherwise); first violation at flat index " +
-        std::to_string(static_cast<std::size_t>(bad)) + " with D=" +
-        std::to_string(static_cast<double>(value)));
-  }
+  if (bad >= 0) throwNonPositiveD(D, bad);
 }
 
 template <class T>
@@ -46,12 +50,14 @@ inline core::Field<T> cellLaplacianImpl(const core::Field<T>& psi, const core::F
   diff::detail::requireAxisExtent("y", grid.getNy(), 1);
   diff::detail::requireAxisExtent("z", grid.getNz(), 1);
   solve::detail::requireSameGrid(psi, D, "fvm::cellLaplacian: D");
-  requirePositiveD(D);
   core::Field<T> result(grid, core::detail::device_only);
   const gc_grid g = hip::toGrid(grid);
-  int64_t bad = -1;
-  hip::check(gc_cell_laplacian(hip::context(), &g, hip::dtype_of<T>::value, psi.deviceData(),
-                               D.deviceData(), result.deviceDataDiscard(), &bad));
+  int64_t bad = -1;  // the D > 0 check runs inside the operator's kernel
+  const gc_status st = gc_cell_laplacian(hip::context(), &g, hip::dtype_of<T>::value,
+                                         psi.deviceData(), D.deviceData(),
+                                         result.deviceDataDiscard(), &bad);
+  if (st == GC_ERR_NONPOS_D && bad >= 0) throwNonPositiveD(D, bad);
+  hip::check(st);
   return result;
 }
 

@@ -115,9 +115,10 @@ gc_status gc_divergence_mat(gc_context* ctx, const gc_grid* g, gc_dtype dt, int 
 /* diff::biharmonic<Order> / diff::d4 (diff/Biharmonic.hpp:48-119). */
 gc_status gc_biharmonic(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
                         const void* in, void* out);
-/* fvm::cellLaplacian(psi, D) (fvm/CellLaplacian.hpp:91-143).  Checks D > 0 first
- * (CellLaplacian.hpp:99-109): on a violation returns GC_ERR_NONPOS_D and, when
- * first_bad is not NULL, stores the smallest violating flat index there. */
+/* fvm::cellLaplacian(psi, D) (fvm/CellLaplacian.hpp:91-143).  Checks D > 0
+ * (CellLaplacian.hpp:99-109) inside the operator's kernel: on a violation returns
+ * GC_ERR_NONPOS_D (out then holds no meaningful values) and, when first_bad is not NULL,
+ * stores the smallest violating flat index there. */
 gc_status gc_cell_laplacian(gc_context* ctx, const gc_grid* g, gc_dtype dt,
                             const void* psi, const void* D, void* out, int64_t* first_bad);
 /* Smallest flat index with !(D > 0), or -1 (the scan of CellLaplacian.hpp:99-109). */

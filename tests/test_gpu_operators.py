@@ -120,6 +120,22 @@ def test_cell_laplacian_rejects_nonpositive_d(gc, ctx):
     assert gc.first_nonpositive(ctx, g, gc.DeviceArray.from_numpy(ctx, np.ones(g.shape))) == -1
 
 
+@pytest.mark.parametrize("shape,bad", [((70, 9, 40), [70 * 9 * 33 + 5, 70 * 9 * 39 + 69]),
+                                       ((64, 8, 32), [64 * 8 * 31 + 64 * 7 + 63])])
+def test_cell_laplacian_reports_first_bad_d_anywhere(gc, ctx, shape, bad):
+    """The D > 0 check runs inside the operator kernel: the smallest failing index is found
+    wherever it lies (tile edges, last z-chunk, the very last site); NaN fails too."""
+    g = gc.Grid.make(*shape)
+    D = np.ones(g.shape)
+    for i, b in enumerate(bad):
+        D.ravel()[b] = np.nan if i == 0 else -2.0
+    psi = gc.DeviceArray.from_numpy(ctx, np.zeros(g.shape))
+    with pytest.raises(gc.GridcalcError) as e:
+        gc.cell_laplacian(ctx, g, psi, gc.DeviceArray.from_numpy(ctx, D))
+    assert e.value.code == gc.ERR_NONPOS_D
+    assert f"flat index {min(bad)}" in str(e.value)
+
+
 @pytest.mark.parametrize("axis,shape", [("x", (4, 16, 16)), ("y", (16, 3, 16)), ("z", (16, 16, 2))])
 def test_extent_rejected(gc, ctx, axis, shape):
     # stencil_axis_extent_test.cpp: 2 <= N <= 2r throws, message names the axis
