@@ -1262,11 +1262,13 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
 // one plane of each quantity, for the in-plane neighbours only.  A Laplacian then reads 4 LDS
 // values (x and y neighbours) instead of 7 and takes its centre and z neighbours from the
 // queue; the LDS addresses are loop invariant and each plane is staged once.  Iteration p
-// loads plane P_p = z0 - 4 + p and runs four phases separated by barriers:
-//   A: mu_A on P_{p-1} (columns at least 1 from the region edge)     [stores mu_B(P_{p-4})]
-//   B: k_A and y' on P_{p-2} (at least 2)                            [stores y(P_p)]
-//   C: mu_B on P_{p-3} (at least 3)                                  [stores mu_A(P_{p-1})]
-//   D: the outputs on P_{p-4} (the tile)                             [stores y'(P_{p-2})]
+// loads plane P_p = z0 - 4 + p and runs two phases separated by barriers, each pairing two
+// independent pieces of work (a plane later for mu_B and the outputs, so that every input a
+// phase needs was formed in an earlier phase):
+//   X: mu_A on P_{p-1} (columns at least 1 from the region edge), mu_B on P_{p-4} (at
+//      least 3)                               [stores mu_A(P_{p-2}), mu_B(P_{p-5})]
+//   Y: k_A and y' on P_{p-2} (at least 2), the outputs on P_{p-5} (the tile)
+//                                             [stores y(P_p), y'(P_{p-3})]
 // Each LDS plane is written in the phase after its last reader and read after the next
 // barrier, so one slot per quantity suffices.  The arithmetic (operands, order) is that of
 // k_ch_pair3d.
@@ -1337,7 +1339,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
     for (int s = 0; s < L; ++s)
       if (dist[s] >= 0) yn[s] = yp[goff[s]];
   };
-  const int last = nout + 7;
+  const int last = nout + 8;
   auto iter = [&](auto rc, int p) {
     constexpr int R0 = decltype(rc)::value, R1 = (R0 + 2) % 3, R2 = (R0 + 1) % 3;
 #pragma unroll
@@ -1351,8 +1353,8 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           if (dist[s] >= 2) pe[s] = pp[goff[s]];
       }
     }
-    if (p >= 8) {
-      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+    if (p >= 9) {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 9 + p);
 #pragma unroll
       for (int s = 0; s < L; ++s) {
         if (oin[s]) {
@@ -1364,26 +1366,50 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
         }
       }
     }
-    // A: mu_A on P_{p-1}; store mu_B(P_{p-4}) (formed in phase C of iteration p-1)
+    // X: mu_A on P_{p-1}, mu_B on P_{p-4}; store mu_A(P_{p-2}) and mu_B(P_{p-5}) (formed in
+    // phase X of iteration p-1)
 #pragma unroll
     for (int s = 0; s < L; ++s) {
       const int o = t + s * NT;
-      if (p >= 7 && dist[s] >= 3) bs[o] = bq[R1][s];
+      if (p >= 3 && dist[s] >= 1) ms[o] = mq[R1][s];
+      if (p >= 8 && dist[s] >= 3) bs[o] = bq[R1][s];
       if (p >= 2 && dist[s] >= 1) {
         const T yy = yq[R1][s];
         const T l = lap_q(ys, o, W, yy, yq[R2][s], yq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2);
         mq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
       }
+      if (p >= 7 && dist[s] >= 3) {  // y' on P_{p-3}, P_{p-4}, P_{p-5}: slots R1, R2, R0
+        const T yy = wq[R2][s];
+        const T l = lap_q(ws, o, W, yy, wq[R0][s], wq[R1][s], cf.ihx2, cf.ihy2, cf.ihz2);
+        bq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
+      }
     }
     __syncthreads();
-    // B: k_A and y' on P_{p-2}; store y(P_p)
+    // Y: the outputs on P_{p-5}, k_A and y' on P_{p-2}; store y(P_p) and y'(P_{p-3}) (formed
+    // in phase Y of iteration p-1)
     {
+      const size_t kb = (size_t)d.plane * (size_t)(z0 - 9 + p);
       const int zA = z0 - 6 + p;
       const bool own = zA >= z0 && zA < z0 + nout;
 #pragma unroll
       for (int s = 0; s < L; ++s) {
         const int o = t + s * NT;
         if (dist[s] >= 0) ys[o] = yq[R0][s];
+        if (p >= 5 && dist[s] >= 2) ws[o] = wq[R1][s];
+        if (p >= 9 && oin[s]) {  // mu_B on P_{p-4}, P_{p-5}, P_{p-6}: slots R0, R1, R2
+          const T kv =
+              lap_q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          if constexpr (PAIR == 0) {
+            __builtin_nontemporal_store(kv, b + kb + goff[s]);
+            __builtin_nontemporal_store(pv[s] + co.half_dt * kv, out + kb + goff[s]);
+          } else {
+            // k3 of P_{p-5}: phase Y of iteration p-3, slot R0 (read before it is refilled
+            // below)
+            const T bsum = bv[s] + kq[R0][s];
+            __builtin_nontemporal_store(pv[s] + (co.sixth_dt * (av[s] + kv) + co.third_dt * bsum),
+                                        out + kb + goff[s]);
+          }
+        }
         if (p >= 4 && dist[s] >= 2) {
           const T kv =
               lap_q(ms, o, W, mq[R1][s], mq[R2][s], mq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
@@ -1394,40 +1420,6 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           } else {
             wq[R0][s] = pe[s] + co.dt * kv;
             kq[R0][s] = kv;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // C: mu_B on P_{p-3}; store mu_A(P_{p-1})
-#pragma unroll
-    for (int s = 0; s < L; ++s) {
-      const int o = t + s * NT;
-      if (p >= 2 && dist[s] >= 1) ms[o] = mq[R0][s];
-      if (p >= 6 && dist[s] >= 3) {
-        const T yy = wq[R1][s];
-        const T l = lap_q(ws, o, W, yy, wq[R2][s], wq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2);
-        bq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
-      }
-    }
-    __syncthreads();
-    // D: outputs on P_{p-4}; store y'(P_{p-2})
-    {
-      const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
-#pragma unroll
-      for (int s = 0; s < L; ++s) {
-        const int o = t + s * NT;
-        if (p >= 4 && dist[s] >= 2) ws[o] = wq[R0][s];
-        if (p >= 8 && oin[s]) {
-          const T kv =
-              lap_q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
-          if constexpr (PAIR == 0) {
-            __builtin_nontemporal_store(kv, b + kb + goff[s]);
-            __builtin_nontemporal_store(pv[s] + co.half_dt * kv, out + kb + goff[s]);
-          } else {
-            const T bsum = bv[s] + kq[R2][s];  // k3 of this plane (phase B of iteration p-2)
-            __builtin_nontemporal_store(pv[s] + (co.sixth_dt * (av[s] + kv) + co.third_dt * bsum),
-                                        out + kb + goff[s]);
           }
         }
       }
