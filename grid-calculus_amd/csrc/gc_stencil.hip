@@ -380,6 +380,122 @@ void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, co
                      dim3(256), 0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen, bad);
 }
 
+// ---- axis stencils: z in register queues (used for the order-4 Laplacian) --------------
+//
+// Operators that read only along the axes through a site take their z neighbours from a
+// per-column register queue of the 2R+1 planes around the output plane; LDS holds one plane
+// with an x/y halo of R for the in-plane neighbours.  The tile sites of that plane come from
+// the owners' queues (no second load), the halo sites from a prefetch.  Per site 4R LDS
+// reads instead of 6R+1 from a (2R+1)-plane ring, and one LDS plane instead of 2R+1 (more
+// workgroups per CU).  The operator code and its accumulation order are the ring engine's.
+template <class T, int R, int PW>
+struct AxisAt {
+  const T* pl;  // the plane, tile + R halo, row width PW
+  int base;     // (oy + R) * PW + (ox + R)
+  const T* q;   // the column over z - R .. z + R
+  __device__ __forceinline__ T operator()(int, int dx, int dy, int dz) const {
+    if (dz != 0 || (dx == 0 && dy == 0)) return q[dz + R];  // axis stencils: dx = dy = 0 here
+    return pl[base + dy * PW + dx];
+  }
+};
+
+template <class T, int OX, int OY, class Op>
+__global__ __launch_bounds__(256) void k_axis_zq(Op op, const T* __restrict__ f,
+                                                 T* __restrict__ out, Dims d, int ntx, int nty,
+                                                 int zlen) {
+  constexpr int R = Op::R, NT = 256, PW = OX + 2 * R, PH = OY + 2 * R, PN = PW * PH;
+  constexpr int ROWS = NT / OX, OR = OX * OY / NT, PL = (PN + NT - 1) / NT;
+  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
+  __shared__ T pl[PN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+  const int ox = t % OX, oy0 = t / OX;
+
+  // halo sites of the plane this thread stages (tile sites come from the owners' queues)
+  unsigned hoff[PL];
+  int hslot[PL];
+#pragma unroll
+  for (int s = 0; s < PL; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % PW, py = idx / PW;
+    const bool tile = px >= R && px < R + OX && py >= R && py < R + OY;
+    hslot[s] = (idx < PN && !tile) ? idx : -1;
+    hoff[s] = hslot[s] >= 0
+                  ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
+                  : 0u;
+  }
+  unsigned coff[OR];
+  bool cin[OR];
+#pragma unroll
+  for (int q = 0; q < OR; ++q) {
+    const int gy = y0 + oy0 + q * ROWS;
+    cin[q] = x0 + ox < d.nx && gy < d.ny;
+    coff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
+  }
+  auto plane_ptr = [&](int z) { return f + (size_t)d.plane * (size_t)wrap_any(z, d.nz); };
+  T qv[OR][2 * R + 1];  // own columns over z - R .. z + R
+#pragma unroll
+  for (int j = 0; j < 2 * R + 1; ++j) {
+    const T* fp = plane_ptr(z0 - R + j);
+#pragma unroll
+    for (int q = 0; q < OR; ++q) qv[q][j] = fp[coff[q]];
+  }
+  T hv[PL], qn[OR];
+  {
+    const T* fp = plane_ptr(z0);
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+      if (hslot[s] >= 0) hv[s] = fp[hoff[s]];
+  }
+  for (int k = 0; k < nout; ++k) {
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+      if (hslot[s] >= 0) pl[hslot[s]] = hv[s];
+#pragma unroll
+    for (int q = 0; q < OR; ++q) pl[(oy0 + q * ROWS + R) * PW + ox + R] = qv[q][R];
+    __syncthreads();
+    if (k + 1 < nout) {
+      const T* fp = plane_ptr(z0 + k + 1);
+#pragma unroll
+      for (int s = 0; s < PL; ++s)
+        if (hslot[s] >= 0) hv[s] = fp[hoff[s]];
+      const T* fq = plane_ptr(z0 + k + 1 + R);
+#pragma unroll
+      for (int q = 0; q < OR; ++q) qn[q] = fq[coff[q]];
+    }
+    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      const int oy = oy0 + q * ROWS;
+      if (cin[q]) {
+        const AxisAt<T, R, PW> F{pl, (oy + R) * PW + (ox + R), qv[q]};
+        op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+#pragma unroll
+      for (int j = 0; j < 2 * R; ++j) qv[q][j] = qv[q][j + 1];
+      qv[q][2 * R] = qn[q];
+    }
+  }
+}
+
+template <class T, class Op>
+void axis_launch(gc_context* ctx, const Dims& d, const Op& op, const T* in, T* out) {
+  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + kZOY - 1) / kZOY;
+  const int nzc = (d.nz + kZLen - 1) / kZLen;
+  hipLaunchKernelGGL((k_axis_zq<T, kZOX, kZOY, Op>), dim3((unsigned)(ntx * nty * nzc)),
+                     dim3(256), 0, ctx->stream, op, in, out, d, ntx, nty, kZLen);
+}
+
 // ---- biharmonic: ring + register queues --------------------------------------------------
 //
 // The biharmonic reads 21 axis values and 75 mixed values per site.  On the ring engine every
@@ -778,10 +894,12 @@ template <class T>
 static gc_status run_laplacian(gc_context* ctx, const gc_grid* g, int order, const T* in, T* out) {
   const Dims d = dims_of(g);
   const Inv<double> ih = inv_h2(g);
+  // order 4 reads 4 z neighbours: the register queue pays (0.49 against 0.55 ms FP64 on
+  // 512^3); order 2 runs as fast on the plane ring
   if (order == 2)
     zmarch_launch<T>(ctx, d, OpLap<T, 2>{T(ih.x), T(ih.y), T(ih.z)}, in, nullptr, out);
   else
-    zmarch_launch<T>(ctx, d, OpLap<T, 4>{T(ih.x), T(ih.y), T(ih.z)}, in, nullptr, out);
+    axis_launch<T>(ctx, d, OpLap<T, 4>{T(ih.x), T(ih.y), T(ih.z)}, in, out);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -817,6 +935,7 @@ void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void
   const T x = T(ih.x), y = T(ih.y), z = T(ih.z);
   switch (which) {
     case 0:
+      // the plane ring: the register-queue engine measured 7 % slower for the gradient
       zmarch_launch<T>(ctx, d, OpGrad<T, O>{x, y, z}, (const T*)in, nullptr, (T*)out);
       break;
     case 1:
