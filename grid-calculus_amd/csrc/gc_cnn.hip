@@ -176,6 +176,109 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
   }
 }
 
+// ---- compensated reductions shared by the learned-integration kernels ------------------
+
+// Sum of the per-tile partial sums part[0..nparts): wave 0 of the block, lane l summing
+// part[l], part[l + 64], ... in order, then a fixed xor tree (every block gets the same bits);
+// the result is stored to *dst (LDS) before the barrier.
+__device__ __forceinline__ void block_sum_parts(const double* __restrict__ part, int nparts,
+                                                double* dst) {
+  if (threadIdx.x < 64) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 64) v += part[i];
+    for (int st = 32; st >= 1; st >>= 1) v += __shfl_xor(v, st);
+    if (threadIdx.x == 0) *dst = v;
+  }
+  __syncthreads();
+}
+
+struct Comp2 {
+  double ms, mc, fs, fc;
+};
+
+__device__ __forceinline__ void neu(double& s, double& c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+// One wave's xor-tree merge of per-lane partials (sums with TwoSum, compensations added);
+// every lane ends with the wave total.
+__device__ __forceinline__ Comp2 wave_merge(Comp2 v) {
+  for (int st = 1; st < 64; st <<= 1) {
+    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
+    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
+    double s1 = lo.ms, c1 = lo.mc + hi.mc;
+    neu(s1, c1, hi.ms);
+    double s2 = lo.fs, c2 = lo.fc + hi.fc;
+    neu(s2, c2, hi.fs);
+    v = Comp2{s1, c1, s2, c2};
+  }
+  return v;
+}
+
+// The first nw waves' totals (sh[w], written by lane 0 of each) merged in wave order; for
+// thread 0.
+__device__ __forceinline__ Comp2 waves_merge(const Comp2* sh, int nw) {
+  Comp2 r = sh[0];
+  for (int w = 1; w < nw; ++w) {
+    double c1 = r.mc + sh[w].mc, c2 = r.fc + sh[w].fc;
+    neu(r.ms, c1, sh[w].ms);
+    neu(r.fs, c2, sh[w].fs);
+    r.mc = c1;
+    r.fc = c2;
+  }
+  return r;
+}
+
+// Mass and Cahn-Hilliard density (examples/common/cahn_hilliard.hpp:65-67) of a site of a 2-D
+// periodic grid with value p and neighbours xm, ym, xp, yp (x - 1, y - 1, x + 1, y + 1),
+// accumulated (Neumaier) into ms/mc and fs/fc.
+__device__ __forceinline__ void site_mass_fe5(double p, double xm, double ym, double xp,
+                                              double yp, double ihx, double ihy, double kappa,
+                                              double& ms, double& mc, double& fs, double& fc) {
+  double dx = 0.0, dy = 0.0;
+  dx += -0.5 * xm;
+  dy += -0.5 * ym;
+  dx += 0.0 * p;
+  dy += 0.0 * p;
+  dx += 0.5 * xp;
+  dy += 0.5 * yp;
+  const double gx = dx * ihx, gy = dy * ihy, gz = 0.0;
+  const double sq = gx * gx + gy * gy + gz * gz;
+  neu(ms, mc, p);
+  neu(fs, fc, -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * sq);
+}
+
+// The same for site (i, j) whose values come from val(flat index).
+template <class V>
+__device__ __forceinline__ void site_mass_fe(const V& val, int i, int j, int nx, int ny,
+                                             double ihx, double ihy, double kappa, double& ms,
+                                             double& mc, double& fs, double& fc) {
+  site_mass_fe5(val((size_t)j * nx + i), val((size_t)j * nx + wrap1(i - 1, nx)),
+                val((size_t)wrap1(j - 1, ny) * nx + i), val((size_t)j * nx + wrap1(i + 1, nx)),
+                val((size_t)wrap1(j + 1, ny) * nx + i), ihx, ihy, kappa, ms, mc, fs, fc);
+}
+
+// One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
+// a fixed xor tree; lane 0 writes the two results.
+__device__ __forceinline__ void mass_fe_reduce(const Comp2* __restrict__ part, int nb,
+                                               double* __restrict__ out) {
+  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    mc += part[b].mc;
+    fc += part[b].fc;
+    neu(ms, mc, part[b].ms);
+    neu(fs, fc, part[b].fs);
+  }
+  const Comp2 v = wave_merge(Comp2{ms, mc, fs, fc});
+  if (threadIdx.x == 0) {
+    out[0] = v.ms + v.mc;
+    out[1] = v.fs + v.fc;
+  }
+}
+
 // ---- whole-network inference in one kernel -------------------------------------------------
 // A block owns an 8 x 32 output tile and evaluates all six layers on it, each on the tile
 // plus the halo the remaining layers need (layer l on a (6-l)-site halo): the state region
@@ -192,7 +295,23 @@ constexpr int kFNB = (kFR + 8) * (kFC + 8);               // layer-2 region
 
 constexpr int kFNT = 512;  // threads of the fused kernel (8 waves)
 
+// Chained mode (learned integration, one image): the state the launch advances is formed on
+// the fly from the previous macro step, c_k = c_{k-1} + scale (out_{k-1} - mean(out_{k-1}))
+// (k_apply's expression, so the same values); the block writes it for its own tile, writes
+// its tile's mass and free-energy partial sums (merged by a later kernel) and evaluates the
+// network on it.  One launch per macro step instead of two.
 template <class TS>
+struct Chain {
+  const TS* cprev;       // state of the previous macro step
+  const float* oprev;    // the network's output for it
+  const double* pprev;   // its per-tile sums
+  int nparts;
+  double scale, ihx, ihy, kappa;
+  TS* cdst;              // this macro step's state (the block's tile)
+  Comp2* mpart;          // per-tile (mass, density) partial sums
+};
+
+template <class TS, bool APPLY>
 __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
                                                    const float* __restrict__ w1,
                                                    const float* __restrict__ b1,
@@ -202,8 +321,9 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
                                                    const float* __restrict__ b6,
                                                    float* __restrict__ out,
                                                    double* __restrict__ part, int H, int W,
-                                                   int tiles_x, int tiles_y) {
+                                                   int tiles_x, int tiles_y, Chain<TS> ch) {
   __shared__ float cs[kFH0 * kFW0];
+  __shared__ Comp2 msh[kFR * kFC / 64];
   __shared__ double psum[kFR * kFC / 64];
   __shared__ float w1s[288 + 32], w6s[288];
   __shared__ __attribute__((aligned(16))) __bf16 bufA[kFNA * kSP];
@@ -213,14 +333,73 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
   const int r1 = blockIdx.x / tiles_x;
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
-  const TS* img = c + (size_t)b * H * W;
-  for (int q = t; q < kFH0 * kFW0; q += kFNT) {
-    const int ly = q / kFW0, lx = q % kFW0;
-    cs[q] = (float)img[(size_t)wrap1(y0 + ly - 6, H) * W + wrap1(x0 + lx - 6, W)];
+  if constexpr (APPLY) {
+    // Every load is issued before the first use: the state region, the tile's five-point
+    // neighbourhoods and the previous tile sums.
+    constexpr int kNH = (kFH0 * kFW0 + kFNT - 1) / kFNT;
+    double hc[kNH];
+    float ho[kNH];
+#pragma unroll
+    for (int r = 0; r < kNH; ++r) {
+      const int q = t + r * kFNT;
+      if (q < kFH0 * kFW0) {
+        const size_t g = (size_t)wrap1(y0 + q / kFW0 - 6, H) * W + wrap1(x0 + q % kFW0 - 6, W);
+        hc[r] = (double)ch.cprev[g];
+        ho[r] = ch.oprev[g];
+      }
+    }
+    const bool own = t < kFR * kFC;
+    const int i = x0 + t % kFC, j = y0 + t / kFC;
+    size_t g5[5];
+    double c5[5];
+    float o5[5];
+    if (own) {
+      g5[0] = (size_t)j * W + i;
+      g5[1] = (size_t)j * W + wrap1(i - 1, W);
+      g5[2] = (size_t)wrap1(j - 1, H) * W + i;
+      g5[3] = (size_t)j * W + wrap1(i + 1, W);
+      g5[4] = (size_t)wrap1(j + 1, H) * W + i;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        c5[k] = (double)ch.cprev[g5[k]];
+        o5[k] = ch.oprev[g5[k]];
+      }
+    }
+    // mean of the previous output, in every wave: lane 0's value of the lane-strided sums
+    // and xor tree of block_sum_parts, so the value k_apply uses
+    double sv = 0.0;
+    for (int k = lane; k < ch.nparts; k += 64) sv += ch.pprev[k];
+    for (int st = 32; st >= 1; st >>= 1) sv += __shfl_xor(sv, st);
+    const double mean = __shfl(sv, 0) / (double)((size_t)H * W);
+    auto nv = [&](double cp, float op) { return (TS)(cp + ch.scale * ((double)op - mean)); };
+#pragma unroll
+    for (int r = 0; r < kNH; ++r) {
+      const int q = t + r * kFNT;
+      if (q < kFH0 * kFW0) cs[q] = (float)nv(hc[r], ho[r]);
+    }
+    if (own) {  // the tile: write the new state, reduce its mass and density
+      TS x[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) x[k] = nv(c5[k], o5[k]);
+      ch.cdst[g5[0]] = x[0];
+      double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+      site_mass_fe5((double)x[0], (double)x[1], (double)x[2], (double)x[3], (double)x[4],
+                    ch.ihx, ch.ihy, ch.kappa, ms, mc, fs, fc);
+      const Comp2 v = wave_merge(Comp2{ms, mc, fs, fc});
+      if (lane == 0) msh[wave] = v;
+    }
+  } else {
+    const TS* img = c + (size_t)b * H * W;
+    for (int q = t; q < kFH0 * kFW0; q += kFNT) {
+      const int ly = q / kFW0, lx = q % kFW0;
+      cs[q] = (float)img[(size_t)wrap1(y0 + ly - 6, H) * W + wrap1(x0 + lx - 6, W)];
+    }
   }
   for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
   for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
   __syncthreads();
+  if constexpr (APPLY)
+    if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
   // layer 1 on the halo-5 region (kFR + 10) x (kFC + 10)
   {
     constexpr int RW = kFC + 10, RN = kFNA;
@@ -586,19 +765,6 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf1
   }
 }
 
-// Sum of the per-tile partial sums part[0..nparts): wave 0 of the block, lane l summing
-// part[l], part[l + 64], ... in order, then a fixed xor tree (every block gets the same bits);
-// the result is stored to *dst (LDS) before the barrier.
-__device__ __forceinline__ void block_sum_parts(const double* __restrict__ part, int nparts,
-                                                double* dst) {
-  if (threadIdx.x < 64) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 64) v += part[i];
-    for (int st = 32; st >= 1; st >>= 1) v += __shfl_xor(v, st);
-    if (threadIdx.x == 0) *dst = v;
-  }
-  __syncthreads();
-}
 
 // Residual update with the mean of the increment removed (mass conservation):
 // c_new = c + scale * (out - sum(out)/N).
@@ -642,6 +808,8 @@ struct gc_cnn {
   unsigned* done = nullptr;  // learned integration: blocks finished (last-block reduction)
   void* hist = nullptr;     // learned integration: proposals of one chunk
   double* diag = nullptr;   // learned integration: (mass, free energy) per proposal
+  float* out2 = nullptr;    // learned integration: second network output (chained steps)
+  double* cpart = nullptr;  // learned integration: second tile sums, then reduction partials
   int adam_t = 0;
   bool packed = false;
 };
@@ -707,12 +875,27 @@ gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
 
 // Inference: the whole network in one kernel per 8 x 32 tile (k_fwd_fused); output to n->out.
 template <class TS>
-gc_status forward_fused(gc_cnn* n, const TS* c, int B) {
+gc_status forward_fused(gc_cnn* n, const TS* c, int B, float* out = nullptr,
+                        double* part = nullptr) {
   if (!n->packed) GC_TRY(pack(n));
   const int tx = n->W / kTC, ty = n->H / kTR;
-  hipLaunchKernelGGL((k_fwd_fused<TS>), dim3((unsigned)(tx * ty * B)), dim3(kFNT), 0,
+  hipLaunchKernelGGL((k_fwd_fused<TS, false>), dim3((unsigned)(tx * ty * B)), dim3(kFNT), 0,
                      n->ctx->stream, c, n->p + kOffW1, n->p + kOffB1, n->wmid, n->p,
-                     n->p + kOffW6, n->p + kOffB6, n->out, n->dpart, n->H, n->W, tx, ty);
+                     n->p + kOffW6, n->p + kOffB6, out ? out : n->out, part ? part : n->dpart,
+                     n->H, n->W, tx, ty, Chain<TS>{});
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// One chained macro step on one image (see Chain): the network's output to out, its tile
+// sums to part.
+template <class TS>
+gc_status forward_chain(gc_cnn* n, const Chain<TS>& ch, float* out, double* part) {
+  if (!n->packed) GC_TRY(pack(n));
+  const int tx = n->W / kTC, ty = n->H / kTR;
+  hipLaunchKernelGGL((k_fwd_fused<TS, true>), dim3((unsigned)(tx * ty)), dim3(kFNT), 0,
+                     n->ctx->stream, (const TS*)nullptr, n->p + kOffW1, n->p + kOffB1, n->wmid,
+                     n->p, n->p + kOffW6, n->p + kOffB6, out, part, n->H, n->W, tx, ty, ch);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -840,7 +1023,8 @@ gc_status gc_cnn_destroy(gc_cnn* n) {
   if (!n) return GC_OK;
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->wmid, n->wmidT, n->dzA, n->dzB,
-                  n->out, n->dout, n->part, n->dpart, n->idx, n->done, n->hist, n->diag};
+                  n->out, n->dout, n->part, n->dpart, n->idx, n->done, n->hist, n->diag,
+                  n->out2, n->cpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < 5; ++l) {
@@ -996,16 +1180,6 @@ gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const v
 
 namespace {
 
-struct Comp2 {
-  double ms, mc, fs, fc;
-};
-
-__device__ __forceinline__ void neu(double& s, double& c, double x) {
-  const double t = s + x;
-  if (fabs(s) >= fabs(x)) c += (s - t) + x;
-  else c += (x - t) + s;
-  s = t;
-}
 
 // Sum of c and of the Cahn-Hilliard density (examples/common/cahn_hilliard.hpp:65-67) over a
 // 2-D periodic grid, compensated, in one pass.
@@ -1015,75 +1189,16 @@ __global__ __launch_bounds__(256) void k_mass_fe(const T* __restrict__ c, int nx
                                                  Comp2* __restrict__ part) {
   __shared__ Comp2 sh[4];
   const long n = (long)nx * ny;
+  auto val = [&](size_t q) { return (double)c[q]; };
   double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
-  for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256) {
-    const int i = (int)(s % nx), j = (int)(s / nx);
-    const double p = (double)c[s];
-    double dx = 0.0, dy = 0.0;
-    dx += -0.5 * (double)c[(size_t)j * nx + wrap1(i - 1, nx)];
-    dy += -0.5 * (double)c[(size_t)wrap1(j - 1, ny) * nx + i];
-    dx += 0.0 * p;
-    dy += 0.0 * p;
-    dx += 0.5 * (double)c[(size_t)j * nx + wrap1(i + 1, nx)];
-    dy += 0.5 * (double)c[(size_t)wrap1(j + 1, ny) * nx + i];
-    const double gx = dx * ihx, gy = dy * ihy, gz = 0.0;
-    const double sq = gx * gx + gy * gy + gz * gz;
-    neu(ms, mc, p);
-    neu(fs, fc, -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * sq);
-  }
-  Comp2 v{ms, mc, fs, fc};
-  for (int st = 1; st < 64; st <<= 1) {
-    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
-    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
-    double s1 = lo.ms, c1 = lo.mc + hi.mc;
-    neu(s1, c1, hi.ms);
-    double s2 = lo.fs, c2 = lo.fc + hi.fc;
-    neu(s2, c2, hi.fs);
-    v = Comp2{s1, c1, s2, c2};
-  }
+  for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256)
+    site_mass_fe(val, (int)(s % nx), (int)(s / nx), nx, ny, ihx, ihy, kappa, ms, mc, fs, fc);
+  const Comp2 v = wave_merge(Comp2{ms, mc, fs, fc});
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    Comp2 r = sh[0];
-    for (int w = 1; w < 4; ++w) {
-      double c1 = r.mc + sh[w].mc, c2 = r.fc + sh[w].fc;
-      neu(r.ms, c1, sh[w].ms);
-      neu(r.fs, c2, sh[w].fs);
-      r.mc = c1;
-      r.fc = c2;
-    }
-    part[blockIdx.x] = r;
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = waves_merge(sh, 4);
 }
 
-// One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
-// a fixed xor tree (sums with TwoSum, compensations added).
-// One wave: lane l merges the block partials l, l + 64, ... in order, then the lanes merge in
-// a fixed xor tree (sums with TwoSum, compensations added); lane 0 writes the two results.
-__device__ __forceinline__ void mass_fe_reduce(const Comp2* __restrict__ part, int nb,
-                                               double* __restrict__ out) {
-  double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 64) {
-    mc += part[b].mc;
-    fc += part[b].fc;
-    neu(ms, mc, part[b].ms);
-    neu(fs, fc, part[b].fs);
-  }
-  Comp2 v{ms, mc, fs, fc};
-  for (int st = 1; st < 64; st <<= 1) {
-    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
-    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
-    double s1 = lo.ms, c1 = lo.mc + hi.mc;
-    neu(s1, c1, hi.ms);
-    double s2 = lo.fs, c2 = lo.fc + hi.fc;
-    neu(s2, c2, hi.fs);
-    v = Comp2{s1, c1, s2, c2};
-  }
-  if (threadIdx.x == 0) {
-    out[0] = v.ms + v.mc;
-    out[1] = v.fs + v.fc;
-  }
-}
 
 // The residual update of k_apply fused with k_mass_fe: each thread forms the new state at its
 // site and at the four neighbours (the same expression as k_apply, so the same values),
@@ -1104,49 +1219,23 @@ __global__ __launch_bounds__(256) void k_apply_mass_fe(const T* __restrict__ c, 
   const long n = (long)nx * ny;
   const double mean = sum / (double)n;
   auto nv = [&](size_t q) { return (T)((double)c[q] + scale * ((double)out[q] - mean)); };
+  auto val = [&](size_t q) { return (double)nv(q); };
   double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
   for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < n; s += (long)gridDim.x * 256) {
-    const int i = (int)(s % nx), j = (int)(s / nx);
-    const T pT = nv((size_t)s);
-    cn[s] = pT;
-    const double p = (double)pT;
-    double dx = 0.0, dy = 0.0;
-    dx += -0.5 * (double)nv((size_t)j * nx + wrap1(i - 1, nx));
-    dy += -0.5 * (double)nv((size_t)wrap1(j - 1, ny) * nx + i);
-    dx += 0.0 * p;
-    dy += 0.0 * p;
-    dx += 0.5 * (double)nv((size_t)j * nx + wrap1(i + 1, nx));
-    dy += 0.5 * (double)nv((size_t)wrap1(j + 1, ny) * nx + i);
-    const double gx = dx * ihx, gy = dy * ihy, gz = 0.0;
-    const double sq = gx * gx + gy * gy + gz * gz;
-    neu(ms, mc, p);
-    neu(fs, fc, -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * sq);
+    cn[s] = nv((size_t)s);
+    site_mass_fe(val, (int)(s % nx), (int)(s / nx), nx, ny, ihx, ihy, kappa, ms, mc, fs, fc);
   }
-  Comp2 v{ms, mc, fs, fc};
-  for (int st = 1; st < 64; st <<= 1) {
-    Comp2 o{__shfl_xor(v.ms, st), __shfl_xor(v.mc, st), __shfl_xor(v.fs, st), __shfl_xor(v.fc, st)};
-    Comp2 lo = (threadIdx.x & st) ? o : v, hi = (threadIdx.x & st) ? v : o;
-    double s1 = lo.ms, c1 = lo.mc + hi.mc;
-    neu(s1, c1, hi.ms);
-    double s2 = lo.fs, c2 = lo.fc + hi.fc;
-    neu(s2, c2, hi.fs);
-    v = Comp2{s1, c1, s2, c2};
-  }
+  const Comp2 v = wave_merge(Comp2{ms, mc, fs, fc});
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    Comp2 r = sh[0];
-    for (int w = 1; w < 4; ++w) {
-      double c1 = r.mc + sh[w].mc, c2 = r.fc + sh[w].fc;
-      neu(r.ms, c1, sh[w].ms);
-      neu(r.fs, c2, sh[w].fs);
-      r.mc = c1;
-      r.fc = c2;
+    part[blockIdx.x] = waves_merge(sh, 4);
+    if (done) {
+      __threadfence();
+      last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
-    part[blockIdx.x] = r;
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
   }
+  if (!done) return;  // the partials are merged by a later kernel
   __syncthreads();
   if (last) {  // every partial is written: reduce them here instead of in another launch
     __threadfence();
@@ -1158,6 +1247,15 @@ __global__ __launch_bounds__(256) void k_apply_mass_fe(const T* __restrict__ c, 
 __global__ __launch_bounds__(64) void k_mass_fe_final(const Comp2* __restrict__ part, int nb,
                                                       double* __restrict__ out) {
   mass_fe_reduce(part, nb, out);
+}
+
+// Block k - 1 merges the partials of proposal k = 1..K (slot k of parts, stride partials per
+// slot; nb partials, nb_last for proposal K) into out[2k..2k+1].
+__global__ __launch_bounds__(64) void k_mass_fe_slots(const Comp2* __restrict__ parts,
+                                                      int stride, int nb, int nb_last, int K,
+                                                      double* __restrict__ out) {
+  const int k = blockIdx.x + 1;
+  mass_fe_reduce(parts + (size_t)k * stride, k == K ? nb_last : nb, out + 2 * k);
 }
 
 constexpr int kMFBlocks = 128;
@@ -1180,23 +1278,39 @@ gc_status mass_fe_async(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c,
   return GC_OK;
 }
 
-// One learned proposal with its diagnostics: c_new = c + scale*(out - mean(out)), and
-// (sum of c_new, sum of its density) into out_dev[0..1].
-gc_status predict_diag(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c, void* c_new,
-                       double kappa, double* out_dev) {
+// Learned proposals of one chunk: slot k of hist receives proposal k (k = 1..K) of the chain
+// starting at c, c_k = c_{k-1} + scale*(out(c_{k-1}) - mean), and diag[2k..2k+1] its (sum of
+// c_k, sum of its density).  Launch 1 evaluates the network on c; launch k = 2..K forms
+// c_{k-1} from the previous launch's output, writes it, reduces its diagnostics and evaluates
+// the network on it (the two network outputs and tile sums alternate); a last launch forms
+// c_K with its diagnostics.  K + 1 launches for K proposals.
+template <class TS>
+gc_status propose_chunk(gc_cnn* n, const gc_grid* g, const TS* c, TS* hist, int K,
+                        double kappa, double* diag) {
   hipStream_t st = n->ctx->stream;
-  if (dt == GC_F64) GC_TRY(forward_fused<double>(n, (const double*)c, 1));
-  else GC_TRY(forward_fused<float>(n, (const float*)c, 1));
+  const size_t ns = sites(g);
   const int T = (n->W / kTC) * (n->H / kTR);
-  Comp2* part = (Comp2*)(n->dpart + T);  // after the tile sums the kernel reads
-  if (dt == GC_F64)
-    hipLaunchKernelGGL((k_apply_mass_fe<double>), dim3(kMFBlocks), dim3(256), 0, st,
-                       (const double*)c, (double*)c_new, n->out, n->dpart, T, n->scale, g->nx,
-                       g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part, n->done, out_dev);
-  else
-    hipLaunchKernelGGL((k_apply_mass_fe<float>), dim3(kMFBlocks), dim3(256), 0, st,
-                       (const float*)c, (float*)c_new, n->out, n->dpart, T, n->scale, g->nx,
-                       g->ny, 1.0 / g->hx, 1.0 / g->hy, kappa, part, n->done, out_dev);
+  float* outs[2] = {n->out, n->out2};
+  double* parts[2] = {n->dpart, n->cpart};
+  // diagnostics partials: slot k (S workgroup partials) for proposal k
+  const int S = std::max(T, kMFBlocks);
+  Comp2* red = (Comp2*)(n->cpart + T);
+  const double ihx = 1.0 / g->hx, ihy = 1.0 / g->hy;
+  auto state = [&](int k) { return k == 0 ? c : hist + (size_t)k * ns; };
+  GC_TRY(forward_fused<TS>(n, c, 1, outs[0], parts[0]));
+  for (int k = 2; k <= K; ++k) {
+    const int a = (k - 2) & 1, b = (k - 1) & 1;
+    Chain<TS> ch{state(k - 2), outs[a], parts[a], T, n->scale, ihx, ihy, kappa,
+                 hist + (size_t)(k - 1) * ns, red + (size_t)(k - 1) * S};
+    GC_TRY(forward_chain<TS>(n, ch, outs[b], parts[b]));
+  }
+  const int a = (K - 1) & 1;
+  hipLaunchKernelGGL((k_apply_mass_fe<TS>), dim3(kMFBlocks), dim3(256), 0, st, state(K - 1),
+                     hist + (size_t)K * ns, outs[a], parts[a], T, n->scale, g->nx, g->ny, ihx,
+                     ihy, kappa, red + (size_t)K * S, (unsigned*)nullptr, (double*)nullptr);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_mass_fe_slots, dim3((unsigned)K), dim3(64), 0, st, red, S, T, kMFBlocks,
+                     K, diag);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -1232,6 +1346,11 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
   const size_t bytes = sites(g) * dsize(dt);
   if (!n->hist) GC_TRY(cnn_alloc(&n->hist, (size_t)(kChunk + 1) * sites(g) * sizeof(double)));
   if (!n->diag) GC_TRY(cnn_alloc((void**)&n->diag, 2 * (kChunk + 1) * sizeof(double)));
+  const int T = (n->W / kTC) * (n->H / kTR);
+  if (!n->out2) GC_TRY(cnn_alloc((void**)&n->out2, sites(g) * sizeof(float)));
+  if (!n->cpart)  // second tile sums, then per proposal one Comp2 (4 doubles) per workgroup
+    GC_TRY(cnn_alloc((void**)&n->cpart,
+                     (T + 4 * (size_t)(kChunk + 1) * std::max(T, kMFBlocks)) * sizeof(double)));
   char* hist = (char*)n->hist;
   const double vol = (g->hx * g->hy) * g->hz;
   const double vol_total = (double)sites(g) * vol;
@@ -1244,12 +1363,10 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
   int m = 0;
   while (m < n_macro) {
     const int K = std::min(kChunk, n_macro - m);
-    const void* src = psi;
-    for (int k = 1; k <= K; ++k) {
-      void* dst = hist + (size_t)k * bytes;
-      GC_TRY(predict_diag(n, g, dt, src, dst, p->kappa, n->diag + 2 * k));
-      src = dst;
-    }
+    if (dt == GC_F64)
+      GC_TRY(propose_chunk<double>(n, g, (const double*)psi, (double*)hist, K, p->kappa, n->diag));
+    else
+      GC_TRY(propose_chunk<float>(n, g, (const float*)psi, (float*)hist, K, p->kappa, n->diag));
     GC_TRY(fetch(n, h + 2, n->diag + 2, 2 * K));
     int good = 0;
     for (int k = 1; k <= K; ++k) {
