@@ -809,6 +809,8 @@ struct gc_cnn {
   void* hist = nullptr;     // learned integration: proposals of one chunk
   double* diag = nullptr;   // learned integration: (mass, free energy) per proposal
   float* out2 = nullptr;    // learned integration: second network output (chained steps)
+  double* hdiag = nullptr;  // learned integration: pinned copies of two chunks' diagnostics
+  hipEvent_t ev[2] = {};    // learned integration: each chunk's diagnostics copied
   double* cpart = nullptr;  // learned integration: second tile sums, then reduction partials
   int adam_t = 0;
   bool packed = false;
@@ -1027,6 +1029,9 @@ gc_status gc_cnn_destroy(gc_cnn* n) {
                   n->out2, n->cpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (n->hdiag) (void)hipHostFree(n->hdiag);
+  for (hipEvent_t e : n->ev)
+    if (e) (void)hipEventDestroy(e);
   for (int l = 0; l < 5; ++l) {
     if (n->z[l]) (void)hipFree(n->z[l]);
     if (n->a[l]) (void)hipFree(n->a[l]);
@@ -1329,7 +1334,9 @@ extern "C" {
 // of a chunk is written to history slot k and its mass and free energy to diag[k].  One copy
 // of diag per chunk then lets the host apply the acceptance rule in order; the state after the
 // last accepted proposal becomes psi, and the first rejected macro step (if any) is recomputed
-// by plain RK4 before the next chunk starts.  The decisions, and so the result, are those of
+// by plain RK4 before the next chunk starts.  While the host decides on one chunk, the GPU
+// already proposes the next one from the chunk's last proposal (two proposal buffers); it is
+// discarded if the chunk has a rejection.  The decisions, and so the result, are those of
 // checking every proposal as it is made.
 gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
                                void* psi, double step, int substeps, int n_macro,
@@ -1344,50 +1351,85 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
   if (n_macro < 0 || substeps < 1 || !(step >= 0.0))
     return set_error(GC_ERR_INVALID, "gridcalc: bad learned-integration arguments");
   const size_t bytes = sites(g) * dsize(dt);
-  if (!n->hist) GC_TRY(cnn_alloc(&n->hist, (size_t)(kChunk + 1) * sites(g) * sizeof(double)));
-  if (!n->diag) GC_TRY(cnn_alloc((void**)&n->diag, 2 * (kChunk + 1) * sizeof(double)));
+  constexpr int kSlots = kChunk + 1;  // per proposal buffer: slot 0 unused, proposals 1..K
+  if (!n->hist) GC_TRY(cnn_alloc(&n->hist, 2 * (size_t)kSlots * sites(g) * sizeof(double)));
+  if (!n->diag) GC_TRY(cnn_alloc((void**)&n->diag, 2 * 2 * kSlots * sizeof(double)));
+  if (!n->hdiag &&
+      hipHostMalloc((void**)&n->hdiag, 2 * 2 * kSlots * sizeof(double), hipHostMallocDefault) !=
+          hipSuccess) {
+    n->hdiag = nullptr;
+    return set_error(GC_ERR_NOMEM, "gridcalc: pinned host buffer allocation failed");
+  }
+  for (hipEvent_t& e : n->ev)
+    if (!e) GC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const int T = (n->W / kTC) * (n->H / kTR);
   if (!n->out2) GC_TRY(cnn_alloc((void**)&n->out2, sites(g) * sizeof(float)));
   if (!n->cpart)  // second tile sums, then per proposal one Comp2 (4 doubles) per workgroup
     GC_TRY(cnn_alloc((void**)&n->cpart,
-                     (T + 4 * (size_t)(kChunk + 1) * std::max(T, kMFBlocks)) * sizeof(double)));
-  char* hist = (char*)n->hist;
+                     (T + 4 * (size_t)kSlots * std::max(T, kMFBlocks)) * sizeof(double)));
+  hipStream_t st = n->ctx->stream;
+  auto slot = [&](int buf, int k) { return (char*)n->hist + ((size_t)buf * kSlots + k) * bytes; };
+  double* hd[2] = {n->hdiag, n->hdiag + 2 * kSlots};
+  // Proposals 1..K from src into buffer buf; their diagnostics reach hd[buf] when ev[buf] fires.
+  auto launch = [&](const void* src, int buf, int K) -> gc_status {
+    double* dd = n->diag + (size_t)buf * 2 * kSlots;
+    if (dt == GC_F64)
+      GC_TRY(propose_chunk<double>(n, g, (const double*)src, (double*)slot(buf, 0), K, p->kappa,
+                                   dd));
+    else
+      GC_TRY(propose_chunk<float>(n, g, (const float*)src, (float*)slot(buf, 0), K, p->kappa,
+                                  dd));
+    GC_HIP(hipMemcpyAsync(hd[buf] + 2, dd + 2, 2 * K * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+    GC_HIP(hipEventRecord(n->ev[buf], st));
+    return GC_OK;
+  };
   const double vol = (g->hx * g->hy) * g->hz;
   const double vol_total = (double)sites(g) * vol;
-  double h[2 * (kChunk + 1)];
+  double h[2];
   GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
   GC_TRY(fetch(n, h, n->diag, 2));
   const double Minit = vol * h[0];
   double Fprev = vol * h[1];
   int fallbacks = 0;
-  int m = 0;
+  int m = 0;  // macro steps decided; psi holds the state after them
+  int cur = 0, K = std::min(kChunk, n_macro);
+  if (K > 0) GC_TRY(launch(psi, cur, K));
+  int ahead = m + K;  // macro steps covered by the chunks in flight
   while (m < n_macro) {
-    const int K = std::min(kChunk, n_macro - m);
-    if (dt == GC_F64)
-      GC_TRY(propose_chunk<double>(n, g, (const double*)psi, (double*)hist, K, p->kappa, n->diag));
-    else
-      GC_TRY(propose_chunk<float>(n, g, (const float*)psi, (float*)hist, K, p->kappa, n->diag));
-    GC_TRY(fetch(n, h + 2, n->diag + 2, 2 * K));
+    // While the host waits for chunk `cur`, the GPU proposes the next chunk from its last
+    // proposal; that chunk stands only if every proposal of `cur` is accepted.
+    const int Kn = std::min(kChunk, n_macro - ahead);
+    if (Kn > 0) GC_TRY(launch(slot(cur, K), 1 - cur, Kn));
+    GC_HIP(hipEventSynchronize(n->ev[cur]));
     int good = 0;
     for (int k = 1; k <= K; ++k) {
-      const double Mk = vol * h[2 * k], Fk = vol * h[2 * k + 1];
+      const double Mk = vol * hd[cur][2 * k], Fk = vol * hd[cur][2 * k + 1];
       if (!(std::isfinite(Fk) && Fk <= Fprev && std::fabs(Mk - Minit) / vol_total <= mass_tol))
         break;
       Fprev = Fk;
       good = k;
     }
-    if (good > 0)
-      GC_HIP(hipMemcpyAsync(psi, hist + (size_t)good * bytes, bytes, hipMemcpyDeviceToDevice,
-                            n->ctx->stream));
+    if (good > 0)  // ordered after the next chunk's reads of slot(cur, K), before its reuse
+      GC_HIP(hipMemcpyAsync(psi, slot(cur, good), bytes, hipMemcpyDeviceToDevice, st));
     m += good;
-    if (good < K) {
-      ++fallbacks;
-      GC_TRY(gc_ch_integrate(n->ctx, g, dt, p, psi, step, substeps, GC_RK4));
-      GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
-      GC_TRY(fetch(n, h, n->diag, 2));
-      Fprev = vol * h[1];
-      m += 1;
+    if (good == K) {
+      if (Kn <= 0) break;
+      cur = 1 - cur;
+      K = Kn;
+      ahead += Kn;
+      continue;
     }
+    // a rejected proposal: the chunk in flight (if any) started from a discarded state
+    ++fallbacks;
+    GC_TRY(gc_ch_integrate(n->ctx, g, dt, p, psi, step, substeps, GC_RK4));
+    GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
+    GC_TRY(fetch(n, h, n->diag, 2));
+    Fprev = vol * h[1];
+    m += 1;
+    K = std::min(kChunk, n_macro - m);
+    if (K > 0) GC_TRY(launch(psi, cur, K));
+    ahead = m + K;
   }
   GC_HIP(hipStreamSynchronize(n->ctx->stream));
   if (n_fallback) *n_fallback = fallbacks;
