@@ -47,6 +47,19 @@ def gelu(z):
     return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
 
 
+# The device evaluates GELU as z sigma(z P(z^2)), sigma(u) = 1 / (1 + exp(-u)), with this P
+# (least maximum error against gelu on |z| <= 9; z clamped to [-9, 9] inside P).
+GELU_FIT = (1.5950206, 0.074006644, -7.0221731e-4)
+
+
+def gelu_fit(z):
+    """The device's GELU form (|gelu_fit - gelu| <= 2.6e-5)."""
+    zc = np.clip(z, -9.0, 9.0)
+    t = zc * zc
+    P = GELU_FIT[0] + t * (GELU_FIT[1] + t * GELU_FIT[2])
+    return z / (1.0 + np.exp(-zc * P))
+
+
 def gelu_grad(z):
     return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
 
@@ -56,15 +69,16 @@ def shift(x, dy, dx):
     return np.roll(np.roll(x, -dy, axis=1), -dx, axis=2)
 
 
-def forward(p, x, keep=False, q=None):
+def forward(p, x, keep=False, q=None, gelu_fn=None):
     """x: (B, H, W) network input.  Returns out (B, H, W) and, with keep, the cache.
     q, if given, is applied to the weights of the four 32 -> 32 layers and to every
     activation (e.g. bf16_round, to model a device that stores those in bfloat16 and keeps
-    the input, W1 and W6 in float32)."""
+    the input, W1 and W6 in float32); gelu_fn replaces gelu (e.g. gelu_fit)."""
+    gelu_ = gelu_fn if gelu_fn is not None else gelu
     W1, b1, mids, W6, b6 = split(p)
     if q is not None:
         mids = [(q(W), b) for W, b in mids]
-    act = (lambda z: q(gelu(z))) if q is not None else gelu
+    act = (lambda z: q(gelu_(z))) if q is not None else gelu_
     z1 = np.zeros(x.shape + (C,))
     for t, (dy, dx) in enumerate(TAPS):
         z1 += shift(x, dy, dx)[..., None] * W1[:, t]

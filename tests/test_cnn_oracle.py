@@ -66,6 +66,19 @@ def test_gradients_match_finite_differences(cnn):
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_gelu_fit_error_bound(cnn):
+    """The device's GELU form against z Phi(z): the stated maximum error, the limits z and 0
+    far out, and z / 2 slope at the origin (a wrong coefficient or sign breaks one)."""
+    z = np.linspace(-40.0, 40.0, 800001)
+    err = np.abs(cnn.gelu_fit(z) - cnn.gelu(z))
+    assert err.max() < 2.6e-5
+    assert err[np.abs(z) <= 9].max() > 2.0e-5  # the fit's equioscillation level, not smaller
+    assert np.array_equal(cnn.gelu_fit(np.array([0.0])), [0.0])
+    assert abs(cnn.gelu_fit(50.0) - 50.0) < 1e-9 and abs(cnn.gelu_fit(-50.0)) < 1e-9
+    h = 1e-6
+    assert abs(cnn.gelu_fit(h) / h - 0.5) < 1e-5
+
+
 def test_gelu_derivative(cnn):
     z = np.linspace(-4, 4, 41)
     fd = (cnn.gelu(z + 1e-6) - cnn.gelu(z - 1e-6)) / 2e-6

@@ -1,7 +1,7 @@
 """GPU kernels of the learned corrector against the FP64 reference network
 (oracle/cnn_oracle.py) evaluated with the same bfloat16 roundings (32 -> 32 weights,
-activations, pre-activation gradients); the remaining differences are FP32 accumulation
-order and single-precision GELU."""
+activations, pre-activation gradients) and the device's GELU form (gelu_fit); the remaining
+differences are FP32 accumulation order and single-precision GELU arithmetic."""
 import importlib.util
 import os
 
@@ -42,7 +42,8 @@ def test_forward_matches_reference(gc, cnn, ctx, dtype):
     net.set_params(p, 0.1)
     d_c = gc.DeviceArray.from_numpy(ctx, c.astype(dtype))
     got = net.forward(d_c, B).astype(np.float64)
-    want = cnn.forward(p, c.astype(dtype).astype(np.float64), q=cnn.bf16_round)
+    want = cnn.forward(p, c.astype(dtype).astype(np.float64), q=cnn.bf16_round,
+                       gelu_fn=cnn.gelu_fit)
     assert _rel(got, want) < 3e-3, _rel(got, want)
     net.close()
 
@@ -60,7 +61,7 @@ def test_gradients_match_reference(gc, cnn, ctx):
     d_t = gc.DeviceArray.from_numpy(ctx, target)
     loss = net.loss_grad(d_c, d_t, B=B)
     g = net.grads().astype(np.float64)
-    out, cache = cnn.forward(p, c, keep=True, q=cnn.bf16_round)
+    out, cache = cnn.forward(p, c, keep=True, q=cnn.bf16_round, gelu_fn=cnn.gelu_fit)
     d = out - (target - c) / scale
     assert abs(loss - np.mean(d * d)) < 3e-3 * np.mean(d * d)
     want = cnn.backward(p, cache, 2 * d / d.size, q=cnn.bf16_round)
