@@ -277,27 +277,32 @@ __device__ __forceinline__ void mass_fe_reduce(const Comp2* __restrict__ part, i
   }
 }
 
-// ---- whole-network inference in one kernel -------------------------------------------------
-// A block owns an 8 x 32 output tile and evaluates all six layers on it, each on the tile
-// plus the halo the remaining layers need (layer l on a (6-l)-site halo): the state region
-// (halo 6) is staged in LDS as FP32, activations ping-pong between two LDS buffers (site
-// stride kSP), and nothing but the output leaves the block.  Every value is formed with the
+// ---- whole-network inference in two kernels ------------------------------------------------
+// A block owns an 8 x 32 output tile.  Kernel A evaluates layers 1..3 on it, each on the tile
+// plus the halo the later layers of the kernel need (layer 1 on a 2-site halo, layer 2 on a
+// 1-site halo, layer 3 on the tile), from the state region (halo 3) staged in LDS as FP32, and
+// writes layer 3's activations (BF16, site-major, like the per-layer path) to global memory;
+// kernel B reads them on a 3-site halo and evaluates layers 4..6 the same way.  Activations
+// ping-pong between two LDS buffers (site stride kSP).  Every value is formed with the
 // per-layer kernels' arithmetic (same FMA order, same MFMA operand fragments and K order), so
-// the output is bit-identical to theirs.  The halos cost 1.9x the tile's matrix work in
-// layers 2..5 (1984 sites in 32-site chunks for 1024) and 3x its layer-1 work, which at one
-// image per launch is cheaper than five more launches.
-constexpr int kFR = kTR, kFC = kTC;                       // output tile
-constexpr int kFH0 = kFR + 12, kFW0 = kFC + 12;           // state region (halo 6)
-constexpr int kFNA = (kFR + 10) * (kFC + 10);             // layer-1 region (largest)
-constexpr int kFNB = (kFR + 8) * (kFC + 8);               // layer-2 region
+// the output is bit-identical to theirs.  Recomputed halos cost 1.34x the tile's matrix work
+// in layers 2..5 (1368 sites in 32-site chunks for 1024) and 1.7x its layer-1 work; one
+// kernel for all six layers needs halos up to 6 sites (1984 sites, 3x layer-1 work), which
+// costs more than the extra launch and the 64 bytes per site of layer-3 activations.
+constexpr int kFR = kTR, kFC = kTC;                // output tile
+constexpr int kAH0 = kFR + 6, kAW0 = kFC + 6;      // A: state region (halo 3)
+constexpr int kAN1 = (kFR + 4) * (kFC + 4);        // A: layer-1 region (halo 2)
+constexpr int kAN2 = (kFR + 2) * (kFC + 2);        // A: layer-2 region (halo 1)
+constexpr int kBN3 = (kFR + 6) * (kFC + 6);        // B: layer-3 region (halo 3)
+constexpr int kBN4 = (kFR + 4) * (kFC + 4);        // B: layer-4 region (halo 2)
 
-constexpr int kFNT = 512;  // threads of the fused kernel (8 waves)
+constexpr int kFNT = 512;  // threads of the network kernels (8 waves)
 
-// Chained mode (learned integration, one image): the state the launch advances is formed on
+// Chained mode (learned integration, one image): the state kernel A advances is formed on
 // the fly from the previous macro step, c_k = c_{k-1} + scale (out_{k-1} - mean(out_{k-1}))
 // (k_apply's expression, so the same values); the block writes it for its own tile, writes
 // its tile's mass and free-energy partial sums (merged by a later kernel) and evaluates the
-// network on it.  One launch per macro step instead of two.
+// network on it.  Two launches per macro step instead of three.
 template <class TS>
 struct Chain {
   const TS* cprev;       // state of the previous macro step
@@ -309,39 +314,89 @@ struct Chain {
   Comp2* mpart;          // per-tile (mass, density) partial sums
 };
 
+// One 32 -> 32 layer (index l of the four) on the matrix cores, output region = tile + halo
+// HO, input (in, LDS, site stride kSP) one site wider.  The output region's sites, row-major,
+// are cut into chunks of 32; a wave takes chunks c and c + 8 (independent accumulators, so
+// the two MFMA chains overlap), c = wave, wave + 16, ...; store(site, g, v) receives channels
+// 8 g + cq .. 8 g + cq + 3 of an output site as BF16.
+template <int HO, class Store>
+__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
+                                          const __bf16* __restrict__ wmid,
+                                          const float* __restrict__ p, int wave, int lane,
+                                          Store store) {
+  constexpr int Wo = kFC + 2 * HO, Ho = kFR + 2 * HO, Wi = Wo + 2, NS = Wo * Ho;
+  constexpr int nch = (NS + 31) / 32;
+  const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
+  bf16x8 wreg[18];
+  const __bf16* wt = wmid + (size_t)l * kMidW;
+#pragma unroll
+  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
+  float bj[16];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bj[4 * g + i] = p[kOffMid + l * kMidStride + kMidW + g * 8 + cq + i];
+  auto emit = [&](const f32x16& acc, int s) {
+    if (s < NS) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        bf16x4 ov;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
+        store(s, g * 8 + cq, ov);
+      }
+    }
+  };
+  for (int c = wave; c < nch; c += 16) {
+    const int c1 = c + 8;
+    const int s0 = min(c * 32 + j, NS - 1), s1 = min(c1 * 32 + j, NS - 1);
+    const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1, i1 = (s1 / Wo + 1) * Wi + s1 % Wo + 1;
+    f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+    for (int kc = 0; kc < 18; ++kc) {
+      const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
+      const int ko = (kc & 1) * 16 + kh;
+      const bf16x8 a0 = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
+      const bf16x8 a1 = *(const bf16x8*)(in + (i1 + off) * kSP + ko);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a1, acc1, 0, 0, 0);
+    }
+    emit(acc0, c * 32 + j);
+    if (c1 < nch) emit(acc1, c1 * 32 + j);
+  }
+}
+
+// Kernel A: layers 1..3 of the network on the block's tile; layer 3's activations to a3.
 template <class TS, bool APPLY>
-__global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
-                                                   const float* __restrict__ w1,
-                                                   const float* __restrict__ b1,
-                                                   const __bf16* __restrict__ wmid,
-                                                   const float* __restrict__ p,
-                                                   const float* __restrict__ w6,
-                                                   const float* __restrict__ b6,
-                                                   float* __restrict__ out,
-                                                   double* __restrict__ part, int H, int W,
-                                                   int tiles_x, int tiles_y, Chain<TS> ch) {
-  __shared__ float cs[kFH0 * kFW0];
+__global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
+                                               const float* __restrict__ w1,
+                                               const float* __restrict__ b1,
+                                               const __bf16* __restrict__ wmid,
+                                               const float* __restrict__ p,
+                                               __bf16* __restrict__ a3, int H, int W,
+                                               int tiles_x, int tiles_y, Chain<TS> ch) {
+  __shared__ float cs[kAH0 * kAW0];
   __shared__ Comp2 msh[kFR * kFC / 64];
-  __shared__ double psum[kFR * kFC / 64];
-  __shared__ float w1s[288 + 32], w6s[288];
-  __shared__ __attribute__((aligned(16))) __bf16 bufA[kFNA * kSP];
-  __shared__ __attribute__((aligned(16))) __bf16 bufB[kFNB * kSP];
+  __shared__ float w1s[288 + 32];
+  __shared__ __attribute__((aligned(16))) __bf16 bufA[kAN1 * kSP];
+  __shared__ __attribute__((aligned(16))) __bf16 bufB[kAN2 * kSP];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
+  constexpr int NR = kAH0 * kAW0;
   if constexpr (APPLY) {
     // Every load is issued before the first use: the state region, the tile's five-point
     // neighbourhoods and the previous tile sums.
-    constexpr int kNH = (kFH0 * kFW0 + kFNT - 1) / kFNT;
+    constexpr int kNH = (NR + kFNT - 1) / kFNT;
     double hc[kNH];
     float ho[kNH];
 #pragma unroll
     for (int r = 0; r < kNH; ++r) {
       const int q = t + r * kFNT;
-      if (q < kFH0 * kFW0) {
-        const size_t g = (size_t)wrap1(y0 + q / kFW0 - 6, H) * W + wrap1(x0 + q % kFW0 - 6, W);
+      if (q < NR) {
+        const size_t g = (size_t)wrap1(y0 + q / kAW0 - 3, H) * W + wrap1(x0 + q % kAW0 - 3, W);
         hc[r] = (double)ch.cprev[g];
         ho[r] = ch.oprev[g];
       }
@@ -373,7 +428,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
 #pragma unroll
     for (int r = 0; r < kNH; ++r) {
       const int q = t + r * kFNT;
-      if (q < kFH0 * kFW0) cs[q] = (float)nv(hc[r], ho[r]);
+      if (q < NR) cs[q] = (float)nv(hc[r], ho[r]);
     }
     if (own) {  // the tile: write the new state, reduce its mass and density
       TS x[5];
@@ -388,24 +443,23 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
     }
   } else {
     const TS* img = c + (size_t)b * H * W;
-    for (int q = t; q < kFH0 * kFW0; q += kFNT) {
-      const int ly = q / kFW0, lx = q % kFW0;
-      cs[q] = (float)img[(size_t)wrap1(y0 + ly - 6, H) * W + wrap1(x0 + lx - 6, W)];
+    for (int q = t; q < NR; q += kFNT) {
+      const int ly = q / kAW0, lx = q % kAW0;
+      cs[q] = (float)img[(size_t)wrap1(y0 + ly - 3, H) * W + wrap1(x0 + lx - 3, W)];
     }
   }
   for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
-  for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
   __syncthreads();
   if constexpr (APPLY)
     if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
-  // layer 1 on the halo-5 region (kFR + 10) x (kFC + 10)
+  // layer 1 on the halo-2 region (kFR + 4) x (kFC + 4)
   {
-    constexpr int RW = kFC + 10, RN = kFNA;
+    constexpr int RW = kFC + 4, RN = kAN1;
     for (int q = t; q < RN; q += kFNT) {
       const int ry = q / RW, rx = q % RW;
       float xv[9];
 #pragma unroll
-      for (int tp = 0; tp < 9; ++tp) xv[tp] = cs[(ry + tp / 3) * kFW0 + rx + tp % 3];
+      for (int tp = 0; tp < 9; ++tp) xv[tp] = cs[(ry + tp / 3) * kAW0 + rx + tp % 3];
 #pragma unroll 1
       for (int co = 0; co < kC; co += 8) {
         bf16x8 av;
@@ -421,56 +475,52 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
     }
   }
   __syncthreads();
-  // layers 2..5 on the matrix cores: output region halo h = 4..1, input one site wider.  The
-  // output region's sites, row-major, are cut into chunks of 32; a wave takes chunks c and
-  // c + 8 (independent accumulators, so the two MFMA chains overlap), c = wave, wave + 16, ...
-  const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
-#pragma unroll 1
-  for (int l = 0; l < 4; ++l) {
-    const int h = 4 - l;
-    const int Wo = kFC + 2 * h, Ho = kFR + 2 * h, Wi = Wo + 2, NS = Wo * Ho;
-    const int nch = (NS + 31) / 32;
-    const __bf16* in = (l & 1) ? bufB : bufA;
-    __bf16* ob = (l & 1) ? bufA : bufB;
-    bf16x8 wreg[18];
-    const __bf16* wt = wmid + (size_t)l * kMidW;
-#pragma unroll
-    for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
-    float bj[16];
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) bj[4 * g + i] = p[kOffMid + l * kMidStride + kMidW + g * 8 + cq + i];
-    auto store = [&](const f32x16& acc, int s) {
-      if (s < NS) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          bf16x4 ov;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
-          *(bf16x4*)(ob + s * kSP + g * 8 + cq) = ov;
-        }
-      }
-    };
-    for (int c = wave; c < nch; c += 16) {
-      const int c1 = c + 8;
-      const int s0 = min(c * 32 + j, NS - 1), s1 = min(c1 * 32 + j, NS - 1);
-      const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1, i1 = (s1 / Wo + 1) * Wi + s1 % Wo + 1;
-      f32x16 acc0 = {}, acc1 = {};
-#pragma unroll
-      for (int kc = 0; kc < 18; ++kc) {
-        const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
-        const int ko = (kc & 1) * 16 + kh;
-        const bf16x8 a0 = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
-        const bf16x8 a1 = *(const bf16x8*)(in + (i1 + off) * kSP + ko);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a1, acc1, 0, 0, 0);
-      }
-      store(acc0, c * 32 + j);
-      if (c1 < nch) store(acc1, c1 * 32 + j);
-    }
-    __syncthreads();
+  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+    *(bf16x4*)(bufB + s * kSP + ch0) = v;
+  });
+  __syncthreads();
+  const size_t img = (size_t)b * H * W;
+  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+    *(bf16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
+  });
+}
+
+// Kernel B: layers 4..6 from layer 3's activations; output to out, the tile's output sum to
+// part[block].
+__global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
+                                               const __bf16* __restrict__ wmid,
+                                               const float* __restrict__ p,
+                                               const float* __restrict__ w6,
+                                               const float* __restrict__ b6,
+                                               float* __restrict__ out,
+                                               double* __restrict__ part, int H, int W,
+                                               int tiles_x, int tiles_y) {
+  __shared__ double psum[kFR * kFC / 64];
+  __shared__ float w6s[288];
+  __shared__ __attribute__((aligned(16))) __bf16 bufA[kBN3 * kSP];
+  __shared__ __attribute__((aligned(16))) __bf16 bufB[kBN4 * kSP];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tx = blockIdx.x % tiles_x;
+  const int r1 = blockIdx.x / tiles_x;
+  const int ty = r1 % tiles_y, b = r1 / tiles_y;
+  const int x0 = tx * kFC, y0 = ty * kFR;
+  const size_t img = (size_t)b * H * W;
+  for (int q = t; q < kBN3 * 4; q += kFNT) {
+    const int site = q >> 2, ch = (q & 3) * 8;
+    const int ly = site / (kFC + 6), lx = site % (kFC + 6);
+    const int gy = wrap1(y0 + ly - 3, H), gx = wrap1(x0 + lx - 3, W);
+    *(bf16x8*)(bufA + site * kSP + ch) = *(const bf16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
   }
+  for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
+  __syncthreads();
+  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+    *(bf16x4*)(bufB + s * kSP + ch0) = v;
+  });
+  __syncthreads();
+  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+    *(bf16x4*)(bufA + s * kSP + ch0) = v;
+  });
+  __syncthreads();
   // layer 6 on the tile from the layer-5 region (halo 1, in bufA)
   if (t < kFR * kFC) {
     constexpr int Wi = kFC + 2;
@@ -486,7 +536,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_fused(const TS* __restrict__ c,
       }
     }
     const float o = acc + b6[0];
-    out[(size_t)b * H * W + (size_t)(y0 + ry) * W + x0 + rx] = o;
+    out[img + (size_t)(y0 + ry) * W + x0 + rx] = o;
     // the tile's sum of the output (for the mean removal), in a fixed order
     double sd = (double)o;
     for (int st = 32; st >= 1; st >>= 1) sd += __shfl_xor(sd, st);
@@ -873,16 +923,21 @@ gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
   return GC_OK;
 }
 
-// Inference: the whole network in one kernel per 8 x 32 tile (k_fwd_fused); output to n->out.
+// Inference: the whole network in two kernels per 8 x 32 tile (k_fwd_a, k_fwd_b), layer 3's
+// activations passing through n->a[2]; output to n->out.
 template <class TS>
 gc_status forward_fused(gc_cnn* n, const TS* c, int B, float* out = nullptr,
                         double* part = nullptr) {
   if (!n->packed) GC_TRY(pack(n));
   const int tx = n->W / kTC, ty = n->H / kTR;
-  hipLaunchKernelGGL((k_fwd_fused<TS, false>), dim3((unsigned)(tx * ty * B)), dim3(kFNT), 0,
-                     n->ctx->stream, c, n->p + kOffW1, n->p + kOffB1, n->wmid, n->p,
+  const dim3 grid((unsigned)(tx * ty * B));
+  hipLaunchKernelGGL((k_fwd_a<TS, false>), grid, dim3(kFNT), 0, n->ctx->stream, c,
+                     n->p + kOffW1, n->p + kOffB1, n->wmid, n->p, n->a[2], n->H, n->W, tx, ty,
+                     Chain<TS>{});
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_fwd_b, grid, dim3(kFNT), 0, n->ctx->stream, n->a[2], n->wmid, n->p,
                      n->p + kOffW6, n->p + kOffB6, out ? out : n->out, part ? part : n->dpart,
-                     n->H, n->W, tx, ty, Chain<TS>{});
+                     n->H, n->W, tx, ty);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -893,9 +948,13 @@ template <class TS>
 gc_status forward_chain(gc_cnn* n, const Chain<TS>& ch, float* out, double* part) {
   if (!n->packed) GC_TRY(pack(n));
   const int tx = n->W / kTC, ty = n->H / kTR;
-  hipLaunchKernelGGL((k_fwd_fused<TS, true>), dim3((unsigned)(tx * ty)), dim3(kFNT), 0,
-                     n->ctx->stream, (const TS*)nullptr, n->p + kOffW1, n->p + kOffB1, n->wmid,
-                     n->p, n->p + kOffW6, n->p + kOffB6, out, part, n->H, n->W, tx, ty, ch);
+  const dim3 grid((unsigned)(tx * ty));
+  hipLaunchKernelGGL((k_fwd_a<TS, true>), grid, dim3(kFNT), 0, n->ctx->stream,
+                     (const TS*)nullptr, n->p + kOffW1, n->p + kOffB1, n->wmid, n->p, n->a[2],
+                     n->H, n->W, tx, ty, ch);
+  GC_LAUNCHED();
+  hipLaunchKernelGGL(k_fwd_b, grid, dim3(kFNT), 0, n->ctx->stream, n->a[2], n->wmid, n->p,
+                     n->p + kOffW6, n->p + kOffB6, out, part, n->H, n->W, tx, ty);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -1283,10 +1342,11 @@ gc_status mass_fe_async(gc_cnn* n, const gc_grid* g, gc_dtype dt, const void* c,
 
 // Learned proposals of one chunk: slot k of hist receives proposal k (k = 1..K) of the chain
 // starting at c, c_k = c_{k-1} + scale*(out(c_{k-1}) - mean), and diag[2k..2k+1] its (sum of
-// c_k, sum of its density).  Launch 1 evaluates the network on c; launch k = 2..K forms
-// c_{k-1} from the previous launch's output, writes it, reduces its diagnostics and evaluates
-// the network on it (the two network outputs and tile sums alternate); a last launch forms
-// c_K with its diagnostics.  K + 1 launches for K proposals.
+// c_k, sum of its density).  Step 1 evaluates the network on c; step k = 2..K forms c_{k-1}
+// from the previous step's output, writes it, reduces its diagnostics and evaluates the
+// network on it (the two network outputs and tile sums alternate); a last kernel forms c_K
+// with its diagnostics, and one more merges the diagnostics of all K.  2K + 2 launches for K
+// proposals.
 template <class TS>
 gc_status propose_chunk(gc_cnn* n, const gc_grid* g, const TS* c, TS* hist, int K,
                         double kappa, double* diag) {
