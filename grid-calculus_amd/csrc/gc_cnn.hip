@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
   const int j = lane & 31, kh = (lane >> 5) * 8;
   bf16x8 wreg[18];
 #pragma unroll
-  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
+  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
   __syncthreads();
 #pragma unroll
   for (int rr = 0; rr < 2; ++rr) {
@@ -330,7 +330,7 @@ __device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
   bf16x8 wreg[18];
   const __bf16* wt = wmid + (size_t)l * kMidW;
 #pragma unroll
-  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)j * 288 + kc * 16 + kh);
+  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
   float bj[16];
 #pragma unroll
   for (int g = 0; g < 4; ++g)
@@ -801,15 +801,24 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
 // FP32 master parameters -> BF16 working copies of the 32 -> 32 matrices [co][k] and
 // their flipped transposes [ci][tap'*32 + co] = W[co][(8 - tap')*32 + ci] (backward-data).
 // W1 and W6 are used in FP32 directly.
+// Weights of the four 32 -> 32 layers in BF16, as the matrix-core A operands: forward
+// (wmid, rows co, columns tap * 32 + ci) and backward-data (wmidT, rows ci, columns
+// tap' * 32 + co with tap' = 8 - tap).  Stored fragment-major: for K block kc (16 columns),
+// lane (h, j) of a wave holds row j, columns kc * 16 + 8 h .. + 7, at
+// ((kc * 64 + 32 h + j) * 8), so each of a wave's 18 fragment loads reads 1 KiB contiguously.
+__device__ __forceinline__ int frag_index(int row, int col) {
+  return ((col / 16) * 64 + ((col % 16) / 8) * 32 + row) * 8 + col % 8;
+}
 __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf16* __restrict__ wmid,
                                               __bf16* __restrict__ wmidT) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < 4 * kMidW) {
     const int l = i / kMidW, e = i % kMidW;
     const float* W = p + kOffMid + l * kMidStride;
-    wmid[i] = (__bf16)W[e];
-    const int ci = e / 288, k = e % 288, tpp = k / 32, co = k % 32;
-    wmidT[i] = (__bf16)W[co * 288 + (8 - tpp) * 32 + ci];
+    const int r = e / 288, k = e % 288;
+    wmid[l * kMidW + frag_index(r, k)] = (__bf16)W[e];
+    const int tpp = k / 32, co = k % 32;  // wmidT row r = ci, column k = tap' * 32 + co
+    wmidT[l * kMidW + frag_index(r, k)] = (__bf16)W[co * 288 + (8 - tpp) * 32 + r];
   }
 }
 
