@@ -314,28 +314,47 @@ struct Chain {
   Comp2* mpart;          // per-tile (mass, density) partial sums
 };
 
-// One 32 -> 32 layer (index l of the four) on the matrix cores, output region = tile + halo
-// HO, input (in, LDS, site stride kSP) one site wider.  The output region's sites, row-major,
-// are cut into chunks of 32; a wave takes chunks c and c + 8 (independent accumulators, so
-// the two MFMA chains overlap), c = wave, wave + 16, ...; store(site, g, v) receives channels
+// A lane's MFMA weight fragments and biases of 32 -> 32 layer l (index of the four).
+// mid_weights issues the loads; pin() then holds the values in registers with an empty asm
+// that reads and redefines them.  Without it the compiler places each fragment's load just
+// before its MFMA, so a layer waits for 18 load latencies one after the other instead of one.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+struct MidWeights {
+  bf16x8 w[18];
+  f32x4 b[4];
+};
+__device__ __forceinline__ void mid_weights(int l, const __bf16* __restrict__ wmid,
+                                            const float* __restrict__ p, int lane,
+                                            MidWeights& mw) {
+  const __bf16* wt = wmid + (size_t)l * kMidW;
+#pragma unroll
+  for (int kc = 0; kc < 18; ++kc) mw.w[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
+  const float* bias = p + kOffMid + l * kMidStride + kMidW + (lane >> 5) * 4;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) mw.b[g] = *(const f32x4*)(bias + g * 8);
+}
+__device__ __forceinline__ void pin(MidWeights& mw) {
+#pragma unroll
+  for (int kc = 0; kc < 18; ++kc) asm volatile("" : "+v"(mw.w[kc]));
+#pragma unroll
+  for (int g = 0; g < 4; ++g) asm volatile("" : "+v"(mw.b[g]));
+}
+
+// One 32 -> 32 layer on the matrix cores, output region = tile + halo HO, input (in, LDS,
+// site stride kSP) one site wider.  The output region's sites, row-major, are cut into
+// chunks of 32; a wave takes chunks c and c + 8 (independent accumulators, so the two MFMA
+// chains overlap), c = wave, wave + 16, ...; store(site, g, v) receives channels
 // 8 g + cq .. 8 g + cq + 3 of an output site as BF16.
 template <int HO, class Store>
-__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
-                                          const __bf16* __restrict__ wmid,
-                                          const float* __restrict__ p, int wave, int lane,
-                                          Store store) {
+__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, const MidWeights& mw,
+                                          int wave, int lane, Store store) {
   constexpr int Wo = kFC + 2 * HO, Ho = kFR + 2 * HO, Wi = Wo + 2, NS = Wo * Ho;
   constexpr int nch = (NS + 31) / 32;
   const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
-  bf16x8 wreg[18];
-  const __bf16* wt = wmid + (size_t)l * kMidW;
-#pragma unroll
-  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
+  const bf16x8* wreg = mw.w;
   float bj[16];
 #pragma unroll
-  for (int g = 0; g < 4; ++g)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bj[4 * g + i] = p[kOffMid + l * kMidStride + kMidW + g * 8 + cq + i];
+  for (int i = 0; i < 16; ++i) bj[i] = mw.b[i / 4][i % 4];
   auto emit = [&](const f32x16& acc, int s) {
     if (s < NS) {
 #pragma unroll
@@ -386,6 +405,8 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   constexpr int NR = kAH0 * kAW0;
+  MidWeights mw;
+  mid_weights(0, wmid, p, lane, mw);
   if constexpr (APPLY) {
     // Every load is issued before the first use: the state region, the tile's five-point
     // neighbourhoods and the previous tile sums.
@@ -443,12 +464,21 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
     }
   } else {
     const TS* img = c + (size_t)b * H * W;
-    for (int q = t; q < NR; q += kFNT) {
-      const int ly = q / kAW0, lx = q % kAW0;
-      cs[q] = (float)img[(size_t)wrap1(y0 + ly - 3, H) * W + wrap1(x0 + lx - 3, W)];
+    constexpr int kNH = (NR + kFNT - 1) / kFNT;
+    TS hv[kNH];
+#pragma unroll
+    for (int r = 0; r < kNH; ++r) {
+      const int q = t + r * kFNT;
+      if (q < NR) hv[r] = img[(size_t)wrap1(y0 + q / kAW0 - 3, H) * W + wrap1(x0 + q % kAW0 - 3, W)];
+    }
+#pragma unroll
+    for (int r = 0; r < kNH; ++r) {
+      const int q = t + r * kFNT;
+      if (q < NR) cs[q] = (float)hv[r];
     }
   }
   for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
+  pin(mw);
   __syncthreads();
   if constexpr (APPLY)
     if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
@@ -475,12 +505,14 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
     }
   }
   __syncthreads();
-  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<1>(bufA, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufB + s * kSP + ch0) = v;
   });
+  mid_weights(1, wmid, p, lane, mw);
+  pin(mw);
   __syncthreads();
   const size_t img = (size_t)b * H * W;
-  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<0>(bufB, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
   });
 }
@@ -505,19 +537,35 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   const size_t img = (size_t)b * H * W;
-  for (int q = t; q < kBN3 * 4; q += kFNT) {
-    const int site = q >> 2, ch = (q & 3) * 8;
-    const int ly = site / (kFC + 6), lx = site % (kFC + 6);
-    const int gy = wrap1(y0 + ly - 3, H), gx = wrap1(x0 + lx - 3, W);
-    *(bf16x8*)(bufA + site * kSP + ch) = *(const bf16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
+  MidWeights mw;
+  mid_weights(2, wmid, p, lane, mw);
+  // layer 3's activations on the halo-3 region: every load issued before the first store
+  constexpr int NQ = (kBN3 * 4 + kFNT - 1) / kFNT;
+  bf16x8 rv[NQ];
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    const int q = t + r * kFNT;
+    if (q < kBN3 * 4) {
+      const int site = q >> 2, ch = (q & 3) * 8;
+      const int gy = wrap1(y0 + site / (kFC + 6) - 3, H), gx = wrap1(x0 + site % (kFC + 6) - 3, W);
+      rv[r] = *(const bf16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
+    }
+  }
+  pin(mw);
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    const int q = t + r * kFNT;
+    if (q < kBN3 * 4) *(bf16x8*)(bufA + (q >> 2) * kSP + (q & 3) * 8) = rv[r];
   }
   for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
   __syncthreads();
-  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<2>(bufA, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufB + s * kSP + ch0) = v;
   });
+  mid_weights(3, wmid, p, lane, mw);
+  pin(mw);
   __syncthreads();
-  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<1>(bufB, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufA + s * kSP + ch0) = v;
   });
   __syncthreads();
