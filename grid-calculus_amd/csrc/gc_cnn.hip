@@ -297,6 +297,7 @@ constexpr int kBN3 = (kFR + 6) * (kFC + 6);        // B: layer-3 region (halo 3)
 constexpr int kBN4 = (kFR + 4) * (kFC + 4);        // B: layer-4 region (halo 2)
 
 constexpr int kFNT = 512;  // threads of the network kernels (8 waves)
+static_assert(kFNT / 2 == kFR * kFC, "layer 6 maps two lanes to each tile site");
 
 // Chained mode (learned integration, one image): the state kernel A advances is formed on
 // the fly from the previous macro step, c_k = c_{k-1} + scale (out_{k-1} - mean(out_{k-1}))
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
                                                float* __restrict__ out,
                                                double* __restrict__ part, int H, int W,
                                                int tiles_x, int tiles_y) {
-  __shared__ double psum[kFR * kFC / 64];
+  __shared__ double psum[kFNT / 64];
   __shared__ float w6s[288];
   __shared__ __attribute__((aligned(16))) __bf16 bufA[kBN3 * kSP];
   __shared__ __attribute__((aligned(16))) __bf16 bufB[kBN4 * kSP];
@@ -569,31 +570,34 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
     *(bf16x4*)(bufA + s * kSP + ch0) = v;
   });
   __syncthreads();
-  // layer 6 on the tile from the layer-5 region (halo 1, in bufA)
-  if (t < kFR * kFC) {
+  // layer 6 on the tile from the layer-5 region (halo 1, in bufA): lanes 0..31 of wave w
+  // take tile sites 32 w .. 32 w + 31 over channels 0..15, lanes 32..63 the same sites over
+  // channels 16..31 (k_conv6's two accumulators)
+  {
     constexpr int Wi = kFC + 2;
-    const int ry = t / kFC, rx = t % kFC;
+    const int ts = wave * 32 + (lane & 31), h = lane >> 5;
+    const int ry = ts / kFC, rx = ts % kFC;
     float acc = 0.0f;
     for (int tp = 0; tp < 9; ++tp) {
       const int site = (ry + tp / 3) * Wi + rx + tp % 3;
 #pragma unroll
-      for (int ch = 0; ch < kC; ch += 8) {
+      for (int ch = 16 * h; ch < 16 * h + 16; ch += 8) {
         const bf16x8 v = *(const bf16x8*)(bufA + site * kSP + ch);
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc = fmaf(w6s[tp * kC + ch + u], (float)v[u], acc);
       }
     }
-    const float o = acc + b6[0];
-    out[img + (size_t)(y0 + ry) * W + x0 + rx] = o;
+    const float o = (acc + __shfl_xor(acc, 32)) + b6[0];
+    if (h == 0) out[img + (size_t)(y0 + ry) * W + x0 + rx] = o;
     // the tile's sum of the output (for the mean removal), in a fixed order
-    double sd = (double)o;
+    double sd = h == 0 ? (double)o : 0.0;
     for (int st = 32; st >= 1; st >>= 1) sd += __shfl_xor(sd, st);
-    if ((t & 63) == 0) psum[t >> 6] = sd;
+    if (lane == 0) psum[wave] = sd;
   }
   __syncthreads();
   if (t == 0) {
     double sd = psum[0];
-    for (int w = 1; w < kFR * kFC / 64; ++w) sd += psum[w];
+    for (int w = 1; w < kFNT / 64; ++w) sd += psum[w];
     part[blockIdx.x] = sd;
   }
 }
@@ -702,17 +706,17 @@ __global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
   if (s >= (long)B * HW) return;
   const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
   const size_t img = (size_t)b * HW;
-  float acc = 0.0f;
+  float acc[2] = {0.0f, 0.0f};  // channels 0..15 and 16..31, each over taps, then channels
   for (int tp = 0; tp < 9; ++tp) {
     const size_t n = img + (size_t)wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W);
 #pragma unroll
     for (int ch = 0; ch < kC; ch += 8) {
       const bf16x8 v = *(const bf16x8*)(a5 + n * kC + ch);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc = fmaf(w6[tp * kC + ch + u], (float)v[u], acc);
+      for (int u = 0; u < 8; ++u) acc[ch / 16] = fmaf(w6[tp * kC + ch + u], (float)v[u], acc[ch / 16]);
     }
   }
-  out[s] = acc + b6[0];
+  out[s] = (acc[0] + acc[1]) + b6[0];
 }
 
 // Loss on the normalised increment: d = out - (target - c)/scale; L = mean(d^2);
