@@ -377,7 +377,9 @@ def learned(a):
     energy.  Error: relative L2 against the plain FP64 GPU RK4 result (bit-identical to the
     CPU oracle's RK4; the parity tests pin that).  The plain path's tile shape is autotuned
     first.  Times are wall-clock, device-synchronised, after one untimed run of each path;
-    n_ic initial states, median time, max error."""
+    n_ic initial states, median time, max error.  Variants: the first macro step by plain RK4
+    (warm1), and the phase-specialised networks (corrector.integrate_phased: first macro step
+    plain, then one fine-tuned copy of the network per range of macro steps)."""
     gc = entry.load_package()
     from gridcalc_amd import corrector as cr  # noqa: E402
     params, meta = cr.load_params(256, 256)
@@ -390,7 +392,7 @@ def learned(a):
     seeds = [991001 + i for i in range(a.n_ic)]
     res = {"plain_fp64": [], "plain_fp32": [], "plain_fp64_graph": [], "learned_fp64": [],
            "learned_fp64_warm1": []}
-    errs, errs32, errs_w, fallbacks = [], [], [], []
+    errs, errs32, errs_w, errs_p, fallbacks, fallbacks_p = [], [], [], [], [], []
     with gc.Context() as ctx:
         tiles = {}
         for dtype, name in ((gc.F64, "f64"), (gc.F32, "f32")):  # the plain path at its best
@@ -398,6 +400,11 @@ def learned(a):
             tiles[name] = gc.ch_config_shape(g, best)[:2]
         net = gc.Corrector(ctx, H, W, 1)
         net.set_params(params, meta["scale"])
+        # phase-specialised copies of the network (first macro step plain, then one fine-tuned
+        # network per range of macro steps), if shipped
+        phased = cr.phase_nets(ctx, H, W)
+        if phased is not None:
+            res["learned_fp64_phased"] = []
 
         def timed(fn):
             ctx.sync()
@@ -415,6 +422,8 @@ def learned(a):
                 for arr in (ref, f32):
                     gc.ch_integrate(ctx, g, arr, 1.0, 1.0, dt, 32, gc.RK4)
                 net.integrate(g, lrn, 1.0, 1.0, dt, sub, 2)
+                for pn, _, _ in phased or []:
+                    pn.integrate(g, lrn, 1.0, 1.0, dt, sub, 2)
                 ref.upload(ctx, c0)
                 f32.upload(ctx, c0.astype(np.float32))
                 lrn.upload(ctx, c0)
@@ -445,15 +454,27 @@ def learned(a):
                 return net.integrate(g, wrm, 1.0, 1.0, dt, sub, n_steps // sub - 1, 1e-6)
             t, _ = timed(warm)
             res["learned_fp64_warm1"].append(t)
+            if phased is not None:
+                phs = gc.DeviceArray.from_numpy(ctx, c0)
+                t, nfp = timed(lambda: cr.integrate_phased(ctx, g, phs, phased, 1.0, 1.0, dt, sub,
+                                                           n_steps // sub))
+                res["learned_fp64_phased"].append(t)
+                fallbacks_p.append(nfp)
             r, l, q = ref.numpy(ctx), lrn.numpy(ctx), f32.numpy(ctx).astype(np.float64)
             assert np.array_equal(ge, r), "one-member ensemble differs from plain RK4"
             w = wrm.numpy(ctx)
+            if phased is not None:
+                ph = phs.numpy(ctx)
+                errs_p.append(float(np.linalg.norm(ph - r) / np.linalg.norm(r)))
+                phs.free()
             errs.append(float(np.linalg.norm(l - r) / np.linalg.norm(r)))
             errs32.append(float(np.linalg.norm(q - r) / np.linalg.norm(r)))
             errs_w.append(float(np.linalg.norm(w - r) / np.linalg.norm(r)))
             for arr in (ref, f32, lrn, wrm):
                 arr.free()
         net.close()
+        for pn, _, _ in phased or []:
+            pn.close()
     med = {k: float(np.median(v)) for k, v in res.items()}
     cpu = None
     if not a.no_cpu:  # the oracle's RK4 (FP64, 1 thread) on a prefix, scaled to T
@@ -468,6 +489,18 @@ def learned(a):
         cpu = {"value": el * n_steps / 160, "unit": "s to T=20", "cores": 1, "kind": "port",
                "sample": f"CPU oracle RK4 (FP64, gcc -O2, 1 thread): the first 160 of 2000 steps "
                          f"({el:.2f} s), scaled"}
+    best_plain = min(med["plain_fp64"], med["plain_fp64_graph"])
+    variants = {"learned_fp64": max(errs), "learned_fp64_warm1": max(errs_w)}
+    if errs_p:
+        variants["learned_fp64_phased"] = max(errs_p)
+    phased_out = {}
+    if errs_p:
+        phased_out = {"rel_l2_learned_phased_vs_fp64": max(errs_p),
+                      "speedup_phased_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64_phased"],
+                      "speedup_phased_vs_plain_fp64_graph":
+                          med["plain_fp64_graph"] / med["learned_fp64_phased"],
+                      "fallbacks_phased": fallbacks_p,
+                      "phases": [[lo, hi] for _, lo, hi, _ in cr.PHASES], "phased_warm": cr.WARM}
     return {"metric": "learned vs plain time-to-T", "T": n_steps * dt, "grid": [H, W],
             "cpu_baseline": cpu,
             "macro_steps": n_steps // sub, "n_ic": len(seeds),
@@ -478,9 +511,10 @@ def learned(a):
             "rel_l2_learned_vs_fp64": max(errs), "rel_l2_plain_fp32_vs_fp64": max(errs32),
             "rel_l2_learned_warm1_vs_fp64": max(errs_w),
             "speedup_warm1_vs_plain_fp64": med["plain_fp64"] / med["learned_fp64_warm1"],
+            **phased_out,
             "fallbacks": fallbacks, "target_rel_l2": 1e-3, "plain_tile_ox_oy": tiles,
-            "target_met": bool(max(errs) <= 1e-3 and min(med["plain_fp64"], med["plain_fp64_graph"])
-                               / med["learned_fp64"] >= 5),
+            # any variant within the error bound at >= 5x the fastest plain path
+            "target_met": any(e <= 1e-3 and best_plain / med[k] >= 5 for k, e in variants.items()),
             "training": {"seconds": meta.get("seconds"), "final_loss": meta["history"][-1],
                          "pairs": "snapshot m -> m+1 of 2000 FP64 GPU RK4 trajectories"}}
 

@@ -9,6 +9,13 @@ a training pair is (snapshot m, snapshot m+1) of one trajectory.
 
     python -m gridcalc_amd.corrector  (or corrector.main()) trains and writes
     data/corrector_<H>x<W>.npy (FP32 parameters) and .json (scale, history, config).
+    corrector.main(["phases=1"]) then fine-tunes one copy of those parameters per phase of a
+    run (PHASES) and writes data/corrector_<H>x<W>_<phase>.npy / .json.
+
+Phase-specialised inference (integrate_phased): the first WARM macro steps by plain RK4,
+then each phase's network on its own range of macro steps.  The network is small; one set
+of weights fitted to every stage of the dynamics (fast early coarsening, slow late
+coarsening) leaves more one-step error than one set per stage.
 """
 from __future__ import annotations
 
@@ -37,13 +44,23 @@ CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000,
               seed=12345, skip_first=0, sample_pow=0.0)
 
 
-def params_path(H, W):
-    return os.path.join(DATA_DIR, f"corrector_{H}x{W}.npy")
+# Phases of a run, as half-open ranges of macro-step indices (macro step m advances the state
+# from t = 16 m dt): (name, first, end, fine-tuning stages (batches, iterations per batch,
+# learning rate max -> min, seed)).  WARM macro steps at the start are plain RK4.
+WARM = 1
+PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666),)),
+          ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555), (12, 2000, 1e-4, 1e-6, 77777))))
 
 
-def load_params(H=256, W=256):
-    """Trained parameters and metadata, or (None, None) if not trained."""
-    p = params_path(H, W)
+def params_path(H, W, phase=None):
+    tag = f"_{phase}" if phase else ""
+    return os.path.join(DATA_DIR, f"corrector_{H}x{W}{tag}.npy")
+
+
+def load_params(H=256, W=256, phase=None):
+    """Trained parameters and metadata (of one phase's fine-tuned copy if phase is given), or
+    (None, None) if not trained."""
+    p = params_path(H, W, phase)
     if not os.path.exists(p):
         return None, None
     meta = json.load(open(p[:-4] + ".json"))
@@ -188,6 +205,107 @@ def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
     return params, meta
 
 
+def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
+    """Continues training `params` on pairs whose first snapshot is macro step m_lo..m_hi-1
+    (uniformly), through `stages` = ((batches, iterations per batch, lr max, lr min, seed),
+    ...), each a cosine schedule over fresh trajectories.  Returns (params, loss history)."""
+    H, W = cfg["H"], cfg["W"]
+    history = []
+    t_start = time.perf_counter()
+    p = np.asarray(params, np.float32)
+    with Context() as ctx:
+        net = Corrector(ctx, H, W, cfg["minibatch"])
+        for n_batches, iters, lr_max, lr_min, seed in stages:
+            net.set_params(p, scale)  # fresh Adam state per stage
+            rng = np.random.default_rng(seed)
+            total = n_batches * iters
+            it = 0
+            for bi in range(n_batches):
+                data, n_macro = generate(ctx, cfg["batch_traj"], cfg["n_steps"], seed + 1000 * bi,
+                                         cfg, lambda *a: None)
+                per_traj = n_macro + 1
+
+                class _Shifted:  # device array view starting one snapshot later
+                    ptr = C.c_void_p(data.ptr.value + H * W * 4)
+                    gc_dtype = F32
+
+                losses = []
+                for _ in range(iters):
+                    tr = rng.integers(0, cfg["batch_traj"], cfg["minibatch"])
+                    m = rng.integers(m_lo, min(m_hi, n_macro), cfg["minibatch"])
+                    idx = (tr * per_traj + m).astype(np.int32)
+                    frac = it / max(1, total - 1)
+                    lr = lr_min + 0.5 * (lr_max - lr_min) * (1 + math.cos(math.pi * frac))
+                    losses.append(net.loss_grad(data, _Shifted, idx=idx))
+                    net.adam(lr)
+                    it += 1
+                data.free()
+                history.append(float(np.mean(losses)))
+                log(f"  [{m_lo}, {m_hi}) batch {bi + 1}/{n_batches}: mean loss {history[-1]:.4e}, "
+                    f"{time.perf_counter() - t_start:.0f} s")
+            p = net.params()
+        net.close()
+    return p, history
+
+
+def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR):
+    """Fine-tunes the trained parameters once per phase (PHASES) and writes them."""
+    H, W = cfg["H"], cfg["W"]
+    base, meta = load_params(H, W)
+    if base is None:
+        raise RuntimeError("train the corrector first")
+    for name, m_lo, m_hi, stages in PHASES:
+        t0 = time.perf_counter()
+        p, history = finetune(base, meta["scale"], m_lo, m_hi, stages, cfg, log)
+        path = os.path.join(out_dir, f"corrector_{H}x{W}_{name}.npy")
+        np.save(path, p)
+        json.dump(dict(scale=meta["scale"], history=history, macro_steps=[m_lo, m_hi],
+                       stages=[list(st) for st in stages], config=cfg,
+                       seconds=time.perf_counter() - t0), open(path[:-4] + ".json", "w"), indent=1)
+        log(f"  wrote {path}")
+
+
+def phase_nets(ctx, H=256, W=256):
+    """One Corrector per phase with its fine-tuned parameters: [(net, first, end), ...], or
+    None if the phase parameters are not there."""
+    nets = []
+    for name, m_lo, m_hi, _ in PHASES:
+        p, meta = load_params(H, W, name)
+        if p is None:
+            for n, _, _ in nets:
+                n.close()
+            return None
+        net = Corrector(ctx, H, W, 1)
+        net.set_params(p, meta["scale"])
+        nets.append((net, m_lo, m_hi))
+    return nets
+
+
+def integrate_phased(ctx, g, psi, nets, M, kappa, dt, substeps, n_macro, mass_tol=1e-6,
+                     warm=WARM):
+    """n_macro macro steps of substeps RK4 steps each: the first `warm` by plain RK4, then
+    every macro step m by the network of the phase whose range holds m (plain RK4 where no
+    phase does), each with the mass / free-energy fallback.  Returns the fallback count."""
+    m = 0
+    fallbacks = 0
+    w = min(warm, n_macro)
+    if w:
+        ch_integrate(ctx, g, psi, M, kappa, dt, w * substeps, RK4)
+        m = w
+    while m < n_macro:
+        cover = [(net, hi) for net, lo, hi in nets if lo <= m < hi]
+        if cover:
+            net, hi = cover[0]
+            n = min(hi, n_macro) - m
+            fallbacks += net.integrate(g, psi, M, kappa, dt, substeps, n, mass_tol)
+        else:
+            nxt = min([lo for _, lo, _ in nets if lo > m] + [n_macro])
+            n = nxt - m
+            ch_integrate(ctx, g, psi, M, kappa, dt, n * substeps, RK4)
+        m += n
+    return fallbacks
+
+
 def one_step_errors(net, cfg=CONFIG, n_traj=8, seed=777001, log=print):
     """Relative L2 error of one macro step (substeps RK4 steps) on held-out trajectories:
     returns (network, identity) where identity is the error of c(t+16 dt) ~ c(t), i.e. of the
@@ -252,17 +370,24 @@ def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6,
 
 
 def main(argv=None):
-    """key=value arguments override CONFIG; out=<dir> changes the output directory."""
+    """key=value arguments override CONFIG; out=<dir> changes the output directory;
+    phases=1 fine-tunes the per-phase copies of the trained parameters instead of training."""
     argv = sys.argv[1:] if argv is None else argv
     cfg = dict(CONFIG)
     out_dir = DATA_DIR
+    phases = False
     for a in argv:
         k, v = a.split("=", 1)
         if k == "out":
             out_dir = v
+        elif k == "phases":
+            phases = v == "1"
         else:
             cfg[k] = type(CONFIG[k])(v)
-    train(cfg, out_dir=out_dir)
+    if phases:
+        train_phases(cfg, out_dir=out_dir)
+    else:
+        train(cfg, out_dir=out_dir)
 
 
 if __name__ == "__main__":

@@ -167,3 +167,69 @@ def test_trained_learned_integration(gc, ctx, trained):
     assert r["mass_drift"] <= 1e-6
     assert r["rel_l2"] < 2e-2
     assert r["fallbacks"] < r["n_macro"]
+
+
+def test_phased_all_warm_is_plain_rk4(gc, ctx):
+    """integrate_phased with every macro step inside the warm-up is plain RK4, bit for bit."""
+    from gridcalc_amd import corrector as cr
+    g = gc.Grid.make(W, H, 1)
+    c0 = _ic(gc, 21)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    b = gc.DeviceArray.from_numpy(ctx, c0)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(_zero_net(gc), 1.0)
+    assert cr.integrate_phased(ctx, g, a, [(net, 0, 100)], 1.0, 1.0, DT, 16, 3, warm=3) == 0
+    gc.ch_integrate(ctx, g, b, 1.0, 1.0, DT, 48, gc.RK4)
+    assert np.array_equal(a.numpy(ctx), b.numpy(ctx))
+    net.close()
+
+
+def test_phased_schedule(gc, ctx):
+    """Networks that propose no change on macro steps [2, 4) and [5, 7) of 8, one warm step:
+    the state equals plain RK4 over the other four macro steps (0, 1, 4, 7), so every step
+    outside the phases is plain RK4 and every step inside is the phase's network."""
+    from gridcalc_amd import corrector as cr
+    g = gc.Grid.make(W, H, 1)
+    c0 = _ic(gc, 22)
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    b = gc.DeviceArray.from_numpy(ctx, c0)
+    nets = []
+    for lo, hi in ((2, 4), (5, 7)):
+        n = gc.Corrector(ctx, H, W, 1)
+        n.set_params(_zero_net(gc), 1.0)
+        nets.append((n, lo, hi))
+    assert cr.integrate_phased(ctx, g, a, nets, 1.0, 1.0, DT, 16, 8, warm=1) == 0
+    gc.ch_integrate(ctx, g, b, 1.0, 1.0, DT, 4 * 16, gc.RK4)
+    assert np.array_equal(a.numpy(ctx), b.numpy(ctx))
+    for n, _, _ in nets:
+        n.close()
+
+
+def test_phased_trained_networks_beat_single(gc, ctx, trained):
+    """T = 20 from a held-out initial state: the phase-specialised networks (first macro step
+    plain) are more accurate than the single trained network with the same plain first step,
+    and conserve mass."""
+    cr, p, meta = trained
+    nets = cr.phase_nets(ctx, H, W)
+    if nets is None:
+        pytest.skip("no phase parameters in the package data")
+    g = gc.Grid.make(W, H, 1)
+    c0 = cr.initial_state(H, W, 991007).astype(np.float64)
+    ref = gc.DeviceArray.from_numpy(ctx, c0)
+    ph = gc.DeviceArray.from_numpy(ctx, c0)
+    one = gc.DeviceArray.from_numpy(ctx, c0)
+    gc.ch_integrate(ctx, g, ref, 1.0, 1.0, DT, 2000, gc.RK4)
+    nf = cr.integrate_phased(ctx, g, ph, nets, 1.0, 1.0, DT, 16, 125)
+    net = gc.Corrector(ctx, H, W, 1)
+    net.set_params(p, meta["scale"])
+    gc.ch_integrate(ctx, g, one, 1.0, 1.0, DT, 16, gc.RK4)
+    net.integrate(g, one, 1.0, 1.0, DT, 16, 124)
+    r, a, b = ref.numpy(ctx), ph.numpy(ctx), one.numpy(ctx)
+    e_ph = np.linalg.norm(a - r) / np.linalg.norm(r)
+    e_one = np.linalg.norm(b - r) / np.linalg.norm(r)
+    print(f"T=20 rel L2: phased {e_ph:.3e}, single network {e_one:.3e}, fallbacks {nf}")
+    assert e_ph < 0.9 * e_one
+    assert abs(a.mean() - c0.mean()) <= 1e-6
+    net.close()
+    for n, _, _ in nets:
+        n.close()
