@@ -46,10 +46,15 @@ CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000,
 
 # Phases of a run, as half-open ranges of macro-step indices (macro step m advances the state
 # from t = 16 m dt): (name, first, end, fine-tuning stages (batches, iterations per batch,
-# learning rate max -> min, seed)).  WARM macro steps at the start are plain RK4.
+# learning rate max -> min, seed, pushforward period)).  WARM macro steps at the start are
+# plain RK4.  With pushforward period k > 0, every k-th iteration from the second on trains
+# on (the network's own one-step prediction from snapshot m, snapshot m + 2) instead of
+# (snapshot m, snapshot m + 1): the network learns to advance the states its own errors lead
+# to, which cuts the error that accumulates over the slow late phase.
 WARM = 1
-PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666),)),
-          ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555), (12, 2000, 1e-4, 1e-6, 77777))))
+PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666, 0),)),
+          ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555, 0), (12, 2000, 1e-4, 1e-6, 77777, 0),
+                             (20, 2000, 1e-4, 1e-6, 133331, 2))))
 
 
 def params_path(H, W, phase=None):
@@ -207,15 +212,24 @@ def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
 
 def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
     """Continues training `params` on pairs whose first snapshot is macro step m_lo..m_hi-1
-    (uniformly), through `stages` = ((batches, iterations per batch, lr max, lr min, seed),
-    ...), each a cosine schedule over fresh trajectories.  Returns (params, loss history)."""
-    H, W = cfg["H"], cfg["W"]
+    (uniformly), through `stages` = ((batches, iterations per batch, lr max, lr min, seed,
+    pushforward period), ...), each a cosine schedule over fresh trajectories (see PHASES for
+    the pushforward iterations).  Returns (params, loss history)."""
+    H, W, mb = cfg["H"], cfg["W"], cfg["minibatch"]
+    hw = H * W
     history = []
     t_start = time.perf_counter()
     p = np.asarray(params, np.float32)
     with Context() as ctx:
-        net = Corrector(ctx, H, W, cfg["minibatch"])
-        for n_batches, iters, lr_max, lr_min, seed in stages:
+        net = Corrector(ctx, H, W, mb)
+        xs, ys, ts = (DeviceArray((mb, H, W), np.float32) for _ in range(3))
+
+        def gather(dst, data, rows):  # snapshots `rows` of data into the images of dst
+            for b, r in enumerate(rows):
+                _check(lib().gc_memcpy_dtod(ctx.handle, C.c_void_p(dst.ptr.value + b * hw * 4),
+                                            C.c_void_p(data.ptr.value + int(r) * hw * 4), hw * 4))
+
+        for n_batches, iters, lr_max, lr_min, seed, pf in stages:
             net.set_params(p, scale)  # fresh Adam state per stage
             rng = np.random.default_rng(seed)
             total = n_batches * iters
@@ -231,12 +245,19 @@ def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
 
                 losses = []
                 for _ in range(iters):
-                    tr = rng.integers(0, cfg["batch_traj"], cfg["minibatch"])
-                    m = rng.integers(m_lo, min(m_hi, n_macro), cfg["minibatch"])
-                    idx = (tr * per_traj + m).astype(np.int32)
                     frac = it / max(1, total - 1)
                     lr = lr_min + 0.5 * (lr_max - lr_min) * (1 + math.cos(math.pi * frac))
-                    losses.append(net.loss_grad(data, _Shifted, idx=idx))
+                    tr = rng.integers(0, cfg["batch_traj"], mb)
+                    if pf and it % pf:  # pushforward: net(snapshot m) -> snapshot m + 2
+                        m = rng.integers(m_lo, min(m_hi, n_macro - 1), mb)
+                        gather(xs, data, tr * per_traj + m)
+                        gather(ts, data, tr * per_traj + m + 2)
+                        net.predict(xs, ys, mb)
+                        losses.append(net.loss_grad(ys, ts, B=mb))
+                    else:
+                        m = rng.integers(m_lo, min(m_hi, n_macro), mb)
+                        idx = (tr * per_traj + m).astype(np.int32)
+                        losses.append(net.loss_grad(data, _Shifted, idx=idx))
                     net.adam(lr)
                     it += 1
                 data.free()
@@ -245,16 +266,21 @@ def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
                     f"{time.perf_counter() - t_start:.0f} s")
             p = net.params()
         net.close()
+        for a in (xs, ys, ts):
+            a.free()
     return p, history
 
 
-def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR):
-    """Fine-tunes the trained parameters once per phase (PHASES) and writes them."""
+def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR, only=None):
+    """Fine-tunes the trained parameters once per phase (PHASES, or those named in `only`)
+    and writes them."""
     H, W = cfg["H"], cfg["W"]
     base, meta = load_params(H, W)
     if base is None:
         raise RuntimeError("train the corrector first")
     for name, m_lo, m_hi, stages in PHASES:
+        if only is not None and name not in only:
+            continue
         t0 = time.perf_counter()
         p, history = finetune(base, meta["scale"], m_lo, m_hi, stages, cfg, log)
         path = os.path.join(out_dir, f"corrector_{H}x{W}_{name}.npy")
@@ -371,21 +397,25 @@ def integration_error(net, cfg=CONFIG, seed=991001, n_steps=None, mass_tol=1e-6,
 
 def main(argv=None):
     """key=value arguments override CONFIG; out=<dir> changes the output directory;
-    phases=1 fine-tunes the per-phase copies of the trained parameters instead of training."""
+    phases=1 fine-tunes the per-phase copies of the trained parameters instead of training
+    (only=a,b: just those phases)."""
     argv = sys.argv[1:] if argv is None else argv
     cfg = dict(CONFIG)
     out_dir = DATA_DIR
     phases = False
+    only = None
     for a in argv:
         k, v = a.split("=", 1)
         if k == "out":
             out_dir = v
         elif k == "phases":
             phases = v == "1"
+        elif k == "only":
+            only = v.split(",")
         else:
             cfg[k] = type(CONFIG[k])(v)
     if phases:
-        train_phases(cfg, out_dir=out_dir)
+        train_phases(cfg, out_dir=out_dir, only=only)
     else:
         train(cfg, out_dir=out_dir)
 
