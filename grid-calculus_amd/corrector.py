@@ -54,7 +54,7 @@ CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000,
 WARM = 1
 PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666, 0),)),
           ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555, 0), (12, 2000, 1e-4, 1e-6, 77777, 0),
-                             (20, 2000, 1e-4, 1e-6, 133331, 2))))
+                             (20, 2000, 1e-4, 1e-6, 133331, 2), (16, 2000, 5e-5, 5e-7, 155551, 2))))
 
 
 def params_path(H, W, phase=None):
@@ -273,7 +273,10 @@ def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
 
 def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR, only=None):
     """Fine-tunes the trained parameters once per phase (PHASES, or those named in `only`)
-    and writes them."""
+    and writes them.  A phase whose shipped parameters were made by a prefix of its declared
+    stages continues from them with the remaining stages (each stage starts from the previous
+    stage's parameters with a fresh optimiser state and its own seed, so the result is that of
+    running every stage)."""
     H, W = cfg["H"], cfg["W"]
     base, meta = load_params(H, W)
     if base is None:
@@ -282,12 +285,22 @@ def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR, only=None):
         if only is not None and name not in only:
             continue
         t0 = time.perf_counter()
-        p, history = finetune(base, meta["scale"], m_lo, m_hi, stages, cfg, log)
+        start, done, hist0, sec0 = base, 0, [], 0.0
+        prev, pmeta = load_params(H, W, name)
+        if prev is not None and pmeta.get("config") == cfg:
+            k = len(pmeta["stages"])
+            if 0 < k < len(stages) and pmeta["stages"] == [list(st) for st in stages[:k]]:
+                start, done, hist0 = prev, k, list(pmeta["history"])
+                sec0 = float(pmeta.get("seconds", 0.0))
+                log(f"  {name}: continuing after stage {k} of {len(stages)}")
+        p, history = finetune(start, meta["scale"], m_lo, m_hi, stages[done:], cfg, log)
+        history = hist0 + history
         path = os.path.join(out_dir, f"corrector_{H}x{W}_{name}.npy")
         np.save(path, p)
         json.dump(dict(scale=meta["scale"], history=history, macro_steps=[m_lo, m_hi],
                        stages=[list(st) for st in stages], config=cfg,
-                       seconds=time.perf_counter() - t0), open(path[:-4] + ".json", "w"), indent=1)
+                       seconds=sec0 + time.perf_counter() - t0), open(path[:-4] + ".json", "w"),
+                  indent=1)
         log(f"  wrote {path}")
 
 
