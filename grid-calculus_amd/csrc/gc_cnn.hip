@@ -68,40 +68,61 @@ __device__ __forceinline__ float gelu_grad_f(float z) {
 
 __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// ---- layer 1 (1 -> 32), vector ALU -------------------------------------------------------
-// One thread per site; the state and W1 enter in FP32 (a BF16-rounded input would carry
-// noise of 2^-9 |c| that the network answers as if it were part of the state).
+// ---- layer 1 (1 -> 32), matrix cores in FP32 ----------------------------------------------
+// The state and W1 enter in FP32 (a BF16-rounded input would carry noise of 2^-9 |c| that the
+// network answers as if it were part of the state).  v_mfma_f32_32x32x2f32: M = 32 output
+// channels, N = 32 sites, K = the 9 taps and a zero tenth, 2 per instruction; lane (h, j)
+// (h = lane / 32) supplies W1[j][2 kk + h] and tap 2 kk + h of site j.  The accumulator
+// layout is that of the 32 -> 32 layers: a lane ends with channels 8 g + 4 h + i of site j
+// in acc[4 g + i].
+__device__ __forceinline__ void conv1_weights(const float* __restrict__ w1, int lane, float (&wa)[5]) {
+  const int j = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    const int tap = 2 * kk + h;
+    wa[kk] = tap < 9 ? w1[j * 9 + tap] : 0.0f;
+  }
+}
+template <class X>
+__device__ __forceinline__ f32x16 conv1_mfma(const float (&wa)[5], int lane, const X& x) {
+  const int h = lane >> 5;
+  f32x16 acc = {};
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    const int tap = 2 * kk + h;
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[kk], tap < 9 ? x(tap) : 0.0f, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Block: 4 waves, each 32 consecutive sites of one row (W is a multiple of 32).
 template <class TS>
 __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const int* __restrict__ idx,
                                                const float* __restrict__ w1,
                                                const float* __restrict__ b1, float* __restrict__ z1,
                                                __bf16* __restrict__ a1, int H, int W, int B) {
   const int HW = H * W;
-  const long s = (long)blockIdx.x * 256 + threadIdx.x;
-  if (s >= (long)B * HW) return;
+  const int lane = threadIdx.x & 63, j = lane & 31, cq = (lane >> 5) * 4;
+  const long s0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  if (s0 >= (long)B * HW) return;
+  const long s = s0 + j;
   const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
   const TS* img = c + (size_t)(idx ? idx[b] : b) * HW;
-  float xv[9];
+  float wa[5];
+  conv1_weights(w1, lane, wa);
+  const f32x16 acc = conv1_mfma(wa, lane, [&](int tap) {
+    return (float)img[wrap1(y + tap / 3 - 1, H) * W + wrap1(x + tap % 3 - 1, W)];
+  });
 #pragma unroll
-  for (int t = 0; t < 9; ++t)
-    xv[t] = (float)img[wrap1(y + t / 3 - 1, H) * W + wrap1(x + t % 3 - 1, W)];
+  for (int g = 0; g < 4; ++g) {
+    float v[4];
+    bf16x4 av;
 #pragma unroll
-  for (int co = 0; co < kC; co += 8) {
-    float zz[8];
+    for (int i = 0; i < 4; ++i) v[i] = acc[4 * g + i] + b1[g * 8 + cq + i];
+    if (z1) *(float4*)(z1 + (size_t)s * kC + g * 8 + cq) = make_float4(v[0], v[1], v[2], v[3]);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      float acc = 0.0f;
-#pragma unroll
-      for (int t = 0; t < 9; ++t) acc = fmaf(w1[(co + u) * 9 + t], xv[t], acc);
-      zz[u] = acc + b1[co + u];
-    }
-    bf16x8 av;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (z1) z1[(size_t)s * kC + co + u] = zz[u];
-      av[u] = (__bf16)gelu_f(zz[u]);
-    }
-    *(bf16x8*)(a1 + (size_t)s * kC + co) = av;
+    for (int i = 0; i < 4; ++i) av[i] = (__bf16)gelu_f(v[i]);
+    *(bf16x4*)(a1 + (size_t)s * kC + g * 8 + cq) = av;
   }
 }
 
@@ -483,25 +504,35 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   __syncthreads();
   if constexpr (APPLY)
     if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
-  // layer 1 on the halo-2 region (kFR + 4) x (kFC + 4)
+  // layer 1 on the halo-2 region (kFR + 4) x (kFC + 4), in 32-site chunks like layers 2..5
   {
-    constexpr int RW = kFC + 4, RN = kAN1;
-    for (int q = t; q < RN; q += kFNT) {
-      const int ry = q / RW, rx = q % RW;
-      float xv[9];
+    constexpr int RW = kFC + 4, RN = kAN1, nch = (RN + 31) / 32;
+    const int j = lane & 31, cq = (lane >> 5) * 4;
+    float wa[5];
+    conv1_weights(w1s, lane, wa);
+    auto chunk = [&](int cc) {
+      const int sr = min(cc * 32 + j, RN - 1), ry = sr / RW, rx = sr % RW;
+      return conv1_mfma(wa, lane, [&](int tap) { return cs[(ry + tap / 3) * kAW0 + rx + tap % 3]; });
+    };
+    auto emit1 = [&](const f32x16& acc, int sr) {
+      if (sr < RN) {
 #pragma unroll
-      for (int tp = 0; tp < 9; ++tp) xv[tp] = cs[(ry + tp / 3) * kAW0 + rx + tp % 3];
-#pragma unroll 1
-      for (int co = 0; co < kC; co += 8) {
-        bf16x8 av;
+        for (int g = 0; g < 4; ++g) {
+          bf16x4 av;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          float acc = 0.0f;
-#pragma unroll
-          for (int tp = 0; tp < 9; ++tp) acc = fmaf(w1s[(co + u) * 9 + tp], xv[tp], acc);
-          av[u] = (__bf16)gelu_f(acc + w1s[288 + co + u]);
+          for (int i = 0; i < 4; ++i) av[i] = (__bf16)gelu_f(acc[4 * g + i] + w1s[288 + g * 8 + cq + i]);
+          *(bf16x4*)(bufA + sr * kSP + g * 8 + cq) = av;
         }
-        *(bf16x8*)(bufA + q * kSP + co) = av;
+      }
+    };
+    for (int cc = wave; cc < nch; cc += 16) {
+      const f32x16 acc0 = chunk(cc);
+      if (cc + 8 < nch) {
+        const f32x16 acc1 = chunk(cc + 8);
+        emit1(acc0, cc * 32 + j);
+        emit1(acc1, (cc + 8) * 32 + j);
+      } else {
+        emit1(acc0, cc * 32 + j);
       }
     }
   }
@@ -966,7 +997,7 @@ template <class TS>
 gc_status forward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool train) {
   if (!n->packed) GC_TRY(pack(n));
   const long ns = (long)B * n->H * n->W;
-  const unsigned g1 = (unsigned)((ns + 255) / 256);
+  const unsigned g1 = (unsigned)((ns + 127) / 128);  // 128 sites per block
   hipStream_t st = n->ctx->stream;
   hipLaunchKernelGGL((k_conv1<TS>), dim3(g1), dim3(256), 0, st, c, d_idx, n->p + kOffW1, n->p + kOffB1,
                      train ? n->z[0] : nullptr, n->a[0], n->H, n->W, B);
