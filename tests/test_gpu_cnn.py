@@ -147,3 +147,16 @@ def test_fused_inference_equals_layer_kernels(gc, cnn, ctx, dtype, shape):
     layers = net.forward(d_c, B, layers=True)
     assert np.array_equal(fused, layers)
     net.close()
+
+
+def test_finetune_with_pushforward_runs(gc, ctx):
+    """corrector.finetune on a small configuration (4 trajectories of 64 steps), one stage
+    with pushforward iterations (every second iteration trains on the network's own one-step
+    prediction): returns finite parameters that moved, and one loss per batch."""
+    from gridcalc_amd import corrector as cr
+    cfg = dict(cr.CONFIG, batch_traj=4, n_steps=64)
+    p0 = cr.init_params(5)
+    p, hist = cr.finetune(p0, 0.01, 0, 3, ((2, 4, 1e-4, 1e-5, 11, 2),), cfg, log=lambda *a: None)
+    assert p.shape == p0.shape and np.isfinite(p).all()
+    assert not np.array_equal(p, p0)
+    assert len(hist) == 2 and all(np.isfinite(h) and h > 0 for h in hist)
