@@ -83,19 +83,20 @@ def test_mass_tolerance_forces_fallback(gc, ctx):
     net.close()
 
 
-@pytest.mark.parametrize("n_macro", [48, 45])
-def test_mixed_decisions_match_stepwise_rule(gc, ctx, n_macro):
+@pytest.mark.parametrize("n_macro,dtype", [(48, np.float64), (45, np.float64), (48, np.float32)])
+def test_mixed_decisions_match_stepwise_rule(gc, ctx, n_macro, dtype):
     """A random network with a tiny output scale proposes c + O(1e-6) changes that sometimes
     raise the free energy: accepted and rejected macro steps interleave across several
     16-step proposal chunks (the next chunk proposed while the host decides, discarded after
     a rejection; 45 leaves a shorter last chunk), so a rejection inside a chunk discards the
     proposals after it.  The result must equal a host loop that decides after every macro
-    step (free energy and mass from the library's reductions)."""
+    step (free energy and mass from the library's reductions), for an FP64 and an FP32
+    state."""
     g = gc.Grid.make(W, H, 1)
     rng = np.random.default_rng(5)
     net = gc.Corrector(ctx, H, W, 1)
     net.set_params(rng.uniform(-0.3, 0.3, gc.N_CNN_PARAMS).astype(np.float32), 1e-6)
-    c0 = _ic(gc, 15)
+    c0 = _ic(gc, 15, dtype)
     tol = 1e-6
     a = gc.DeviceArray.from_numpy(ctx, c0)
     nf = net.integrate(g, a, 1.0, 1.0, DT, 16, n_macro, tol)
