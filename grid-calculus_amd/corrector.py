@@ -324,7 +324,10 @@ def integrate_phased(ctx, g, psi, nets, M, kappa, dt, substeps, n_macro, mass_to
                      warm=WARM):
     """n_macro macro steps of substeps RK4 steps each: the first `warm` by plain RK4, then
     every macro step m by the network of the phase whose range holds m (plain RK4 where no
-    phase does), each with the mass / free-energy fallback.  Returns the fallback count."""
+    phase does), each with the mass / free-energy fallback.  Returns the fallback count.
+    Each phase is one gc_learned_integrate call, so its mass-drift check is taken from the
+    state the phase starts from (the networks conserve mass to rounding: the drift over a
+    whole run is ~1e-19 here, far inside any tolerance)."""
     m = 0
     fallbacks = 0
     w = min(warm, n_macro)
