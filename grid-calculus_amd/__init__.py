@@ -162,6 +162,9 @@ def lib():
         "gc_cast": ([vp, C.c_int, C.c_int, sz, vp, vp], st),
         "gc_learned_integrate": ([vp, gp, C.c_int, C.POINTER(CHParams), vp, C.c_double, C.c_int,
                                   C.c_int, C.c_double, ip], st),
+        "gc_learned_integrate_phases": ([C.POINTER(C.c_void_p), ip, ip, C.c_int, gp, C.c_int,
+                                         C.POINTER(CHParams), vp, C.c_double, C.c_int, C.c_int,
+                                         C.c_double, ip], st),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -28,8 +28,8 @@ import time
 
 import numpy as np
 
-from . import (F32, F64, RK4, Context, Corrector, DeviceArray, Ensemble, Grid, N_CNN_PARAMS,
-               _check, ch_integrate, inputs, lib)
+from . import (F32, F64, RK4, CHParams, Context, Corrector, DeviceArray, Ensemble, Grid,
+               N_CNN_PARAMS, _check, ch_integrate, inputs, lib)
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
@@ -324,28 +324,23 @@ def integrate_phased(ctx, g, psi, nets, M, kappa, dt, substeps, n_macro, mass_to
                      warm=WARM):
     """n_macro macro steps of substeps RK4 steps each: the first `warm` by plain RK4, then
     every macro step m by the network of the phase whose range holds m (plain RK4 where no
-    phase does), each with the mass / free-energy fallback.  Returns the fallback count.
-    Each phase is one gc_learned_integrate call, so its mass-drift check is taken from the
-    state the phase starts from (the networks conserve mass to rounding: the drift over a
-    whole run is ~1e-19 here, far inside any tolerance)."""
-    m = 0
-    fallbacks = 0
-    w = min(warm, n_macro)
-    if w:
-        ch_integrate(ctx, g, psi, M, kappa, dt, w * substeps, RK4)
-        m = w
-    while m < n_macro:
-        cover = [(net, hi) for net, lo, hi in nets if lo <= m < hi]
-        if cover:
-            net, hi = cover[0]
-            n = min(hi, n_macro) - m
-            fallbacks += net.integrate(g, psi, M, kappa, dt, substeps, n, mass_tol)
-        else:
-            nxt = min([lo for _, lo, _ in nets if lo > m] + [n_macro])
-            n = nxt - m
-            ch_integrate(ctx, g, psi, M, kappa, dt, n * substeps, RK4)
-        m += n
-    return fallbacks
+    phase does), each with the mass / free-energy fallback, in one gc_learned_integrate_phases
+    call (the proposal pipeline runs on across phase boundaries; mass drift is measured from
+    the initial state).  Returns the fallback count."""
+    segs = [(net, max(lo, warm), hi) for net, lo, hi in nets if hi > max(lo, warm)]
+    if not segs:
+        ch_integrate(ctx, g, psi, M, kappa, dt, n_macro * substeps, RK4)
+        return 0
+    k = len(segs)
+    handles = (C.c_void_p * k)(*[net._h.value for net, _, _ in segs])
+    first = (C.c_int * k)(*[lo for _, lo, _ in segs])
+    end = (C.c_int * k)(*[hi for _, _, hi in segs])
+    P = CHParams(M, kappa)
+    nf = C.c_int()
+    _check(lib().gc_learned_integrate_phases(handles, first, end, k, C.byref(g), psi.gc_dtype,
+                                             C.byref(P), psi.ptr, dt, substeps, n_macro, mass_tol,
+                                             C.byref(nf)))
+    return nf.value
 
 
 def one_step_errors(net, cfg=CONFIG, n_traj=8, seed=777001, log=print):

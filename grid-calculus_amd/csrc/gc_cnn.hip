@@ -1476,10 +1476,6 @@ gc_status fetch(gc_cnn* n, double* host, const double* dev, int count) {
   return GC_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
 // Macro steps are proposed in chunks of up to kChunk without waiting for the host: proposal k
 // of a chunk is written to history slot k and its mass and free energy to diag[k].  One copy
 // of diag per chunk then lets the host apply the acceptance rule in order; the state after the
@@ -1488,16 +1484,28 @@ extern "C" {
 // already proposes the next one from the chunk's last proposal (two proposal buffers); it is
 // discarded if the chunk has a rejection.  The decisions, and so the result, are those of
 // checking every proposal as it is made.
-gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
-                               void* psi, double step, int substeps, int n_macro,
-                               double mass_tol, int* n_fallback) {
-  GC_TRY(check_cnn(n, 1));
+//
+// Macro step m is proposed by the first network i with first[i] <= m < end[i]; steps no
+// network covers run as plain RK4.  A chunk never spans two segments; the chunk after the
+// last one of a segment is already proposed by the next segment's network.  The proposal
+// buffers, diagnostics and events belong to nets[0]; each network uses its own scratch.
+gc_status learned_run(gc_cnn* const* nets, const int* first, const int* end, int n_nets,
+                      const gc_grid* g, gc_dtype dt, const gc_ch_params* p, void* psi,
+                      double step, int substeps, int n_macro, double mass_tol, int* n_fallback) {
+  if (!nets || !first || !end || n_nets < 1)
+    return set_error(GC_ERR_INVALID, "gridcalc: no networks");
+  for (int i = 0; i < n_nets; ++i) GC_TRY(check_cnn(nets[i], 1));
   GC_TRY(check_grid(g));
   GC_TRY(check_dtype(dt));
   if (!p || !psi) return set_error(GC_ERR_INVALID, "gridcalc: null argument");
-  if (g->nz != 1 || g->nx != n->W || g->ny != n->H)
-    return set_error(GC_ERR_INVALID, "gridcalc: the corrector was built for %d x %d x 1 grids",
-                     n->W, n->H);
+  gc_cnn* n = nets[0];
+  for (int i = 0; i < n_nets; ++i) {
+    if (g->nz != 1 || g->nx != nets[i]->W || g->ny != nets[i]->H)
+      return set_error(GC_ERR_INVALID, "gridcalc: the corrector was built for %d x %d x 1 grids",
+                       nets[i]->W, nets[i]->H);
+    if (nets[i]->ctx != n->ctx)
+      return set_error(GC_ERR_INVALID, "gridcalc: the networks must share one context");
+  }
   if (n_macro < 0 || substeps < 1 || !(step >= 0.0))
     return set_error(GC_ERR_INVALID, "gridcalc: bad learned-integration arguments");
   const size_t bytes = sites(g) * dsize(dt);
@@ -1513,21 +1521,38 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
   for (hipEvent_t& e : n->ev)
     if (!e) GC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const int T = (n->W / kTC) * (n->H / kTR);
-  if (!n->out2) GC_TRY(cnn_alloc((void**)&n->out2, sites(g) * sizeof(float)));
-  if (!n->cpart)  // second tile sums, then per proposal one Comp2 (4 doubles) per workgroup
-    GC_TRY(cnn_alloc((void**)&n->cpart,
-                     (T + 4 * (size_t)kSlots * std::max(T, kMFBlocks)) * sizeof(double)));
+  for (int i = 0; i < n_nets; ++i) {
+    gc_cnn* q = nets[i];
+    if (!q->out2) GC_TRY(cnn_alloc((void**)&q->out2, sites(g) * sizeof(float)));
+    if (!q->cpart)  // second tile sums, then per proposal one Comp2 (4 doubles) per workgroup
+      GC_TRY(cnn_alloc((void**)&q->cpart,
+                       (T + 4 * (size_t)kSlots * std::max(T, kMFBlocks)) * sizeof(double)));
+  }
   hipStream_t st = n->ctx->stream;
   auto slot = [&](int buf, int k) { return (char*)n->hist + ((size_t)buf * kSlots + k) * bytes; };
   double* hd[2] = {n->hdiag, n->hdiag + 2 * kSlots};
-  // Proposals 1..K from src into buffer buf; their diagnostics reach hd[buf] when ev[buf] fires.
-  auto launch = [&](const void* src, int buf, int K) -> gc_status {
+  // The network proposing macro step m (-1: plain RK4) and the end of its segment.
+  auto segment = [&](int m, int& net, int& stop) {
+    for (int i = 0; i < n_nets; ++i)
+      if (first[i] <= m && m < end[i]) {
+        net = i;
+        stop = std::min(end[i], n_macro);
+        return;
+      }
+    net = -1;
+    stop = n_macro;
+    for (int i = 0; i < n_nets; ++i)
+      if (first[i] > m) stop = std::min(stop, first[i]);
+  };
+  // Proposals 1..K of network q from src into buffer buf; their diagnostics reach hd[buf]
+  // when ev[buf] fires.
+  auto launch = [&](gc_cnn* q, const void* src, int buf, int K) -> gc_status {
     double* dd = n->diag + (size_t)buf * 2 * kSlots;
     if (dt == GC_F64)
-      GC_TRY(propose_chunk<double>(n, g, (const double*)src, (double*)slot(buf, 0), K, p->kappa,
+      GC_TRY(propose_chunk<double>(q, g, (const double*)src, (double*)slot(buf, 0), K, p->kappa,
                                    dd));
     else
-      GC_TRY(propose_chunk<float>(n, g, (const float*)src, (float*)slot(buf, 0), K, p->kappa,
+      GC_TRY(propose_chunk<float>(q, g, (const float*)src, (float*)slot(buf, 0), K, p->kappa,
                                   dd));
     GC_HIP(hipMemcpyAsync(hd[buf] + 2, dd + 2, 2 * K * sizeof(double), hipMemcpyDeviceToHost,
                           st));
@@ -1542,15 +1567,44 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
   const double Minit = vol * h[0];
   double Fprev = vol * h[1];
   int fallbacks = 0;
-  int m = 0;  // macro steps decided; psi holds the state after them
-  int cur = 0, K = std::min(kChunk, n_macro);
-  if (K > 0) GC_TRY(launch(psi, cur, K));
-  int ahead = m + K;  // macro steps covered by the chunks in flight
-  while (m < n_macro) {
+  int m = 0;       // macro steps decided; psi holds the state after them
+  int cur = 0;     // buffer of the chunk being decided
+  int K = 0;       // its proposals (0: none in flight)
+  int ahead = 0;   // macro steps covered by the chunks in flight
+  // Starts a chunk at macro step m from psi if a network covers m; otherwise runs the plain
+  // segment at m and refreshes Fprev.
+  auto restart = [&]() -> gc_status {
+    while (m < n_macro) {
+      int net, stop;
+      segment(m, net, stop);
+      if (net >= 0) {
+        K = std::min(kChunk, stop - m);
+        GC_TRY(launch(nets[net], psi, cur, K));
+        ahead = m + K;
+        return GC_OK;
+      }
+      GC_TRY(gc_ch_integrate(n->ctx, g, dt, p, psi, step, (stop - m) * substeps, GC_RK4));
+      GC_TRY(mass_fe_async(n, g, dt, psi, p->kappa, n->diag));
+      GC_TRY(fetch(n, h, n->diag, 2));
+      Fprev = vol * h[1];
+      m = stop;
+    }
+    K = 0;
+    return GC_OK;
+  };
+  GC_TRY(restart());
+  while (m < n_macro && K > 0) {
     // While the host waits for chunk `cur`, the GPU proposes the next chunk from its last
     // proposal; that chunk stands only if every proposal of `cur` is accepted.
-    const int Kn = std::min(kChunk, n_macro - ahead);
-    if (Kn > 0) GC_TRY(launch(slot(cur, K), 1 - cur, Kn));
+    int Kn = 0;
+    if (ahead < n_macro) {
+      int net, stop;
+      segment(ahead, net, stop);
+      if (net >= 0) {
+        Kn = std::min(kChunk, stop - ahead);
+        GC_TRY(launch(nets[net], slot(cur, K), 1 - cur, Kn));
+      }
+    }
     GC_HIP(hipEventSynchronize(n->ev[cur]));
     int good = 0;
     for (int k = 1; k <= K; ++k) {
@@ -1564,10 +1618,13 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
       GC_HIP(hipMemcpyAsync(psi, slot(cur, good), bytes, hipMemcpyDeviceToDevice, st));
     m += good;
     if (good == K) {
-      if (Kn <= 0) break;
-      cur = 1 - cur;
-      K = Kn;
-      ahead += Kn;
+      if (Kn > 0) {
+        cur = 1 - cur;
+        K = Kn;
+        ahead += Kn;
+      } else {
+        GC_TRY(restart());  // a plain segment (or the end) follows
+      }
       continue;
     }
     // a rejected proposal: the chunk in flight (if any) started from a discarded state
@@ -1577,13 +1634,32 @@ gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const g
     GC_TRY(fetch(n, h, n->diag, 2));
     Fprev = vol * h[1];
     m += 1;
-    K = std::min(kChunk, n_macro - m);
-    if (K > 0) GC_TRY(launch(psi, cur, K));
-    ahead = m + K;
+    GC_TRY(restart());
   }
-  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  GC_HIP(hipStreamSynchronize(st));
   if (n_fallback) *n_fallback = fallbacks;
   return GC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
+                               void* psi, double step, int substeps, int n_macro,
+                               double mass_tol, int* n_fallback) {
+  const int first = 0, end = n_macro;
+  return learned_run(&n, &first, &end, 1, g, dt, p, psi, step, substeps, n_macro, mass_tol,
+                     n_fallback);
+}
+
+gc_status gc_learned_integrate_phases(gc_cnn* const* nets, const int* first, const int* end,
+                                      int n_nets, const gc_grid* g, gc_dtype dt,
+                                      const gc_ch_params* p, void* psi, double step,
+                                      int substeps, int n_macro, double mass_tol,
+                                      int* n_fallback) {
+  return learned_run(nets, first, end, n_nets, g, dt, p, psi, step, substeps, n_macro, mass_tol,
+                     n_fallback);
 }
 
 }  // extern "C"

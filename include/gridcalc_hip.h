@@ -295,6 +295,18 @@ gc_status gc_cnn_predict(gc_cnn* n, gc_dtype dt, const void* c, void* c_new, int
 gc_status gc_learned_integrate(gc_cnn* n, const gc_grid* g, gc_dtype dt, const gc_ch_params* p,
                                void* psi, double step, int substeps, int n_macro,
                                double mass_tol, int* n_fallback);
+/* gc_learned_integrate with one network per segment of macro steps (no counterpart in the
+ * reference; the phase-specialised corrector): macro step m is proposed by the first
+ * network i with first[i] <= m < end[i], and runs as plain RK4 (gc_ch_integrate) where no
+ * network covers it.  The networks must share one context and be built for the grid.  The
+ * proposal pipeline continues across segment boundaries, and the mass drift is measured
+ * from psi's mean at the call.  gc_learned_integrate(n, ...) is this with the one segment
+ * [0, n_macro). */
+gc_status gc_learned_integrate_phases(gc_cnn* const* nets, const int* first, const int* end,
+                                      int n_nets, const gc_grid* g, gc_dtype dt,
+                                      const gc_ch_params* p, void* psi, double step,
+                                      int substeps, int n_macro, double mass_tol,
+                                      int* n_fallback);
 
 /* Element-wise conversion between GC_F64 and GC_F32 device arrays. */
 gc_status gc_cast(gc_context* ctx, gc_dtype from, gc_dtype to, size_t n, const void* src,
