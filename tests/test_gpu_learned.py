@@ -236,3 +236,64 @@ def test_phased_trained_networks_beat_single(gc, ctx, trained):
     net.close()
     for n, _, _ in nets:
         n.close()
+
+
+def test_phases_mixed_decisions_match_stepwise_rule(gc, ctx):
+    """gc_learned_integrate_phases with two random networks (tiny output scale) on macro
+    steps [0, 20) and [20, 45): accepts and rejects interleave and the proposal pipeline
+    crosses the segment boundary; the result equals a host loop that decides after every
+    macro step with the segment's network (mass drift from the initial state)."""
+    from gridcalc_amd import corrector as cr
+    g = gc.Grid.make(W, H, 1)
+    nets = []
+    for seed, lo, hi in ((5, 0, 20), (6, 20, 45)):
+        n = gc.Corrector(ctx, H, W, 1)
+        n.set_params(np.random.default_rng(seed).uniform(-0.3, 0.3, gc.N_CNN_PARAMS)
+                     .astype(np.float32), 1e-6)
+        nets.append((n, lo, hi))
+    c0 = _ic(gc, 16)
+    tol, n_macro = 1e-6, 45
+    a = gc.DeviceArray.from_numpy(ctx, c0)
+    nf = cr.integrate_phased(ctx, g, a, nets, 1.0, 1.0, DT, 16, n_macro, tol, warm=0)
+
+    psi = gc.DeviceArray.from_numpy(ctx, c0)
+    prop = gc.DeviceArray.from_numpy(ctx, c0)
+    F0 = gc.ch_free_energy(ctx, g, psi, 1.0, gc.KAHAN)
+    m_init = gc.integrate(ctx, g, psi, gc.KAHAN)
+    want_nf = 0
+    for m in range(n_macro):
+        net = nets[0][0] if m < 20 else nets[1][0]
+        net.predict(psi, prop, 1)
+        F1 = gc.ch_free_energy(ctx, g, prop, 1.0, gc.KAHAN)
+        m1 = gc.integrate(ctx, g, prop, gc.KAHAN)
+        if np.isfinite(F1) and F1 <= F0 and abs(m1 - m_init) / (H * W) <= tol:
+            psi.copy_from(ctx, prop)
+            F0 = F1
+        else:
+            want_nf += 1
+            gc.ch_integrate(ctx, g, psi, 1.0, 1.0, DT, 16, gc.RK4)
+            F0 = gc.ch_free_energy(ctx, g, psi, 1.0, gc.KAHAN)
+    print(f"fallbacks {nf} of {n_macro}")
+    assert 1 <= want_nf <= n_macro - 1
+    assert nf == want_nf
+    assert np.array_equal(a.numpy(ctx), psi.numpy(ctx))
+    for n, _, _ in nets:
+        n.close()
+
+
+def test_phases_rejects_bad_arguments(gc, ctx):
+    """gc_learned_integrate_phases refuses networks built for another grid or no networks."""
+    import ctypes as C
+    g = gc.Grid.make(W, H, 1)
+    small = gc.Corrector(ctx, 32, 64, 1)
+    a = gc.DeviceArray.from_numpy(ctx, _ic(gc, 17))
+    P = gc.CHParams(1.0, 1.0)
+    handles = (C.c_void_p * 1)(small._h.value)
+    lo, hi = (C.c_int * 1)(0), (C.c_int * 1)(4)
+    st = gc.lib().gc_learned_integrate_phases(handles, lo, hi, 1, C.byref(g), a.gc_dtype,
+                                              C.byref(P), a.ptr, DT, 16, 4, 1e-6, None)
+    assert st != 0 and b"built for" in gc.lib().gc_last_error()
+    st = gc.lib().gc_learned_integrate_phases(handles, lo, hi, 0, C.byref(g), a.gc_dtype,
+                                              C.byref(P), a.ptr, DT, 16, 4, 1e-6, None)
+    assert st != 0
+    small.close()
