@@ -50,7 +50,8 @@ CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000,
 # plain RK4.  With pushforward period k > 0, every k-th iteration from the second on trains
 # on (the network's own one-step prediction from snapshot m, snapshot m + 2) instead of
 # (snapshot m, snapshot m + 1): the network learns to advance the states its own errors lead
-# to, which cuts the error that accumulates over the slow late phase.
+# to, which cuts the error that accumulates over the slow late phase (the pushforward trick of
+# Brandstetter, Worrall and Welling, "Message passing neural PDE solvers", ICLR 2022).
 WARM = 1
 PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666, 0),)),
           ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555, 0), (12, 2000, 1e-4, 1e-6, 77777, 0),
