@@ -1537,6 +1537,8 @@ gc_status learned_run(gc_cnn* const* nets, const int* first, const int* end, int
       if (first[i] <= m && m < end[i]) {
         net = i;
         stop = std::min(end[i], n_macro);
+        for (int j = 0; j < i; ++j)  // an earlier network takes over where its range starts
+          if (first[j] > m && first[j] < end[j]) stop = std::min(stop, first[j]);
         return;
       }
     net = -1;
