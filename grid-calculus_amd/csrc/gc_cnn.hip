@@ -317,8 +317,9 @@ constexpr int kAN2 = (kFR + 2) * (kFC + 2);        // A: layer-2 region (halo 1)
 constexpr int kBN3 = (kFR + 6) * (kFC + 6);        // B: layer-3 region (halo 3)
 constexpr int kBN4 = (kFR + 4) * (kFC + 4);        // B: layer-4 region (halo 2)
 
-constexpr int kFNT = 512;  // threads of the network kernels (8 waves)
-static_assert(kFNT / 2 == kFR * kFC, "layer 6 maps two lanes to each tile site");
+constexpr int kFNT = 1024;  // threads of the network kernels (16 waves)
+constexpr int kFNW = kFNT / 64;
+static_assert(kFNT >= 2 * kFR * kFC, "layer 6 maps two lanes to each tile site");
 
 // Chained mode (learned integration, one image): the state kernel A advances is formed on
 // the fly from the previous macro step, c_k = c_{k-1} + scale (out_{k-1} - mean(out_{k-1}))
@@ -336,74 +337,45 @@ struct Chain {
   Comp2* mpart;          // per-tile (mass, density) partial sums
 };
 
-// A lane's MFMA weight fragments and biases of 32 -> 32 layer l (index of the four).
-// mid_weights issues the loads; pin() then holds the values in registers with an empty asm
-// that reads and redefines them.  Without it the compiler places each fragment's load just
-// before its MFMA, so a layer waits for 18 load latencies one after the other instead of one.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-struct MidWeights {
-  bf16x8 w[18];
-  f32x4 b[4];
-};
-__device__ __forceinline__ void mid_weights(int l, const __bf16* __restrict__ wmid,
-                                            const float* __restrict__ p, int lane,
-                                            MidWeights& mw) {
-  const __bf16* wt = wmid + (size_t)l * kMidW;
-#pragma unroll
-  for (int kc = 0; kc < 18; ++kc) mw.w[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
-  const float* bias = p + kOffMid + l * kMidStride + kMidW + (lane >> 5) * 4;
-#pragma unroll
-  for (int g = 0; g < 4; ++g) mw.b[g] = *(const f32x4*)(bias + g * 8);
-}
-__device__ __forceinline__ void pin(MidWeights& mw) {
-#pragma unroll
-  for (int kc = 0; kc < 18; ++kc) asm volatile("" : "+v"(mw.w[kc]));
-#pragma unroll
-  for (int g = 0; g < 4; ++g) asm volatile("" : "+v"(mw.b[g]));
-}
-
-// One 32 -> 32 layer on the matrix cores, output region = tile + halo HO, input (in, LDS,
-// site stride kSP) one site wider.  The output region's sites, row-major, are cut into
-// chunks of 32; a wave takes chunks c and c + 8 (independent accumulators, so the two MFMA
-// chains overlap), c = wave, wave + 16, ...; store(site, g, v) receives channels
-// 8 g + cq .. 8 g + cq + 3 of an output site as BF16.
+// One 32 -> 32 layer (index l of the four) on the matrix cores, output region = tile + halo
+// HO, input (in, LDS, site stride kSP) one site wider.  The output region's sites, row-major,
+// are cut into chunks of 32, one per wave (16 waves; every region here has at most 14);
+// store(site, g, v) receives channels 8 g + cq .. 8 g + cq + 3 of an output site as BF16.
+// The weight fragments (fragment-major, one contiguous 1 KiB per load) are read as the MFMAs
+// need them: four waves per SIMD hide the load latency.
 template <int HO, class Store>
-__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, const MidWeights& mw,
-                                          int wave, int lane, Store store) {
+__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
+                                          const __bf16* __restrict__ wmid,
+                                          const float* __restrict__ p, int wave, int lane,
+                                          Store store) {
   constexpr int Wo = kFC + 2 * HO, Ho = kFR + 2 * HO, Wi = Wo + 2, NS = Wo * Ho;
   constexpr int nch = (NS + 31) / 32;
   const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
-  const bf16x8* wreg = mw.w;
+  const __bf16* wt = wmid + (size_t)l * kMidW;
   float bj[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) bj[i] = mw.b[i / 4][i % 4];
-  auto emit = [&](const f32x16& acc, int s) {
-    if (s < NS) {
+  for (int i = 0; i < 16; ++i) bj[i] = p[kOffMid + l * kMidStride + kMidW + (i / 4) * 8 + cq + i % 4];
+  for (int c = wave; c < nch; c += kFNW) {
+    const int s0 = min(c * 32 + j, NS - 1);
+    const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1;
+    f32x16 acc = {};
+#pragma unroll
+    for (int kc = 0; kc < 18; ++kc) {
+      const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
+      const int ko = (kc & 1) * 16 + kh;
+      const bf16x8 a = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8), a, acc, 0, 0, 0);
+    }
+    if (c * 32 + j < NS) {
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         bf16x4 ov;
 #pragma unroll
         for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
-        store(s, g * 8 + cq, ov);
+        store(c * 32 + j, g * 8 + cq, ov);
       }
     }
-  };
-  for (int c = wave; c < nch; c += 16) {
-    const int c1 = c + 8;
-    const int s0 = min(c * 32 + j, NS - 1), s1 = min(c1 * 32 + j, NS - 1);
-    const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1, i1 = (s1 / Wo + 1) * Wi + s1 % Wo + 1;
-    f32x16 acc0 = {}, acc1 = {};
-#pragma unroll
-    for (int kc = 0; kc < 18; ++kc) {
-      const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
-      const int ko = (kc & 1) * 16 + kh;
-      const bf16x8 a0 = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
-      const bf16x8 a1 = *(const bf16x8*)(in + (i1 + off) * kSP + ko);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a1, acc1, 0, 0, 0);
-    }
-    emit(acc0, c * 32 + j);
-    if (c1 < nch) emit(acc1, c1 * 32 + j);
   }
 }
 
@@ -427,8 +399,6 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   constexpr int NR = kAH0 * kAW0;
-  MidWeights mw;
-  mid_weights(0, wmid, p, lane, mw);
   if constexpr (APPLY) {
     // Every load is issued before the first use: the state region, the tile's five-point
     // neighbourhoods and the previous tile sums.
@@ -500,7 +470,6 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
     }
   }
   for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
-  pin(mw);
   __syncthreads();
   if constexpr (APPLY)
     if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
@@ -525,26 +494,15 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
         }
       }
     };
-    for (int cc = wave; cc < nch; cc += 16) {
-      const f32x16 acc0 = chunk(cc);
-      if (cc + 8 < nch) {
-        const f32x16 acc1 = chunk(cc + 8);
-        emit1(acc0, cc * 32 + j);
-        emit1(acc1, (cc + 8) * 32 + j);
-      } else {
-        emit1(acc0, cc * 32 + j);
-      }
-    }
+    for (int cc = wave; cc < nch; cc += kFNW) emit1(chunk(cc), cc * 32 + j);
   }
   __syncthreads();
-  mid_layer<1>(bufA, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufB + s * kSP + ch0) = v;
   });
-  mid_weights(1, wmid, p, lane, mw);
-  pin(mw);
   __syncthreads();
   const size_t img = (size_t)b * H * W;
-  mid_layer<0>(bufB, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
   });
 }
@@ -559,7 +517,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
                                                float* __restrict__ out,
                                                double* __restrict__ part, int H, int W,
                                                int tiles_x, int tiles_y) {
-  __shared__ double psum[kFNT / 64];
+  __shared__ double psum[kFR * kFC / 32];
   __shared__ float w6s[288];
   __shared__ __attribute__((aligned(16))) __bf16 bufA[kBN3 * kSP];
   __shared__ __attribute__((aligned(16))) __bf16 bufB[kBN4 * kSP];
@@ -569,8 +527,6 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   const size_t img = (size_t)b * H * W;
-  MidWeights mw;
-  mid_weights(2, wmid, p, lane, mw);
   // layer 3's activations on the halo-3 region: every load issued before the first store
   constexpr int NQ = (kBN3 * 4 + kFNT - 1) / kFNT;
   bf16x8 rv[NQ];
@@ -583,7 +539,6 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
       rv[r] = *(const bf16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
     }
   }
-  pin(mw);
 #pragma unroll
   for (int r = 0; r < NQ; ++r) {
     const int q = t + r * kFNT;
@@ -591,20 +546,18 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
   }
   for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
   __syncthreads();
-  mid_layer<2>(bufA, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufB + s * kSP + ch0) = v;
   });
-  mid_weights(3, wmid, p, lane, mw);
-  pin(mw);
   __syncthreads();
-  mid_layer<1>(bufB, mw, wave, lane, [&](int s, int ch0, const bf16x4& v) {
+  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
     *(bf16x4*)(bufA + s * kSP + ch0) = v;
   });
   __syncthreads();
   // layer 6 on the tile from the layer-5 region (halo 1, in bufA): lanes 0..31 of wave w
   // take tile sites 32 w .. 32 w + 31 over channels 0..15, lanes 32..63 the same sites over
   // channels 16..31 (k_conv6's two accumulators)
-  {
+  if (wave < kFR * kFC / 32) {
     constexpr int Wi = kFC + 2;
     const int ts = wave * 32 + (lane & 31), h = lane >> 5;
     const int ry = ts / kFC, rx = ts % kFC;
@@ -628,7 +581,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
   __syncthreads();
   if (t == 0) {
     double sd = psum[0];
-    for (int w = 1; w < kFNT / 64; ++w) sd += psum[w];
+    for (int w = 1; w < kFR * kFC / 32; ++w) sd += psum[w];
     part[blockIdx.x] = sd;
   }
 }
