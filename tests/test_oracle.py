@@ -358,3 +358,121 @@ def test_ch_free_energy_decreases(orc, gc):
         psi = orc.ch_integrate(g, psi, 1.0, 1.0, 0.02, 60, orc.RK4)
         F.append(orc.free_energy(g, psi, 1.0))
     assert all(b <= a for a, b in zip(F, F[1:])), F
+
+
+# ---- closed forms added for the CH right-hand side and the face-D scan --------------------
+
+def _plane_wave(n3, h3, k3, amp, phase=0.3):
+    """amp*sin(k.x + phase) on an (nx, ny, nz) grid with integer mode numbers k3; returns the
+    (nz, ny, nx) field and theta = k.x + phase."""
+    (nx, ny, nz), (hx, hy, hz) = n3, h3
+    i = np.arange(nx)[None, None, :]
+    j = np.arange(ny)[None, :, None]
+    k = np.arange(nz)[:, None, None]
+    th = 2 * np.pi * (k3[0] * i / nx + k3[1] * j / ny + k3[2] * k / nz) + phase
+    return amp * np.sin(th), th
+
+
+def _lap2_symbol(n3, h3, k3, mult=1):
+    """Eigenvalue of the order-2 discrete Laplacian on sin(mult*(k.x) + phase)."""
+    return sum((2 * np.cos(2 * np.pi * mult * kk / nn) - 2) / (hh * hh)
+               for nn, hh, kk in zip(n3, h3, k3))
+
+
+@pytest.mark.parametrize("n3,h3,k3,amp", [
+    ((16, 1, 1), (1.0, 1.0, 1.0), (3, 0, 0), 0.6),
+    ((12, 10, 9), (0.8, 1.1, 1.3), (2, 1, 4), 0.9),
+    ((20, 16, 1), (1.0, 1.0, 1.0), (1, 5, 0), 1.2),
+])
+def test_ch_rhs_two_mode_closed_form(orc, n3, h3, k3, amp):
+    """psi = A sin(th) with th = k.x + phase.  sin^3 th = (3 sin th - sin 3th)/4, and a plane
+    wave of mode m*k is an eigenvector of the O2 Laplacian with eigenvalue L(m*k), so
+      mu  = (-A - kappa*L(k)*A + 3A^3/4) sin th - (A^3/4) sin 3th
+      rhs = M*[L(k)*(-A - kappa*L(k)*A + 3A^3/4) sin th - L(3k)*(A^3/4) sin 3th]
+    exactly (aliasing of 3k on the grid changes nothing: the symbol is evaluated at the grid's
+    own mode).  Pins the cubic term's sign and factor, kappa, M and the per-axis spacing."""
+    M, kappa = 1.3, 0.7
+    g = orc.Grid.make(*n3, h3)
+    psi, th = _plane_wave(n3, h3, k3, amp)
+    L1, L3 = _lap2_symbol(n3, h3, k3, 1), _lap2_symbol(n3, h3, k3, 3)
+    want = M * (L1 * (-amp - kappa * L1 * amp + 0.75 * amp ** 3) * np.sin(th)
+                - L3 * 0.25 * amp ** 3 * np.sin(3 * th))
+    got = orc.ch_rhs(g, psi, M, kappa)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    # the same field with the cubic term's sign flipped differs by far more than rounding
+    wrong = M * (L1 * (-amp - kappa * L1 * amp - 0.75 * amp ** 3) * np.sin(th)
+                 + L3 * 0.25 * amp ** 3 * np.sin(3 * th))
+    assert np.abs(got - wrong).max() > 1e-3 * np.abs(want).max()
+
+
+def test_max_face_d_known_answers(orc):
+    """Largest harmonic-mean face D (solve/Diffusion.hpp:187-207), hand-computed:
+    - x strip D = [1, 4, 4, 1]: x faces 1.6, 4, 1.6, 1; the degenerate y and z axes wrap to
+      the cell itself, so their faces are hmean(D, D) = D: maximum 4;
+    - 2x2x2 checkerboard of 1 and 9: every face joins 1 and 9, 2*9/10 = 1.8 (an arithmetic
+      face mean would give 5, max(D) 9);
+    - D = 1 except two cells of 8 adjacent along one axis a: the face between them is 8, all
+      others at most hmean(8, 1) = 16/9; scanning every axis but a gives 16/9."""
+    g = orc.Grid.make(4, 1, 1)
+    assert orc.max_face_d(g, [1.0, 4.0, 4.0, 1.0]) == 4.0
+    g = orc.Grid.make(2, 2, 2)
+    i, j, k = np.meshgrid(np.arange(2), np.arange(2), np.arange(2), indexing="ij")
+    D = np.where((i + j + k) % 2 == 0, 1.0, 9.0).transpose(2, 1, 0)
+    assert orc.max_face_d(g, D) == 2.0 * 1.0 * 9.0 / 10.0
+    g = orc.Grid.make(5, 6, 7)
+    for axis in range(3):
+        D = np.ones(g.shape)  # (nz, ny, nx)
+        a = [2, 3, 4]  # (i, j, k) of the first cell
+        b = list(a)
+        b[axis] += 1
+        D[a[2], a[1], a[0]] = 8.0
+        D[b[2], b[1], b[0]] = 8.0
+        assert orc.max_face_d(g, D) == 8.0, axis
+    # a periodic face: the last and first cells along y
+    D = np.ones(g.shape)
+    D[0, 0, 0] = D[0, g.ny - 1, 0] = 3.0
+    assert orc.max_face_d(g, D) == 3.0
+    # no pair of large neighbours: the largest face joins 8 and 1
+    D = np.ones(g.shape)
+    D[3, 3, 3] = 8.0
+    assert orc.max_face_d(g, D) == 2.0 * 8.0 / 9.0
+
+
+def test_hetero_diffusion_matches_constant_d(orc):
+    """diffusion_test.cpp:216-237: with D uniform the heterogeneous path follows the
+    constant-D trajectory within 1e-12 (sin x cos y sin z, N = 16, 50 Euler steps)."""
+    N = 16
+    h = 2 * np.pi / N
+    g = orc.Grid.make(N, N, N, h)
+    x = np.arange(N) * h
+    psi = np.sin(x)[None, None, :] * np.cos(x)[None, :, None] * np.sin(x)[:, None, None]
+    D, dt, n = 0.1, 0.4 * h * h / (6 * 0.1), 50
+    a = orc.diffuse_const(g, psi, D, dt, n, orc.EULER)
+    b = orc.diffuse_field(g, psi, np.full(g.shape, D), dt, n, orc.EULER)
+    assert np.abs(a - b).max() < 1e-12
+    a = orc.diffuse_const(g, psi, D, dt, n, orc.RK4)
+    b = orc.diffuse_field(g, psi, np.full(g.shape, D), dt, n, orc.RK4)
+    assert np.abs(a - b).max() < 1e-12
+
+
+def test_hetero_cfl_uses_max_face_d(orc):
+    """diffusion_test.cpp:239-254: D = 0.5 + 0.5 cos x (bumped to 1e-6 where it vanishes),
+    N = 32: the largest face D is hmean of the two cells beside x = 0, a hair under 1, so
+    dt = h^2/3 (CFL 1 > 0.5) is rejected and dt at 0.499 of the bound by that face passes."""
+    N = 32
+    h = 2 * np.pi / N
+    g = orc.Grid.make(N, N, N, h)
+    x = np.arange(N) * h
+    Dx = 0.5 + 0.5 * np.cos(x)
+    Dx[Dx <= 0.0] = 1e-6
+    D = np.broadcast_to(Dx[None, None, :], g.shape).copy()
+    face = max(2 * Dx[i] * Dx[(i + 1) % N] / (Dx[i] + Dx[(i + 1) % N]) for i in range(N))
+    # y and z faces join two cells of equal D: hmean(D, D), which rounds like any face mean
+    face = max(face, max(2 * d * d / (d + d) for d in Dx))
+    assert orc.max_face_d(g, D) == face
+    psi = np.ones(g.shape)
+    with pytest.raises(orc.OracleError) as e:
+        orc.diffuse_field(g, psi, D, h * h / 3.0, 1, orc.EULER)
+    assert e.value.code == orc.CFL
+    ok_dt = 0.499 * h * h / (3 * face)
+    orc.diffuse_field(g, psi, D, ok_dt, 1, orc.EULER)
