@@ -3,21 +3,25 @@
 
 Default workload (the RK4 configuration of the headline metric): 3-D Cahn-Hilliard on a
 256^3 periodic grid, dx=1, M=1, kappa=0.5, dt=0.005, c0 = 0.1*U(-1,1); stencils and RK4
-stages in FP32, total mass and free energy reduced in FP64 (compensated) every 500 steps,
-as the reference driver prints diagnostics per snapshot chunk.  One "step" is one RK4
-step of the whole grid: 4 fused stage kernels.  Inputs are resident in HBM before timing.
+stages in FP64 (the reference's precision) for the headline line, and the FP32 form
+(FP32 stencils and stages) in extra.fp32; total mass and free energy are reduced in FP64
+(compensated) every 500 steps and at the end, as the reference driver prints diagnostics
+per snapshot chunk.  One "step" is one RK4 step of the whole grid: two two-stage kernels.
+Inputs are resident in HBM before timing; every kernel of the timed region has run once
+before it.
 
-Also reported: the FP32 result's L2/Linf error against an FP64 GPU run of the same steps,
-the FP64 pipeline's throughput, the roofline position of the fused step, and the
-single-core CPU oracle (a C restatement of the reference path, FP64) timed on a bounded
-prefix of the same workload.
+Also reported: the FP32 result's L2/Linf error against the FP64 GPU run of the same steps,
+the roofline position of the fused step (kernel time from HIP events over 200 steps), and
+the single-core CPU oracle (a C restatement of the reference path, FP64, gcc -O3) timed on
+a bounded prefix of the same workload, with the host CPU model and the oracle's flags.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-fp64]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-fp32]
   python bench.py --workload sweep [--n 512]      # operator GB/s (3-D operator sweep)
   python bench.py --workload ensemble             # 64 uneven 2-D members, graph-replayed
   python bench.py --workload learned [--n-ic 4]   # learned corrector vs plain RK4 to T=20
   python bench.py --workload tune [--n 256]       # autotuner: ms/step of every tile shape
   python bench.py --workload diffuse [--n 256]    # constant- and heterogeneous-D diffusion RK4
+  python bench.py --workload reduce [--n 256]     # integrate() and the fused free energy
 """
 from __future__ import annotations
 
@@ -47,12 +51,12 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
-                                                          "learned", "tune", "diffuse"])
+                                                          "learned", "tune", "diffuse", "reduce"])
     p.add_argument("--n-ic", type=int, default=4)
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--diag-every", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-tune", action="store_true")
     p.add_argument("--ch-config", type=int, default=None,
                    help="force this tile candidate (gc_ch_set_config) in both precisions")
@@ -89,6 +93,17 @@ def run_timed(gc, ctx, g, d, steps, diag_every, kappa, dt):
     return ms, diags
 
 
+def warm(gc, ctx, g, c0, np_dtype, steps, kappa, dt):
+    """Untimed first launches of every kernel the timed region runs: the stage pipeline and
+    both compensated reductions."""
+    w = gc.DeviceArray.from_numpy(ctx, c0.astype(np_dtype))
+    gc.ch_integrate(ctx, g, w, 1.0, kappa, dt, max(steps, 2), gc.RK4)
+    gc.integrate(ctx, g, w, gc.KAHAN)
+    gc.ch_free_energy(ctx, g, w, kappa, gc.KAHAN)
+    ctx.sync()
+    w.free()
+
+
 def step_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
     """ms per fused RK4 step: HIP events on the context's stream around `steps` (even) steps
     and nothing else (2 two-stage or 4 one-stage kernel launches per step)."""
@@ -102,6 +117,17 @@ def step_ms(gc, ctx, g, c0, dtype, steps, kappa, dt):
     return ms / steps
 
 
+def cpu_record(orc, extra_sample):
+    """The oracle's build and this host's CPU, recorded beside every CPU timing."""
+    info = orc.host_info()
+    return {"cores": 1, "kind": "port", "cpu_model": info["cpu_model"],
+            "host_nproc": info["nproc"], "build": info["oracle_build"],
+            "sample": extra_sample}
+
+
+KERNEL_TIMING_STEPS = 200
+
+
 def ch3d(a):
     gc = entry.load_package()
     n = a.n or 256
@@ -110,54 +136,55 @@ def ch3d(a):
     c0 = gc.inputs.ch_initial(n, n, n, 0.1, seed=3)
     sites = n ** 3
     extra = {}
+    legs = {}
     with gc.Context() as ctx:
-        tiles = {}
-        for dtype, name in ((gc.F32, "f32"), (gc.F64, "f64")):
+        tiles, kinds = {}, {}
+        for dtype, name in ((gc.F64, "f64"), (gc.F32, "f32")):
             if a.ch_config is not None:
                 gc.ch_set_config(g, dtype, a.ch_config)
             elif not a.no_tune:
-                best, ms = gc.ch_tune(ctx, g, dtype)
-            tiles[name] = gc.ch_config_shape(g, gc.ch_get_config(g, dtype))
+                gc.ch_tune(ctx, g, dtype)
+            cfg = gc.ch_get_config(g, dtype)
+            tiles[name] = gc.ch_config_shape(g, cfg)
+            kinds[name] = gc.ch_config_kind(g, cfg)
         extra["tile_ox_oy_zc"] = tiles
-        kinds = {name: gc.ch_config_kind(g, gc.ch_get_config(g, dtype))
-                 for dtype, name in ((gc.F32, "f32"), (gc.F64, "f64"))}
-        fused = kinds["f32"] > 0
-        kname = "k_ch_pairq3d" if kinds["f32"] == 2 else "k_ch_pair3d"
         extra["two_stage_kernels"] = {k: ["none", "plane rings", "register queues"][v]
                                       for k, v in kinds.items()}
-
-        d32 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
-        if a.warmup:
-            w = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float32))
-            gc.ch_integrate(ctx, g, w, 1.0, kappa, dt, a.warmup, gc.RK4)
-            ctx.sync()
-            w.free()
-        ms, diags = run_timed(gc, ctx, g, d32, a.steps, a.diag_every, kappa, dt)
-        value = sites * a.steps / (ms / 1e3)
-        sms = step_ms(gc, ctx, g, c0, np.float32, min(a.steps, 400), kappa, dt)
-        bytes_step = WORDS_PER_SITE_STEP * 4 * sites
-        achieved = bytes_step / (sms / 1e3) / 1e9
-        extra["diagnostics_fp32"] = [{"step": s, "mass": m, "F": f} for s, m, f in diags]
-        if not a.no_fp64:
-            d64 = gc.DeviceArray.from_numpy(ctx, c0)
-            ms64, diags64 = run_timed(gc, ctx, g, d64, a.steps, a.diag_every, kappa, dt)
-            r32 = d32.numpy(ctx).astype(np.float64)
-            r64 = d64.numpy(ctx)
-            extra["fp32_vs_fp64"] = {
-                "l2_rel": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64)),
-                "linf_rel": float(np.abs(r32 - r64).max() / np.abs(r64).max()),
-                "mass_fp32": diags[-1][1], "mass_fp64": diags64[-1][1],
-                "F_fp32": diags[-1][2], "F_fp64": diags64[-1][2]}
-            sms64 = step_ms(gc, ctx, g, c0, np.float64, min(a.steps, 200), kappa, dt)
-            extra["fp64"] = {
-                "site_updates_per_s": sites * a.steps / (ms64 / 1e3),
-                "ms_per_step": ms64 / a.steps,
-                "step_kernels_ms": sms64,
-                "hbm_GBs_alg": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9,
-                "hbm_frac": WORDS_PER_SITE_STEP * 8 * sites / (sms64 / 1e3) / 1e9 / HBM_PEAK_GBS,
-                "traffic": profiled_traffic(
-                    f"ch3d_rk4_{'pair' if kinds['f64'] else 'stage'}_kernels_{n}_f32.md",
-                    "traffic_bytes_per_step_f64")}
+        arrays = {}
+        legs_wanted = [("f64", gc.F64, np.float64, 8)]
+        if not a.no_fp32:
+            legs_wanted.append(("f32", gc.F32, np.float32, 4))
+        for name, dtype, np_dtype, w in legs_wanted:
+            warm(gc, ctx, g, c0, np_dtype, a.warmup, kappa, dt)
+            d = gc.DeviceArray.from_numpy(ctx, c0.astype(np_dtype))
+            ms, diags = run_timed(gc, ctx, g, d, a.steps, a.diag_every, kappa, dt)
+            sms = step_ms(gc, ctx, g, c0, np_dtype, KERNEL_TIMING_STEPS, kappa, dt)
+            bytes_step = WORDS_PER_SITE_STEP * w * sites
+            legs[name] = {"ms": ms, "diags": diags, "step_kernels_ms": sms,
+                          "bytes_step": bytes_step,
+                          "achieved": bytes_step / (sms / 1e3) / 1e9}
+            arrays[name] = d
+        if "f32" in legs:
+            r64 = arrays["f64"].numpy(ctx)
+            r32 = arrays["f32"].numpy(ctx).astype(np.float64)
+            L = legs["f32"]
+            extra["fp32"] = {
+                "site_updates_per_s": sites * a.steps / (L["ms"] / 1e3),
+                "ms_per_step": L["ms"] / a.steps,
+                "step_kernels_ms": L["step_kernels_ms"],
+                "hbm_GBs_alg": L["achieved"], "hbm_frac": L["achieved"] / HBM_PEAK_GBS,
+                "kernel": _kernel_name(kinds["f32"], "float", tiles["f32"]),
+                "traffic": profiled_traffic(PROFILE_CH3D, "traffic_bytes_per_step_f32"),
+                "diagnostics": [{"step": s, "mass": m, "F": f} for s, m, f in L["diags"]],
+                "vs_fp64": {
+                    "l2_rel": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64)),
+                    "linf_rel": float(np.abs(r32 - r64).max() / np.abs(r64).max()),
+                    "mass_fp32": L["diags"][-1][1], "mass_fp64": legs["f64"]["diags"][-1][1],
+                    "F_fp32": L["diags"][-1][2], "F_fp64": legs["f64"]["diags"][-1][2]}}
+        for d in arrays.values():
+            d.free()
+    H = legs["f64"]
+    extra["diagnostics_fp64"] = [{"step": s, "mass": m, "F": f} for s, m, f in H["diags"]]
 
     if not a.no_learned:
         extra["learned"] = learned(a)
@@ -171,39 +198,51 @@ def ch3d(a):
         t0 = time.perf_counter()
         orc.ch_integrate(og, c0, 1.0, kappa, dt, a.cpu_steps, orc.RK4)
         el = time.perf_counter() - t0
-        cpu = {"value": sites * a.cpu_steps / el, "unit": "site-updates/s", "cores": 1,
-               "kind": "port",
-               "sample": f"CPU oracle (C restatement of the reference path, FP64, gcc -O2, "
-                         f"1 thread): RK4 on the same {n}^3 input, first {a.cpu_steps} steps "
-                         f"({el:.1f} s)"}
+        cpu = {"value": sites * a.cpu_steps / el, "unit": "site-updates/s",
+               **cpu_record(orc, f"CPU oracle (C restatement of the reference path, FP64, "
+                                 f"1 thread pinned to one core): RK4 on the same {n}^3 input; the "
+                                 f"rate of the first {a.cpu_steps} steps ({el:.1f} s), not a full "
+                                 f"{a.steps}-step run")}
 
     return {
         "metric": "RK4 site-updates/s",
-        "value": value,
+        "value": sites * a.steps / (H["ms"] / 1e3),
         "unit": "site-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms / a.steps,
+        "ms_per_step": H["ms"] / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3-D Cahn-Hilliard fused RK4, {n}^3, dx=1, M=1, kappa=0.5, "
-                               f"dt=0.005, c0=0.1*U(-1,1) seed 3; FP32 stages, FP64 "
-                               f"compensated mass/F every {a.diag_every} steps",
+                               f"dt=0.005, c0=0.1*U(-1,1) seed 3; FP64 stages (the FP32 form in "
+                               f"extra.fp32), FP64 compensated mass/F every {a.diag_every} steps "
+                               f"and at the end",
                    "grid": [n, n, n], "dt": dt, "kappa": kappa, "M": 1.0},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(f"ch3d_rk4_{'pair' if fused else 'stage'}_kernels_{n}_f32.md"),
-                     "kernel": (f"{kname}<float,{tiles['f32'][0]},{tiles['f32'][1]},{tiles['f32'][2]},...> "
-                                "x2 (one RK4 step = S1+S2 and S3+S4 launches)") if fused else
-                               "k_ch_stage3d<float,...> x4 (one RK4 step = 4 stage launches)",
-                     "step_kernels_ms": sms, "alg_bytes_per_step": bytes_step},
+        "roofline": {"bound": "hbm", "achieved": H["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": H["achieved"] / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(PROFILE_CH3D, "traffic_bytes_per_step_f64"),
+                     "kernel": _kernel_name(kinds["f64"], "double", tiles["f64"]),
+                     "step_kernels_ms": H["step_kernels_ms"],
+                     "kernel_timing_steps": KERNEL_TIMING_STEPS,
+                     "alg_bytes_per_step": H["bytes_step"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }
+
+
+PROFILE_CH3D = "ch3d_rk4_pairq3d_256.md"
+
+
+def _kernel_name(kind, T, tile):
+    if kind == 0:
+        return f"k_ch_stage3d<{T},...> x4 (one RK4 step = 4 stage launches)"
+    kname = "k_ch_pairq3d" if kind == 2 else "k_ch_pair3d"
+    return (f"{kname}<{T},{tile[0]},{tile[1]},{tile[2]},...> x2 (one RK4 step = S1+S2 and "
+            "S3+S4 launches)")
 
 
 def sweep(a):
@@ -292,9 +331,9 @@ def sweep(a):
             gpu["cpu_ms"] = el * 1e3
             gpu["speedup_vs_cpu"] = el * 1e3 / gpu["ms"]
             cpu[f"{name}{order}"] = el * 1e3
-        cpu = {"ms_per_op": cpu, "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (C restatement, FP64, gcc -O2, 1 thread), each operator once "
-                         f"on the same {n}^3 input"}
+        cpu = {"ms_per_op": cpu,
+               **cpu_record(orc, f"CPU oracle (C restatement, FP64, 1 thread), each operator "
+                                 f"once on the same {n}^3 input")}
     return {"metric": "operator GB/s (algorithmic bytes)", "n": n, "rows": rows, "cpu_baseline": cpu}
 
 
@@ -351,9 +390,9 @@ def ensemble(a):
             og = orc.Grid.make(m["nx"], m["ny"], 1, 1.0)
             orc.ch_integrate(og, c, 1.0, m["kappa"], dt, 2, orc.RK4)
         el = time.perf_counter() - t0
-        cpu = {"value": sites * 2 / el, "unit": "site-updates/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle RK4 (FP64, gcc -O2, 1 thread): the first 2 steps of all 64 "
-                         f"members ({el:.1f} s)"}
+        cpu = {"value": sites * 2 / el, "unit": "site-updates/s",
+               **cpu_record(orc, f"CPU oracle RK4 (FP64, 1 thread): the rate of the first 2 "
+                                 f"steps of all 64 members ({el:.1f} s)")}
     return {"metric": "ensemble RK4 site-updates/s", "value": value, "unit": "site-updates/s",
             "cpu_baseline": cpu,
             "dtype": "f32", "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps,
@@ -486,9 +525,9 @@ def learned(a):
         t0 = time.perf_counter()
         orc.ch_integrate(og, c0, 1.0, 1.0, dt, 160, orc.RK4)
         el = time.perf_counter() - t0
-        cpu = {"value": el * n_steps / 160, "unit": "s to T=20", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle RK4 (FP64, gcc -O2, 1 thread): the first 160 of 2000 steps "
-                         f"({el:.2f} s), scaled"}
+        cpu = {"value": el * n_steps / 160, "unit": "s to T=20",
+               **cpu_record(orc, f"CPU oracle RK4 (FP64, 1 thread): the first 160 of 2000 steps "
+                                 f"({el:.2f} s), extrapolated to T=20")}
     best_plain = min(med["plain_fp64"], med["plain_fp64_graph"])
     variants = {"learned_fp64": max(errs), "learned_fp64_warm1": max(errs_w)}
     if errs_p:
@@ -566,9 +605,9 @@ def diffuse(a):
         t0 = time.perf_counter()
         orc.diffuse_field(og, psi, Dh, dt, 2, orc.RK4)
         el = time.perf_counter() - t0
-        cpu = {"value": n ** 3 * 2 / el, "unit": "site-updates/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle heterogeneous-D RK4 (FP64, gcc -O2, 1 thread): 2 steps of "
-                         f"the same {n}^3 input ({el:.1f} s)"}
+        cpu = {"value": n ** 3 * 2 / el, "unit": "site-updates/s",
+               **cpu_record(orc, f"CPU oracle heterogeneous-D RK4 (FP64, 1 thread): the rate of "
+                                 f"2 steps of the same {n}^3 input ({el:.1f} s)")}
     return {"metric": "diffusion RK4 ms per step", "grid": [n, n, n], "steps": steps, "rows": rows,
             "cpu_baseline": cpu}
 
@@ -589,6 +628,38 @@ def tune(a):
                              "hbm_frac_16W": WORDS_PER_SITE_STEP * w * n ** 3 / (m / 1e3) / 1e9
                              / HBM_PEAK_GBS, "best": i == best})
     return {"metric": "RK4 ms per step by tile shape", "grid": [n, n, n], "rows": rows}
+
+
+def reduce(a):
+    """The reduction path on the 3-D workload's grid (default 256^3): integrate() pairwise and
+    Kahan, and the fused Cahn-Hilliard free energy (gradient + density + reduction) pairwise
+    and Kahan, FP64 and FP32 inputs; ms per call (HIP events around 20 calls, each ending
+    with its result on the host) and GB/s on the algorithmic 1 word per site."""
+    gc = entry.load_package()
+    n = a.n or 256
+    g = gc.Grid.make(n, n, n, 1.0)
+    c = gc.inputs.ch_initial(n, n, n, 0.1, seed=3)
+    rows = []
+    with gc.Context() as ctx:
+        for np_dtype, w, name in ((np.float64, 8, "f64"), (np.float32, 4, "f32")):
+            d = gc.DeviceArray.from_numpy(ctx, c.astype(np_dtype))
+            calls = {"integrate_pairwise": lambda: gc.integrate(ctx, g, d, gc.PAIRWISE),
+                     "integrate_kahan": lambda: gc.integrate(ctx, g, d, gc.KAHAN),
+                     "free_energy_pairwise": lambda: gc.ch_free_energy(ctx, g, d, 0.5, gc.PAIRWISE),
+                     "free_energy_kahan": lambda: gc.ch_free_energy(ctx, g, d, 0.5, gc.KAHAN)}
+            for op, fn in calls.items():
+                fn()
+                reps = 20
+                ctx.timer_start()
+                for _ in range(reps):
+                    fn()
+                ms = ctx.timer_stop() / reps
+                gbs = w * n ** 3 / (ms / 1e3) / 1e9
+                rows.append({"op": op, "dtype": name, "ms_per_call": ms, "GBs_alg": gbs,
+                             "hbm_frac": gbs / HBM_PEAK_GBS})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+            d.free()
+    return {"metric": "reduction ms per call", "grid": [n, n, n], "rows": rows}
 
 
 def copy(a):
@@ -618,7 +689,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble,
-           "learned": learned, "tune": tune, "diffuse": diffuse}[a.workload](a)
+           "learned": learned, "tune": tune, "diffuse": diffuse, "reduce": reduce}[a.workload](a)
     print(json.dumps(out))
 
 

@@ -119,7 +119,12 @@ gc_status gc_context_create(gc_context** out, void* stream) {
     return set_error(GC_ERR_DEVICE, "gridcalc: no HIP device available");
   }
   gc_context* ctx = new gc_context();
-  GC_HIP(hipGetDevice(&ctx->device));
+  const hipError_t ed = hipGetDevice(&ctx->device);
+  if (ed != hipSuccess) {
+    delete ctx;
+    (void)hipGetLastError();
+    return set_error(GC_ERR_DEVICE, "hipGetDevice failed: %s", hipGetErrorString(ed));
+  }
   if (stream) {
     ctx->stream = (hipStream_t)stream;
   } else {

@@ -12,6 +12,9 @@
 // see the reference's semantics, and chains of library operators never leave the device.
 // Const host access mutates the cache, so a Field must not be read from several threads
 // at once while its device copy is newer.
+// The host accessors that may download (operator(), data(), begin(), end()) are not
+// noexcept, unlike the reference's host-only ones: a device failure while downloading
+// throws std::runtime_error with the library's message instead of ending the process.
 #pragma once
 
 #include <cstddef>
@@ -95,11 +98,11 @@ class Field {
   }
   ~Field() { hip::release(_dev, bytes()); }
 
-  T& operator()(int i, int j, int k) noexcept {
+  T& operator()(int i, int j, int k) {
     toHost(true);
     return _data[index(i, j, k)];
   }
-  const T& operator()(int i, int j, int k) const noexcept {
+  const T& operator()(int i, int j, int k) const {
     toHost(false);
     return _data[index(i, j, k)];
   }
@@ -108,18 +111,18 @@ class Field {
   std::size_t getSize() const noexcept { return _grid.getSize(); }
   std::size_t size() const noexcept { return getSize(); }
 
-  T* data() noexcept {
+  T* data() {
     toHost(true);
     return _data.data();
   }
-  const T* data() const noexcept {
+  const T* data() const {
     toHost(false);
     return _data.data();
   }
-  T* begin() noexcept { return data(); }
-  const T* begin() const noexcept { return data(); }
-  T* end() noexcept { return data() + getSize(); }
-  const T* end() const noexcept { return data() + getSize(); }
+  T* begin() { return data(); }
+  const T* begin() const { return data(); }
+  T* end() { return data() + getSize(); }
+  const T* end() const { return data() + getSize(); }
 
   // ---- device residency -----------------------------------------------------------
   // Device pointer to current contents (uploads a newer host copy).
@@ -161,7 +164,7 @@ class Field {
     return _grid.getLinearIndex(Policy::wrap(i, _grid.getNx()), Policy::wrap(j, _grid.getNy()),
                                 Policy::wrap(k, _grid.getNz()));
   }
-  void toHost(bool will_write) const noexcept {
+  void toHost(bool will_write) const {
     if (_state == kDevice) {
       if (_data.size() != getSize()) _data.resize(getSize());
       hip::download(_data.data(), _dev, bytes());

@@ -2,9 +2,11 @@
 // the process-wide execution context, translation of gc_status codes into the exceptions
 // the reference throws, and a caching device-memory pool for Field storage.
 //
-// Every Field operation of this library runs on one gc_context (one HIP stream), so device
-// buffers returned to the pool can be handed out again at once: later work on the same
-// stream is ordered after the work that last used them.
+// Every Field operation of this library runs on one gc_context (one HIP stream) at a time,
+// so device buffers returned to the pool can be handed out again at once: later work on the
+// same stream is ordered after the work that last used them.  Switching to another context
+// (setContext) first waits for the outgoing context's stream, so a block released by work
+// still queued there is never reused by work on the new stream.
 #pragma once
 
 #include <cstddef>
@@ -60,10 +62,15 @@ inline gc_context* context() {
 }
 
 // Installs a caller-owned context (e.g. bound to the caller's HIP stream); nullptr
-// restores the private one.  The caller keeps ownership.
+// restores the private one.  The caller keeps ownership.  Waits for all work queued on the
+// outgoing context first (pooled blocks are reused without per-block events, so the old
+// stream must be idle before another stream can receive them).
 inline void setContext(gc_context* ctx) {
   auto& s = detail::state();
   std::lock_guard<std::mutex> lock(s.mu);
+  gc_context* outgoing = s.user_ctx ? s.user_ctx : s.ctx;
+  gc_context* incoming = ctx ? ctx : s.ctx;
+  if (outgoing && outgoing != incoming) check(gc_context_sync(outgoing));
   s.user_ctx = ctx;
 }
 

@@ -27,6 +27,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef OR_CFLAGS
+#define OR_CFLAGS "unknown flags"
+#endif
+
+/* The compiler and flags this library was built with (recorded beside CPU timings). */
+const char* or_build_info(void) { return "gcc " __VERSION__ "; " OR_CFLAGS; }
+
 typedef struct {
   int nx, ny, nz;
   double hx, hy, hz;

@@ -98,8 +98,32 @@ def lib():
         L.or_diffuse_field.restype = C.c_int
         L.or_stencil_table.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), dp]
         L.or_stencil_table.restype = C.c_int
+        L.or_build_info.argtypes = []
+        L.or_build_info.restype = C.c_char_p
         _lib = L
     return _lib
+
+
+def build_info():
+    """Compiler version and flags of liboracle.so."""
+    return lib().or_build_info().decode()
+
+
+def host_info():
+    """CPU model and logical core count of this host (for CPU-baseline records)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "oracle_build": build_info()}
 
 
 def _dp(a):

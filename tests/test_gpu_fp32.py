@@ -1,0 +1,196 @@
+"""The FP32 form against the FP64 oracle.
+
+FP32 stencils and RK4 stages, FP64 reductions: every FP32 kernel family of the 3-D
+Cahn-Hilliard pipeline (one kernel per stage, two-stage kernels with LDS plane rings,
+two-stage kernels with register queues along z) and the FP32 diffusion kernels (constant and
+heterogeneous D, Euler and RK4) run from an FP32 initial state; the FP64 oracle runs the same
+steps from the same values widened to FP64.  Each result's relative L2 and Linf error is
+bounded in the test.  The FP64 mass and free energy of an FP32 field are checked against the
+oracle's reductions of the same values: pairwise bit for bit, Kahan within its error bound.
+
+Bounds: after 101 steps of the 3-D Cahn-Hilliard configuration (kappa = 0.5, dt = 0.005,
+c0 = 0.1*U(-1,1)) a plain float32 restatement of the same RK4 differs from the FP64 oracle by
+1.4e-7 (L2) and 2.8e-7 (Linf), relative; the bounds below allow about seven times that.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KAPPA, DT, AMP = 0.5, 0.005, 0.1     # the 3-D Cahn-Hilliard configuration's parameters
+STEPS = 101                          # odd: the two-stage kernels end with one copy back
+L2_BOUND, LINF_BOUND = 1e-6, 2e-6
+
+SHAPES_3D = [(36, 20, 70), (70, 9, 33)]
+
+
+def _rel_errors(got, want):
+    got = np.asarray(got, np.float64)
+    return (float(np.linalg.norm(got - want) / np.linalg.norm(want)),
+            float(np.abs(got - want).max() / np.abs(want).max()))
+
+
+@pytest.fixture(scope="module")
+def ch3d_oracle(orc, gc):
+    """FP64 oracle runs of the configuration from FP32-rounded initial states, per shape."""
+    cache = {}
+
+    def get(shape):
+        if shape not in cache:
+            c0 = gc.inputs.ch_initial(*shape, AMP, seed=3).astype(np.float32)
+            og = orc.Grid.make(*shape)
+            want = orc.ch_integrate(og, c0.astype(np.float64), 1.0, KAPPA, DT, STEPS, orc.RK4)
+            cache[shape] = (c0, want)
+        return cache[shape]
+    return get
+
+
+@pytest.mark.parametrize("shape", SHAPES_3D, ids=lambda s: "x".join(map(str, s)))
+def test_ch3d_fp32_every_family_vs_fp64_oracle(gc, orc, ctx, ch3d_oracle, shape):
+    g = gc.Grid.make(*shape)
+    og = orc.Grid.make(*shape)
+    c0, want = ch3d_oracle(shape)
+    n = gc.ch_config_count(g)
+    kinds = {gc.ch_config_kind(g, i) for i in range(n)}
+    assert kinds == {0, 1, 2}, kinds
+    report = []
+    try:
+        for i in range(n):
+            gc.ch_set_config(g, gc.F32, i)
+            d = gc.DeviceArray.from_numpy(ctx, c0)
+            gc.ch_integrate(ctx, g, d, 1.0, KAPPA, DT, STEPS, gc.RK4)
+            got = d.numpy(ctx)
+            l2, linf = _rel_errors(got, want)
+            report.append((i, gc.ch_config_kind(g, i), gc.ch_config_shape(g, i), l2, linf))
+            assert l2 < L2_BOUND and linf < LINF_BOUND, report[-1]
+            # FP64 reductions of the FP32 field: the oracle's on the same values
+            g64 = got.astype(np.float64)
+            assert gc.integrate(ctx, g, d, gc.PAIRWISE) == orc.integrate(og, g64, orc.PAIRWISE)
+            mk = gc.integrate(ctx, g, d, gc.KAHAN)
+            assert abs(mk - orc.integrate(og, g64, orc.KAHAN)) <= 4e-16 * np.abs(g64).sum()
+            assert gc.ch_free_energy(ctx, g, d, KAPPA) == orc.free_energy(og, g64, KAPPA)
+            d.free()
+    finally:
+        gc.ch_set_config(g, gc.F32, -1)
+    print("FP32 3-D CH vs FP64 oracle (id, kind, tile, L2, Linf):", *report, sep="\n  ")
+    # mass and free energy of the FP32 run against those of the FP64 trajectory
+    F64 = orc.free_energy(og, want, KAPPA)
+    Fk = orc.free_energy(og, got.astype(np.float64), KAPPA)
+    assert abs(Fk - F64) <= 1e-6 * abs(F64)
+    m0 = orc.integrate(og, c0.astype(np.float64), orc.KAHAN)
+    m64 = orc.integrate(og, want, orc.KAHAN)
+    assert abs(m64 - m0) <= 1e-12 * np.abs(c0).sum()           # FP64: rounding only
+    assert abs(orc.integrate(og, got.astype(np.float64), orc.KAHAN) - m0) <= 1e-7 * np.abs(c0).sum()
+
+
+def test_ch3d_fp32_256_properties(gc, ctx):
+    """The full 3-D configuration's grid (256^3, FP32 stages, default tile): over 2000 steps
+    in 8 diagnostic chunks the FP64 Kahan mass stays within 1e-7 of sum|c0| of its start and
+    the FP64 free energy never increases; after the first 200 steps the FP32 state is within
+    the L2/Linf bounds of an FP64 GPU run of the same steps (the FP64 pipeline equals the
+    oracle bit for bit on every tested shape)."""
+    n = 256
+    g = gc.Grid.make(n, n, n)
+    c0 = gc.inputs.ch_initial(n, n, n, AMP, seed=3).astype(np.float32)
+    d32 = gc.DeviceArray.from_numpy(ctx, c0)
+    d64 = gc.DeviceArray.from_numpy(ctx, c0.astype(np.float64))
+    gc.ch_integrate(ctx, g, d32, 1.0, KAPPA, DT, 200, gc.RK4)
+    gc.ch_integrate(ctx, g, d64, 1.0, KAPPA, DT, 200, gc.RK4)
+    l2, linf = _rel_errors(d32.numpy(ctx), d64.numpy(ctx))
+    print(f"256^3 FP32 vs FP64 after 200 steps: L2 {l2:.3e}  Linf {linf:.3e}")
+    assert l2 < L2_BOUND and linf < LINF_BOUND, (l2, linf)
+    d64.free()
+    d32.upload(ctx, c0)
+    sum_abs = float(np.abs(c0.astype(np.float64)).sum())
+    m0 = gc.integrate(ctx, g, d32, gc.KAHAN)
+    F = [gc.ch_free_energy(ctx, g, d32, KAPPA, gc.KAHAN)]
+    drift = []
+    for _ in range(8):
+        gc.ch_integrate(ctx, g, d32, 1.0, KAPPA, DT, 250, gc.RK4)
+        drift.append(gc.integrate(ctx, g, d32, gc.KAHAN) - m0)
+        F.append(gc.ch_free_energy(ctx, g, d32, KAPPA, gc.KAHAN))
+    print("mass drift / sum|c0|:", [f"{x / sum_abs:.2e}" for x in drift])
+    print("F:", F)
+    assert max(abs(x) for x in drift) <= 1e-7 * sum_abs
+    assert all(b <= a for a, b in zip(F, F[1:])), F
+    d32.free()
+
+
+# ---- diffusion ---------------------------------------------------------------------------
+
+# (16,16,16) and (70,9,40): two-stage 3-D kernels; (8,12,5): one-stage 3-D kernel;
+# (32,24,1) and (40,8,1): the untiled kernel (Nz < 4)
+DIFF_SHAPES = [(16, 16, 16), (70, 9, 40), (8, 12, 5), (32, 24, 1)]
+
+
+@pytest.mark.parametrize("integ", [0, 1], ids=["euler", "rk4"])
+@pytest.mark.parametrize("shape", DIFF_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_diffuse_const_fp32_vs_fp64_oracle(gc, orc, ctx, integ, shape):
+    h = 2 * np.pi / shape[0]
+    g, og = gc.Grid.make(*shape, h), orc.Grid.make(*shape, h)
+    psi = gc.inputs.symmetric(31, g.size).reshape(g.shape).astype(np.float32)
+    D, dt = 0.1, 0.2 * h * h
+    d = gc.DeviceArray.from_numpy(ctx, psi)
+    gc.diffuse_const(ctx, g, d, D, dt, 25, integ)
+    want = orc.diffuse_const(og, psi.astype(np.float64), D, dt, 25, integ)
+    l2, linf = _rel_errors(d.numpy(ctx), want)
+    assert l2 < L2_BOUND and linf < LINF_BOUND, (l2, linf)
+
+
+@pytest.mark.parametrize("integ", [0, 1], ids=["euler", "rk4"])
+@pytest.mark.parametrize("shape", DIFF_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_diffuse_field_fp32_vs_fp64_oracle(gc, orc, ctx, integ, shape):
+    h = 2 * np.pi / shape[0]
+    g, og = gc.Grid.make(*shape, h), orc.Grid.make(*shape, h)
+    psi = gc.inputs.symmetric(41, g.size).reshape(g.shape).astype(np.float32)
+    D = (0.05 + 0.1 * gc.inputs.uniform(42, g.size).reshape(g.shape)).astype(np.float32)
+    dt = 0.3 * h * h
+    d = gc.DeviceArray.from_numpy(ctx, psi)
+    gc.diffuse_field(ctx, g, d, gc.DeviceArray.from_numpy(ctx, D), dt, 20, integ)
+    want = orc.diffuse_field(og, psi.astype(np.float64), D.astype(np.float64), dt, 20, integ)
+    l2, linf = _rel_errors(d.numpy(ctx), want)
+    assert l2 < L2_BOUND and linf < LINF_BOUND, (l2, linf)
+
+
+def test_max_face_d_known_answers_device(gc, ctx):
+    """The device scan behind the heterogeneous CFL check, on the oracle's hand-computed
+    cases: a 1/9 checkerboard (every face 1.8) and one large face per axis (8)."""
+    for dtype in (np.float64, np.float32):
+        g = gc.Grid.make(2, 2, 2)
+        i, j, k = np.meshgrid(np.arange(2), np.arange(2), np.arange(2), indexing="ij")
+        D = np.where((i + j + k) % 2 == 0, 1.0, 9.0).transpose(2, 1, 0)
+        assert gc.max_face_d(ctx, g, gc.DeviceArray.from_numpy(ctx, D.astype(dtype))) == 1.8
+        g = gc.Grid.make(5, 6, 7)
+        for axis in range(3):
+            D = np.ones(g.shape)
+            a = [2, 3, 4]
+            b = list(a)
+            b[axis] += 1
+            D[a[2], a[1], a[0]] = D[b[2], b[1], b[0]] = 8.0
+            assert gc.max_face_d(ctx, g, gc.DeviceArray.from_numpy(ctx, D.astype(dtype))) == 8.0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ch_rhs_two_mode_closed_form_device(gc, ctx, dtype):
+    """The device right-hand side on A sin(k.x + phase): the exact two-mode closed form
+    (sin^3 = (3 sin - sin 3th)/4 and the O2 symbol per mode), FP64 to 1e-13 and FP32 to 1e-5
+    of the largest value."""
+    nx, ny, nz = 12, 10, 9
+    hx, hy, hz = 0.8, 1.1, 1.3
+    k3, A, M, kappa = (2, 1, 4), 0.9, 1.3, 0.7
+    i = np.arange(nx)[None, None, :]
+    j = np.arange(ny)[None, :, None]
+    k = np.arange(nz)[:, None, None]
+    th = 2 * np.pi * (k3[0] * i / nx + k3[1] * j / ny + k3[2] * k / nz) + 0.3
+    psi = A * np.sin(th)
+
+    def sym(m):
+        return sum((2 * np.cos(2 * np.pi * m * kk / nn) - 2) / (hh * hh)
+                   for nn, hh, kk in zip((nx, ny, nz), (hx, hy, hz), k3))
+    L1, L3 = sym(1), sym(3)
+    want = M * (L1 * (-A - kappa * L1 * A + 0.75 * A ** 3) * np.sin(th)
+                - L3 * 0.25 * A ** 3 * np.sin(3 * th))
+    g = gc.Grid.make(nx, ny, nz, (hx, hy, hz))
+    got = gc.ch_rhs(ctx, g, gc.DeviceArray.from_numpy(ctx, psi.astype(dtype)), M, kappa).numpy(ctx)
+    tol = 1e-13 if dtype == np.float64 else 1e-5
+    assert np.abs(got - want).max() < tol * np.abs(want).max()
