@@ -86,9 +86,15 @@ def test_ch3d_fp32_every_family_vs_fp64_oracle(gc, orc, ctx, ch3d_oracle, shape)
 def test_ch3d_fp32_256_properties(gc, ctx):
     """The full 3-D configuration's grid (256^3, FP32 stages, default tile): over 2000 steps
     in 8 diagnostic chunks the FP64 Kahan mass stays within 1e-7 of sum|c0| of its start and
-    the FP64 free energy never increases; after the first 200 steps the FP32 state is within
-    the L2/Linf bounds of an FP64 GPU run of the same steps (the FP64 pipeline equals the
-    oracle bit for bit on every tested shape)."""
+    the FP64 free energy never increases from the first diagnostic (t = 1.25) on; after the
+    first 200 steps the FP32 state is within the L2/Linf bounds of an FP64 GPU run of the same
+    steps (the FP64 pipeline equals the oracle bit for bit on every tested shape).
+
+    From white noise the free energy as the reference defines it (central-difference gradient)
+    first rises: the decaying near-Nyquist modes carry negative energy under that gradient,
+    while the dynamics dissipate the energy built on the compact Laplacian
+    (test_oracle.py::test_ch_energy_rise_from_white_noise pins this on the FP64 oracle), so
+    the t = 0 value is not part of the monotone sequence."""
     n = 256
     g = gc.Grid.make(n, n, n)
     c0 = gc.inputs.ch_initial(n, n, n, AMP, seed=3).astype(np.float32)
@@ -112,7 +118,7 @@ def test_ch3d_fp32_256_properties(gc, ctx):
     print("mass drift / sum|c0|:", [f"{x / sum_abs:.2e}" for x in drift])
     print("F:", F)
     assert max(abs(x) for x in drift) <= 1e-7 * sum_abs
-    assert all(b <= a for a, b in zip(F, F[1:])), F
+    assert all(b <= a for a, b in zip(F[1:], F[2:])), F
     d32.free()
 
 
