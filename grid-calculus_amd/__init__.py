@@ -122,6 +122,7 @@ def lib():
         "gc_first_nonpositive": ([vp, gp, C.c_int, vp, i64p], st),
         "gc_max_face_d": ([vp, gp, C.c_int, vp, dp], st),
         "gc_integrate": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
+        "gc_integrate_noalloc": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
         "gc_ch_free_energy": ([vp, gp, C.c_int, C.c_double, C.c_int, vp, dp], st),
         "gc_gradient_energy": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_axpy": ([vp, C.c_int, sz, vp, C.c_double, vp, vp], st),
@@ -370,6 +371,13 @@ def integrate(ctx, g, f, method=PAIRWISE):
     r = C.c_double()
     _check(lib().gc_integrate(ctx.handle, C.byref(g), _dt(f), method, f.ptr, C.byref(r)))
     return r.value
+
+
+def integrate_noalloc(ctx, g, f, method, result_dev):
+    """gc_integrate_noalloc: cellVolume * sum of f written to the device array result_dev
+    (one float64), queued on the context's stream without allocating or synchronising."""
+    _check(lib().gc_integrate_noalloc(ctx.handle, C.byref(g), _dt(f), method, f.ptr,
+                                      result_dev.ptr))
 
 
 def ch_free_energy(ctx, g, psi, kappa, method=PAIRWISE):

@@ -17,6 +17,7 @@
 
 #include <climits>
 #include <cstring>
+#include <mutex>
 
 namespace gc {
 namespace {
@@ -173,34 +174,57 @@ __global__ __launch_bounds__(256) void k_tree_pass(const double* __restrict__ in
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-template <class L>
-gc_status pairwise(gc_context* ctx, L ld, size_t n, double* result) {
+// Scratch of the pairwise reduction of n values: two ping-pong arrays of node sums.
+inline size_t pairwise_scratch_bytes(size_t n, unsigned* nb_out, int* depth_out) {
   int E = 0;
   while (((n + ((size_t)1 << E) - 1) >> E) > 64) ++E;
   const int depth = E > 0 ? E - 1 : 0;
   const unsigned P = 1u << depth;
   const unsigned per_block = 64u * kWaves;
   const unsigned nb = (P + per_block - 1) / per_block;
-  void* scr = nullptr;
-  GC_TRY(scratch(ctx, 2 * (size_t)nb * sizeof(double) + 256, &scr));
+  if (nb_out) *nb_out = nb;
+  if (depth_out) *depth_out = depth;
+  return 2 * (size_t)nb * sizeof(double) + 256;
+}
+
+// Launches the pairwise reduction on `stream` with scratch `scr` (at least
+// pairwise_scratch_bytes(n) bytes); *final points at the device double holding the sum once
+// the launches complete.
+template <class L>
+gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, const double** final) {
+  unsigned nb = 0;
+  int depth = 0;
+  pairwise_scratch_bytes(n, &nb, &depth);
+  const unsigned P = 1u << depth;
+  const unsigned per_block = 64u * kWaves;
   double* a = (double*)scr;
   double* b = a + nb;
   const int maxsize = (int)((n + ((size_t)1 << depth) - 1) >> depth);
-  hipLaunchKernelGGL((k_pairwise_leaves<L>), dim3(nb), dim3(per_block), 0, ctx->stream, ld, n,
-                     depth, P, maxsize, a);
+  hipLaunchKernelGGL((k_pairwise_leaves<L>), dim3(nb), dim3(per_block), 0, stream, ld, n, depth,
+                     P, maxsize, a);
   GC_LAUNCHED();
   unsigned cnt = nb;
   while (cnt > 1) {
     const unsigned g = cnt < 256u ? cnt : 256u;
     const unsigned blocks = cnt / g;
-    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(256), 0, ctx->stream, a, cnt, b);
+    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(256), 0, stream, a, cnt, b);
     GC_LAUNCHED();
     cnt = blocks;
     double* t = a;
     a = b;
     b = t;
   }
-  GC_HIP(hipMemcpyAsync(ctx->host_result, a, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  *final = a;
+  return GC_OK;
+}
+
+template <class L>
+gc_status pairwise(gc_context* ctx, L ld, size_t n, double* result) {
+  void* scr = nullptr;
+  GC_TRY(scratch(ctx, pairwise_scratch_bytes(n, nullptr, nullptr), &scr));
+  const double* fin = nullptr;
+  GC_TRY(pairwise_launch(ctx->stream, ld, n, scr, &fin));
+  GC_HIP(hipMemcpyAsync(ctx->host_result, fin, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   GC_HIP(hipStreamSynchronize(ctx->stream));
   *result = ctx->host_result[0];
   return GC_OK;
@@ -260,19 +284,34 @@ __global__ __launch_bounds__(256) void k_comp_final(const Comp* __restrict__ par
   if (threadIdx.x == 0) out[0] = a.s + a.c;
 }
 
-template <class L>
-gc_status compensated(gc_context* ctx, L ld, size_t n, double* result) {
+inline unsigned comp_blocks(size_t n) {
   const size_t want = (n + 255) / 256;
-  const unsigned nb = (unsigned)(want < 2048 ? (want == 0 ? 1 : want) : 2048);
-  void* scr = nullptr;
-  GC_TRY(scratch(ctx, (size_t)nb * sizeof(Comp) + 256, &scr));
+  return (unsigned)(want < 2048 ? (want == 0 ? 1 : want) : 2048);
+}
+
+// Scratch of the compensated reduction: one (sum, compensation) pair per block + the result.
+inline size_t comp_scratch_bytes(size_t n) { return (size_t)comp_blocks(n) * sizeof(Comp) + 256; }
+
+template <class L>
+gc_status compensated_launch(hipStream_t stream, L ld, size_t n, void* scr, const double** final) {
+  const unsigned nb = comp_blocks(n);
   Comp* part = (Comp*)scr;
   double* out = (double*)(part + nb);
-  hipLaunchKernelGGL((k_comp_partial<L>), dim3(nb), dim3(256), 0, ctx->stream, ld, n, part);
+  hipLaunchKernelGGL((k_comp_partial<L>), dim3(nb), dim3(256), 0, stream, ld, n, part);
   GC_LAUNCHED();
-  hipLaunchKernelGGL(k_comp_final, dim3(1), dim3(256), 0, ctx->stream, part, nb, out);
+  hipLaunchKernelGGL(k_comp_final, dim3(1), dim3(256), 0, stream, part, nb, out);
   GC_LAUNCHED();
-  GC_HIP(hipMemcpyAsync(ctx->host_result, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  *final = out;
+  return GC_OK;
+}
+
+template <class L>
+gc_status compensated(gc_context* ctx, L ld, size_t n, double* result) {
+  void* scr = nullptr;
+  GC_TRY(scratch(ctx, comp_scratch_bytes(n), &scr));
+  const double* fin = nullptr;
+  GC_TRY(compensated_launch(ctx->stream, ld, n, scr, &fin));
+  GC_HIP(hipMemcpyAsync(ctx->host_result, fin, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   GC_HIP(hipStreamSynchronize(ctx->stream));
   *result = ctx->host_result[0];
   return GC_OK;
@@ -454,11 +493,87 @@ gc_status free_energy_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, double
                 sum_out);
 }
 
+// ---- integrate() without allocation --------------------------------------------------------
+//
+// gc_integrate_noalloc runs gc_integrate's kernels with their partial sums in a fixed device
+// array instead of the context's scratch: sized for the largest grid the library accepts
+// (2^31-1 sites: 2^24 pairwise nodes -> 2 x 65536 node sums; the compensated sum needs
+// 2048 x 16 B), plus the final scaled result slot.  Every kernel writes each partial before
+// any kernel reads it, so nothing needs clearing between calls.  One array per device, shared
+// by every call on that device.
+
+constexpr size_t kNoAllocScratchBytes = (size_t(1) << 20) + 4096;
+
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of this device's noalloc_scratch (looked up once per device, with the
+// device current; the host-side symbol is not a device address).
+gc_status noalloc_scratch_ptr(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(GC_ERR_UNSUPPORTED, "gridcalc: device ordinal %d beyond %d", device,
+                     kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    GC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    g_noalloc_ptr[device] = p;
+  }
+  *out = g_noalloc_ptr[device];
+  return GC_OK;
+}
+
+__global__ void k_store_scaled(const double* __restrict__ sum, double vol,
+                               double* __restrict__ out) {
+  out[0] = vol * sum[0];
+}
+
+}  // namespace
+
 }  // namespace gc
 
 using namespace gc;
 
 extern "C" {
+
+gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
+                               const void* in, double* result_dev) {
+  GC_TRY(field_checks(ctx, g, dt, in));
+  GC_TRY(check_method(method));
+  if (!result_dev) return set_error(GC_ERR_INVALID, "gridcalc: null result pointer");
+  const size_t n = sites(g);
+  const size_t need = (method == GC_PAIRWISE ? pairwise_scratch_bytes(n, nullptr, nullptr)
+                                             : comp_scratch_bytes(n));
+  if (need > kNoAllocScratchBytes)
+    return set_error(GC_ERR_UNSUPPORTED,
+                     "gridcalc: gc_integrate_noalloc needs %zu bytes of partial sums for %zu "
+                     "sites; its fixed scratch holds %zu",
+                     need, n, kNoAllocScratchBytes);
+  void* scr = nullptr;
+  GC_TRY(noalloc_scratch_ptr(ctx->device, &scr));
+  const double* fin = nullptr;
+  if (method == GC_PAIRWISE) {
+    if (dt == GC_F64)
+      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, &fin));
+    else
+      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, &fin));
+  } else {
+    if (dt == GC_F64)
+      GC_TRY(compensated_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, &fin));
+    else
+      GC_TRY(compensated_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, &fin));
+  }
+  // func/Integrate.hpp:118-119: getCellVolume() * sum, cell volume (hx*hy)*hz
+  hipLaunchKernelGGL(k_store_scaled, dim3(1), dim3(1), 0, ctx->stream, fin,
+                     g->hx * g->hy * g->hz, result_dev);
+  GC_LAUNCHED();
+  return GC_OK;
+}
 
 gc_status gc_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method, const void* in,
                        double* result) {

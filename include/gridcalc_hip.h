@@ -137,6 +137,18 @@ gc_status gc_max_face_d(gc_context* ctx, const gc_grid* g, gc_dtype dt, const vo
  * parallel sum in double precision. */
 gc_status gc_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
                        const void* in, double* result);
+/* No counterpart in the reference: func::integrate for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and the same result bits as gc_integrate, written as one double
+ * (cellVolume * sum) to the device address result_dev, asynchronously on the context's
+ * stream: no allocation, no host copy, no synchronisation.  Partial sums live in a fixed
+ * per-device static array of about 1 MiB, enough for every grid the library accepts
+ * (sizes beyond it fail with GC_ERR_UNSUPPORTED).  All calls on one device share that array:
+ * calls that overlap on one GPU (from different streams or contexts) race; calls ordered on
+ * one stream do not.  The context must have been created beforehand (gc_context_create
+ * allocates). */
+gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
+                               const void* in, double* result_dev);
 /* Cahn-Hilliard free energy func::evaluate(psi, f(p, grad p)) with
  * f = -p^2/2 + p^4/4 + kappa/2 |grad p|^2 (func/Functional.hpp:122-130,
  * examples/common/cahn_hilliard.hpp:64-68), fused: gradient + density + reduction. */
