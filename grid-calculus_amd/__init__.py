@@ -74,6 +74,16 @@ class CHParams(C.Structure):
     _fields_ = [("M", C.c_double), ("kappa", C.c_double)]
 
 
+class Density(C.Structure):
+    """gc_density: a recognised func::evaluate density and its two parameters."""
+    _fields_ = [("kind", C.c_int32), ("a", C.c_double), ("b", C.c_double)]
+
+
+# gc_density_kind
+DENSITY = dict(value=0, gradient_energy=1, cahn_hilliard=2, ginzburg_landau=3, pfc=4,
+               linear_elastic=5)
+
+
 def build(force=False):
     """Compiles libgridcalc_hip.so in-tree for gfx950 (hipcc via make)."""
     args = ["make", "-s", "-C", PKG_DIR, "-j8"]
@@ -123,6 +133,7 @@ def lib():
         "gc_max_face_d": ([vp, gp, C.c_int, vp, dp], st),
         "gc_integrate": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_integrate_noalloc": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_evaluate": ([vp, gp, C.c_int, C.POINTER(Density), C.c_int, vp, dp], st),
         "gc_ch_free_energy": ([vp, gp, C.c_int, C.c_double, C.c_int, vp, dp], st),
         "gc_gradient_energy": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_axpy": ([vp, C.c_int, sz, vp, C.c_double, vp, vp], st),
@@ -383,6 +394,16 @@ def integrate_noalloc(ctx, g, f, method, result_dev):
 def ch_free_energy(ctx, g, psi, kappa, method=PAIRWISE):
     r = C.c_double()
     _check(lib().gc_ch_free_energy(ctx.handle, C.byref(g), _dt(psi), kappa, method, psi.ptr, C.byref(r)))
+    return r.value
+
+
+def evaluate(ctx, g, kind, field, a=0.0, b=0.0, method=PAIRWISE):
+    """func::evaluate of a recognised density (a name of DENSITY) on a scalar field, or on a
+    Vec3 field (shape (..., 3)) for "linear_elastic"; one fused pass on the device."""
+    r = C.c_double()
+    D = Density(DENSITY[kind], a, b)
+    _check(lib().gc_evaluate(ctx.handle, C.byref(g), _dt(field), C.byref(D), method, field.ptr,
+                             C.byref(r)))
     return r.value
 
 

@@ -45,7 +45,8 @@ struct gc_context {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  double* host_result = nullptr;  // pinned, small: reduction results
+  double* host_result = nullptr;      // pinned, small: reduction results
+  double* host_result_dev = nullptr;  // its device address (final reduction kernels store here)
 };
 
 namespace gc {

@@ -136,7 +136,8 @@ gc_status gc_context_create(gc_context** out, void* stream) {
     ctx->owns_stream = true;
   }
   if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->host_result, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->host_result, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&ctx->host_result_dev, ctx->host_result, 0) != hipSuccess) {
     (void)hipGetLastError();
     gc_context_destroy(ctx);
     return set_error(GC_ERR_DEVICE, "gridcalc: context set-up failed");

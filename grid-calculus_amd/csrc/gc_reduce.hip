@@ -34,58 +34,186 @@ struct LoadPlain {
   __device__ __forceinline__ double operator()(size_t c) const { return (double)x[c]; }
 };
 
-// Gradient (Order 2, diff/Gradient.hpp:59-69) of psi at flat index c, in double.
-template <class T>
-__device__ __forceinline__ void grad2_at(const T* p, const Dims& d, size_t c, double ihx,
-                                         double ihy, double ihz, double& gx, double& gy,
-                                         double& gz) {
-  const int i = (int)(c % (size_t)d.nx);
+// ---- per-site derivatives for fused densities ---------------------------------------------
+//
+// func::evaluate (func/Functional.hpp:88-148) materialises the Order-2 gradient, Laplacian and
+// biharmonic of psi and applies the density site by site.  Here the reduction's load computes
+// them at the site it reads, in double precision from the stored values, with the operators'
+// accumulation order (per-axis sums in stencil-offset order from 0, then the scaled terms left
+// to right), so each value equals the materialised operator's bit for bit.
+
+struct SiteIdx {
+  int i, j, k;
+};
+
+__device__ __forceinline__ SiteIdx site_of(const Dims& d, size_t c) {
   const size_t r = c / (size_t)d.nx;
-  const int j = (int)(r % (size_t)d.ny);
-  const int k = (int)(r / (size_t)d.ny);
-  const size_t row = (size_t)j * d.nx, pl = (size_t)k * d.plane;
+  return SiteIdx{(int)(c - r * (size_t)d.nx), (int)(r % (size_t)d.ny), (int)(r / (size_t)d.ny)};
+}
+
+template <class T>
+struct Sampler {  // wrapped read of component `comp` of an NC-component AoS field
+  const T* p;
+  Dims d;
+  __device__ __forceinline__ double operator()(int i, int j, int k, int NC = 1, int comp = 0) const {
+    const size_t s = (size_t)wrap(k, d.nz) * d.plane + (size_t)wrap(j, d.ny) * d.nx + wrap(i, d.nx);
+    return (double)p[s * NC + comp];
+  }
+};
+
+// diff/Gradient.hpp:59-69 (Order 2) of component comp of an NC-component field
+template <class T>
+__device__ __forceinline__ void grad2_site(const Sampler<T>& P, SiteIdx s, double ihx, double ihy,
+                                           double ihz, double& gx, double& gy, double& gz,
+                                           int NC = 1, int comp = 0) {
   double dx = 0.0, dy = 0.0, dz = 0.0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const int o = m - 1;
     const double w = D1<2>::w(m);
-    dx += w * (double)p[pl + row + wrap(i + o, d.nx)];
-    dy += w * (double)p[pl + (size_t)wrap(j + o, d.ny) * d.nx + i];
-    dz += w * (double)p[(size_t)wrap(k + o, d.nz) * d.plane + row + i];
+    dx += w * P(s.i + o, s.j, s.k, NC, comp);
+    dy += w * P(s.i, s.j + o, s.k, NC, comp);
+    dz += w * P(s.i, s.j, s.k + o, NC, comp);
   }
   gx = dx * ihx;
   gy = dy * ihy;
   gz = dz * ihz;
 }
 
-// examples/common/cahn_hilliard.hpp:65-67:
-//   -0.5*p*p + 0.25*p*p*p*p + 0.5*kappa*g.squaredNorm(),  |g|^2 = (gx^2 + gy^2) + gz^2
+// diff/Laplacian.hpp:56-71 (Order 2)
 template <class T>
-struct LoadCHDensity {
+__device__ __forceinline__ double lap2_site(const Sampler<T>& P, SiteIdx s, double ihx2,
+                                            double ihy2, double ihz2) {
+  double dxx = 0.0, dyy = 0.0, dzz = 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int o = m - 1;
+    const double w = D2<2>::w(m);
+    dxx += w * P(s.i + o, s.j, s.k);
+    dyy += w * P(s.i, s.j + o, s.k);
+    dzz += w * P(s.i, s.j, s.k + o);
+  }
+  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+}
+
+// diff/Biharmonic.hpp:67-101 (Order 2): the fourth-derivative loop, the mixed loop (mb outer,
+// ma inner), then the final expression as written
+template <class T>
+__device__ __forceinline__ double bih2_site(const Sampler<T>& P, SiteIdx s, double ihx2,
+                                            double ihy2, double ihz2, double ihx4, double ihy4,
+                                            double ihz4) {
+  double dx4 = 0.0, dy4 = 0.0, dz4 = 0.0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    const int o = m - 2;
+    const double w = D4<2>::w(m);
+    dx4 += w * P(s.i + o, s.j, s.k);
+    dy4 += w * P(s.i, s.j + o, s.k);
+    dz4 += w * P(s.i, s.j, s.k + o);
+  }
+  double dxy = 0.0, dxz = 0.0, dyz = 0.0;
+#pragma unroll
+  for (int mb = 0; mb < 3; ++mb) {
+    const int ob = mb - 1;
+    const double wb = D2<2>::w(mb);
+#pragma unroll
+    for (int ma = 0; ma < 3; ++ma) {
+      const int oa = ma - 1;
+      const double wab = D2<2>::w(ma) * wb;
+      dxy += wab * P(s.i + oa, s.j + ob, s.k);
+      dxz += wab * P(s.i + oa, s.j, s.k + ob);
+      dyz += wab * P(s.i, s.j + oa, s.k + ob);
+    }
+  }
+  return dx4 * ihx4 + dy4 * ihy4 + dz4 * ihz4 +
+         2.0 * (dxy * ihx2 * ihy2 + dxz * ihx2 * ihz2 + dyz * ihy2 * ihz2);
+}
+
+struct DensityCoef {
+  double ihx, ihy, ihz;     // 1/h per axis (gradient)
+  double ihx2, ihy2, ihz2;  // 1/h^2 (Laplacian, biharmonic)
+  double ihx4, ihy4, ihz4;  // (1/h^2)^2 (biharmonic)
+  double a, b;              // the density's parameters (gc_density)
+};
+
+// The density of gc_density kind K at flat site c.  Each formula is the one its C++
+// counterpart in func/Densities.hpp evaluates, operation for operation:
+//  VALUE             p                                             (arity 1)
+//  GRADIENT_ENERGY   |g|^2 = (gx*gx + gy*gy) + gz*gz               (arity 2)
+//  CAHN_HILLIARD     -0.5*p*p + 0.25*p*p*p*p + 0.5*a*|g|^2         (arity 2, a = kappa)
+//  GINZBURG_LANDAU   q = p*p - 1; 0.5*|g|^2 + a*q*q                (arity 3, a = W)
+//  PFC               q2 = a*a; q4 = q2*q2; p2 = p*p;               (arity 4, a = q0, b = eps)
+//                    0.5*(q4 - b)*p2 + q2*p*lap + 0.5*p*bih + 0.25*p2*p2
+//  LINEAR_ELASTIC    e = 0.5*(G + G^T), G(r,c) = d u_r/d x_c;      (Vec3 field, a = lambda,
+//                    tr = (e00 + e11) + e22; ee = sum of e_i*e_i in   b = mu)
+//                    column-major order; 0.5*(a*tr*tr + 2.0*b*ee)
+template <class T, int K>
+struct LoadDensity {
   const T* p;
   Dims d;
-  double ihx, ihy, ihz, kappa;
+  DensityCoef k;
   __device__ __forceinline__ double operator()(size_t c) const {
-    double gx, gy, gz;
-    grad2_at(p, d, c, ihx, ihy, ihz, gx, gy, gz);
-    const double v = (double)p[c];
-    const double sq = gx * gx + gy * gy + gz * gz;
-    return -0.5 * v * v + 0.25 * v * v * v * v + 0.5 * kappa * sq;
+    const Sampler<T> P{p, d};
+    if constexpr (K == GC_DENSITY_VALUE) {
+      return (double)p[c];
+    } else if constexpr (K == GC_DENSITY_LINEAR_ELASTIC) {
+      const SiteIdx s = site_of(d, c);
+      double G[9];  // column-major: G(r, c) at r + 3c
+#pragma unroll
+      for (int r = 0; r < 3; ++r) grad2_site(P, s, k.ihx, k.ihy, k.ihz, G[r], G[r + 3], G[r + 6], 3, r);
+      double e[9];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) e[r + 3 * cc] = 0.5 * (G[r + 3 * cc] + G[cc + 3 * r]);
+      const double tr = e[0] + e[4] + e[8];
+      double ee = e[0] * e[0];
+#pragma unroll
+      for (int q = 1; q < 9; ++q) ee += e[q] * e[q];
+      return 0.5 * (k.a * tr * tr + 2.0 * k.b * ee);
+    } else {
+      const SiteIdx s = site_of(d, c);
+      const double v = (double)p[c];
+      if constexpr (K == GC_DENSITY_PFC) {
+        const double lap = lap2_site(P, s, k.ihx2, k.ihy2, k.ihz2);
+        const double bih = bih2_site(P, s, k.ihx2, k.ihy2, k.ihz2, k.ihx4, k.ihy4, k.ihz4);
+        const double q2 = k.a * k.a;
+        const double q4 = q2 * q2;
+        const double p2 = v * v;
+        return 0.5 * (q4 - k.b) * p2 + q2 * v * lap + 0.5 * v * bih + 0.25 * p2 * p2;
+      } else {
+        double gx, gy, gz;
+        grad2_site(P, s, k.ihx, k.ihy, k.ihz, gx, gy, gz);
+        const double sq = gx * gx + gy * gy + gz * gz;
+        if constexpr (K == GC_DENSITY_GRADIENT_ENERGY) {
+          return sq;
+        } else if constexpr (K == GC_DENSITY_CAHN_HILLIARD) {
+          return -0.5 * v * v + 0.25 * v * v * v * v + 0.5 * k.a * sq;
+        } else {
+          static_assert(K == GC_DENSITY_GINZBURG_LANDAU, "density kind");
+          const double q = v * v - 1.0;
+          return 0.5 * sq + k.a * q * q;
+        }
+      }
+    }
   }
 };
 
-// examples/common/cahn_hilliard.hpp:74-76: g.squaredNorm()
-template <class T>
-struct LoadGradSq {
-  const T* p;
-  Dims d;
-  double ihx, ihy, ihz;
-  __device__ __forceinline__ double operator()(size_t c) const {
-    double gx, gy, gz;
-    grad2_at(p, d, c, ihx, ihy, ihz, gx, gy, gz);
-    return gx * gx + gy * gy + gz * gz;
-  }
-};
+inline DensityCoef density_coef(const gc_grid* g, double a, double b) {
+  DensityCoef k;
+  k.ihx = 1.0 / g->hx;
+  k.ihy = 1.0 / g->hy;
+  k.ihz = 1.0 / g->hz;
+  k.ihx2 = 1.0 / (g->hx * g->hx);
+  k.ihy2 = 1.0 / (g->hy * g->hy);
+  k.ihz2 = 1.0 / (g->hz * g->hz);
+  k.ihx4 = k.ihx2 * k.ihx2;
+  k.ihy4 = k.ihy2 * k.ihy2;
+  k.ihz4 = k.ihz2 * k.ihz2;
+  k.a = a;
+  k.b = b;
+  return k;
+}
 
 // ---- exact pairwise tree ------------------------------------------------------------
 
@@ -117,7 +245,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_pairwise_leaves(L ld, size_t n,
   __shared__ int nsize[kWaves][64];
   __shared__ double sh[kWaves];
   const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const unsigned q = blockIdx.x * (64u * kWaves) + threadIdx.x;
+  // logical block: each XCD takes one contiguous range of nodes, so the neighbour rows and
+  // planes a stencil-reading load touches stay in that XCD's L2
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned q = blk * (64u * kWaves) + threadIdx.x;
 
   size_t off = 0, size = 0;
   if (q < P) {
@@ -160,7 +291,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_pairwise_leaves(L ld, size_t n,
   double s = (split == INT_MAX) ? s0 : s0 + s1;
   const unsigned cnt = P < 64u * kWaves ? P : 64u * kWaves;
   s = block_tree_sum(s, cnt, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial[blk] = s;
 }
 
 // Perfect-tree sum of consecutive groups of min(cnt, 256) values.
@@ -188,10 +319,10 @@ inline size_t pairwise_scratch_bytes(size_t n, unsigned* nb_out, int* depth_out)
 }
 
 // Launches the pairwise reduction on `stream` with scratch `scr` (at least
-// pairwise_scratch_bytes(n) bytes); *final points at the device double holding the sum once
-// the launches complete.
+// pairwise_scratch_bytes(n) bytes); the sum lands in *final_out (device memory, or the
+// device address of pinned host memory) once the launches complete.
 template <class L>
-gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, const double** final) {
+gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, double* final_out) {
   unsigned nb = 0;
   int depth = 0;
   pairwise_scratch_bytes(n, &nb, &depth);
@@ -201,20 +332,20 @@ gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, const d
   double* b = a + nb;
   const int maxsize = (int)((n + ((size_t)1 << depth) - 1) >> depth);
   hipLaunchKernelGGL((k_pairwise_leaves<L>), dim3(nb), dim3(per_block), 0, stream, ld, n, depth,
-                     P, maxsize, a);
+                     P, maxsize, nb == 1 ? final_out : a);
   GC_LAUNCHED();
   unsigned cnt = nb;
   while (cnt > 1) {
     const unsigned g = cnt < 256u ? cnt : 256u;
     const unsigned blocks = cnt / g;
-    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(256), 0, stream, a, cnt, b);
+    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(256), 0, stream, a, cnt,
+                       blocks == 1 ? final_out : b);
     GC_LAUNCHED();
     cnt = blocks;
     double* t = a;
     a = b;
     b = t;
   }
-  *final = a;
   return GC_OK;
 }
 
@@ -222,9 +353,7 @@ template <class L>
 gc_status pairwise(gc_context* ctx, L ld, size_t n, double* result) {
   void* scr = nullptr;
   GC_TRY(scratch(ctx, pairwise_scratch_bytes(n, nullptr, nullptr), &scr));
-  const double* fin = nullptr;
-  GC_TRY(pairwise_launch(ctx->stream, ld, n, scr, &fin));
-  GC_HIP(hipMemcpyAsync(ctx->host_result, fin, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  GC_TRY(pairwise_launch(ctx->stream, ld, n, scr, ctx->host_result_dev));
   GC_HIP(hipStreamSynchronize(ctx->stream));
   *result = ctx->host_result[0];
   return GC_OK;
@@ -265,14 +394,20 @@ __device__ __forceinline__ Comp block_comp_sum(Comp v, Comp* sh) {
   return v;
 }
 
+// Logical block b sums the contiguous range [b*chunk, (b+1)*chunk) (XCD-aware order: each
+// XCD takes one contiguous run of blocks, so stencil-reading loads find their neighbours in
+// that XCD's L2); the partition depends on n only, so the result is deterministic.
 template <class L>
-__global__ __launch_bounds__(256) void k_comp_partial(L ld, size_t n, Comp* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_comp_partial(L ld, size_t n, size_t chunk,
+                                                      Comp* __restrict__ partial) {
   __shared__ Comp sh[4];
-  const size_t G = (size_t)gridDim.x * 256u;
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const size_t begin = (size_t)blk * chunk;
+  const size_t end = begin + chunk < n ? begin + chunk : n;
   Comp a{0.0, 0.0};
-  for (size_t c = (size_t)blockIdx.x * 256u + threadIdx.x; c < n; c += G) neu_add(a, ld(c));
+  for (size_t c = begin + threadIdx.x; c < end; c += 256u) neu_add(a, ld(c));
   a = block_comp_sum(a, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+  if (threadIdx.x == 0) partial[blk] = a;
 }
 
 __global__ __launch_bounds__(256) void k_comp_final(const Comp* __restrict__ partial, unsigned nb,
@@ -284,24 +419,26 @@ __global__ __launch_bounds__(256) void k_comp_final(const Comp* __restrict__ par
   if (threadIdx.x == 0) out[0] = a.s + a.c;
 }
 
+constexpr unsigned kCompMaxBlocks = 4096;
+
 inline unsigned comp_blocks(size_t n) {
   const size_t want = (n + 255) / 256;
-  return (unsigned)(want < 2048 ? (want == 0 ? 1 : want) : 2048);
+  return (unsigned)(want < kCompMaxBlocks ? (want == 0 ? 1 : want) : kCompMaxBlocks);
 }
 
-// Scratch of the compensated reduction: one (sum, compensation) pair per block + the result.
+// Scratch of the compensated reduction: one (sum, compensation) pair per block.
 inline size_t comp_scratch_bytes(size_t n) { return (size_t)comp_blocks(n) * sizeof(Comp) + 256; }
 
 template <class L>
-gc_status compensated_launch(hipStream_t stream, L ld, size_t n, void* scr, const double** final) {
+gc_status compensated_launch(hipStream_t stream, L ld, size_t n, void* scr, double* final_out) {
   const unsigned nb = comp_blocks(n);
+  const size_t per = (n + nb - 1) / nb;
+  const size_t chunk = (per + 255) / 256 * 256;
   Comp* part = (Comp*)scr;
-  double* out = (double*)(part + nb);
-  hipLaunchKernelGGL((k_comp_partial<L>), dim3(nb), dim3(256), 0, stream, ld, n, part);
+  hipLaunchKernelGGL((k_comp_partial<L>), dim3(nb), dim3(256), 0, stream, ld, n, chunk, part);
   GC_LAUNCHED();
-  hipLaunchKernelGGL(k_comp_final, dim3(1), dim3(256), 0, stream, part, nb, out);
+  hipLaunchKernelGGL(k_comp_final, dim3(1), dim3(256), 0, stream, part, nb, final_out);
   GC_LAUNCHED();
-  *final = out;
   return GC_OK;
 }
 
@@ -309,9 +446,7 @@ template <class L>
 gc_status compensated(gc_context* ctx, L ld, size_t n, double* result) {
   void* scr = nullptr;
   GC_TRY(scratch(ctx, comp_scratch_bytes(n), &scr));
-  const double* fin = nullptr;
-  GC_TRY(compensated_launch(ctx->stream, ld, n, scr, &fin));
-  GC_HIP(hipMemcpyAsync(ctx->host_result, fin, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  GC_TRY(compensated_launch(ctx->stream, ld, n, scr, ctx->host_result_dev));
   GC_HIP(hipStreamSynchronize(ctx->stream));
   *result = ctx->host_result[0];
   return GC_OK;
@@ -481,16 +616,20 @@ gc_status integrate_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, int meth
   return reduce(ctx, method, LoadPlain<float>{(const float*)in}, n, sum_out);
 }
 
-gc_status free_energy_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, double kappa,
-                          int method, const void* psi, double* sum_out) {
+template <int K>
+gc_status density_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
+                      const void* field, double a, double b, double* sum_out) {
   const size_t n = sites(g);
   const Dims d = dims_of(g);
-  const double ihx = 1.0 / g->hx, ihy = 1.0 / g->hy, ihz = 1.0 / g->hz;
+  const DensityCoef k = density_coef(g, a, b);
   if (dt == GC_F64)
-    return reduce(ctx, method, LoadCHDensity<double>{(const double*)psi, d, ihx, ihy, ihz, kappa},
-                  n, sum_out);
-  return reduce(ctx, method, LoadCHDensity<float>{(const float*)psi, d, ihx, ihy, ihz, kappa}, n,
-                sum_out);
+    return reduce(ctx, method, LoadDensity<double, K>{(const double*)field, d, k}, n, sum_out);
+  return reduce(ctx, method, LoadDensity<float, K>{(const float*)field, d, k}, n, sum_out);
+}
+
+gc_status free_energy_raw(gc_context* ctx, const gc_grid* g, gc_dtype dt, double kappa,
+                          int method, const void* psi, double* sum_out) {
+  return density_raw<GC_DENSITY_CAHN_HILLIARD>(ctx, g, dt, method, psi, kappa, 0.0, sum_out);
 }
 
 // ---- integrate() without allocation --------------------------------------------------------
@@ -556,17 +695,18 @@ gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, i
                      need, n, kNoAllocScratchBytes);
   void* scr = nullptr;
   GC_TRY(noalloc_scratch_ptr(ctx->device, &scr));
-  const double* fin = nullptr;
+  // the sum goes to the array's last 256-byte slot, past every partial (need <= capacity)
+  double* fin = (double*)((unsigned char*)scr + kNoAllocScratchBytes - 256);
   if (method == GC_PAIRWISE) {
     if (dt == GC_F64)
-      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, &fin));
+      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, fin));
     else
-      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, &fin));
+      GC_TRY(pairwise_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, fin));
   } else {
     if (dt == GC_F64)
-      GC_TRY(compensated_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, &fin));
+      GC_TRY(compensated_launch(ctx->stream, LoadPlain<double>{(const double*)in}, n, scr, fin));
     else
-      GC_TRY(compensated_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, &fin));
+      GC_TRY(compensated_launch(ctx->stream, LoadPlain<float>{(const float*)in}, n, scr, fin));
   }
   // func/Integrate.hpp:118-119: getCellVolume() * sum, cell volume (hx*hy)*hz
   hipLaunchKernelGGL(k_store_scaled, dim3(1), dim3(1), 0, ctx->stream, fin,
@@ -605,15 +745,55 @@ gc_status gc_gradient_energy(gc_context* ctx, const gc_grid* g, gc_dtype dt, int
   GC_TRY(check_method(method));
   GC_TRY(check_extent(g, 1));
   if (!E) return set_error(GC_ERR_INVALID, "gridcalc: null result pointer");
-  const size_t n = sites(g);
-  const Dims d = dims_of(g);
-  const double ihx = 1.0 / g->hx, ihy = 1.0 / g->hy, ihz = 1.0 / g->hz;
   double s = 0.0;
-  if (dt == GC_F64)
-    GC_TRY(reduce(ctx, method, LoadGradSq<double>{(const double*)psi, d, ihx, ihy, ihz}, n, &s));
-  else
-    GC_TRY(reduce(ctx, method, LoadGradSq<float>{(const float*)psi, d, ihx, ihy, ihz}, n, &s));
+  GC_TRY(density_raw<GC_DENSITY_GRADIENT_ENERGY>(ctx, g, dt, method, psi, 0.0, 0.0, &s));
   *E = (g->hx * g->hy * g->hz) * s;
+  return GC_OK;
+}
+
+gc_status gc_evaluate(gc_context* ctx, const gc_grid* g, gc_dtype dt, const gc_density* f,
+                      int method, const void* field, double* result) {
+  GC_TRY(field_checks(ctx, g, dt, field));
+  GC_TRY(check_method(method));
+  if (!f) return set_error(GC_ERR_INVALID, "gridcalc: null density");
+  if (!result) return set_error(GC_ERR_INVALID, "gridcalc: null result pointer");
+  // the operators func::evaluate materialises, in its order: gradient, Laplacian, biharmonic
+  // (Functional.hpp:100-147), each with its own extent check
+  switch (f->kind) {
+    case GC_DENSITY_VALUE: break;
+    case GC_DENSITY_GRADIENT_ENERGY:
+    case GC_DENSITY_CAHN_HILLIARD:
+    case GC_DENSITY_GINZBURG_LANDAU:
+    case GC_DENSITY_LINEAR_ELASTIC: GC_TRY(check_extent(g, 1)); break;
+    case GC_DENSITY_PFC:
+      GC_TRY(check_extent(g, 1));
+      GC_TRY(check_extent(g, 2));
+      break;
+    default: return set_error(GC_ERR_INVALID, "gridcalc: unknown density kind %d", (int)f->kind);
+  }
+  double s = 0.0;
+  switch (f->kind) {
+    case GC_DENSITY_VALUE:
+      GC_TRY(density_raw<GC_DENSITY_VALUE>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+    case GC_DENSITY_GRADIENT_ENERGY:
+      GC_TRY(density_raw<GC_DENSITY_GRADIENT_ENERGY>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+    case GC_DENSITY_CAHN_HILLIARD:
+      GC_TRY(density_raw<GC_DENSITY_CAHN_HILLIARD>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+    case GC_DENSITY_GINZBURG_LANDAU:
+      GC_TRY(density_raw<GC_DENSITY_GINZBURG_LANDAU>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+    case GC_DENSITY_PFC:
+      GC_TRY(density_raw<GC_DENSITY_PFC>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+    default:
+      GC_TRY(density_raw<GC_DENSITY_LINEAR_ELASTIC>(ctx, g, dt, method, field, f->a, f->b, &s));
+      break;
+  }
+  // func/Integrate.hpp:118-119: getCellVolume() * sum
+  *result = (g->hx * g->hy * g->hz) * s;
   return GC_OK;
 }
 

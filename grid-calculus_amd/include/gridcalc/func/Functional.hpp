@@ -1,10 +1,11 @@
 // func::evaluate (func/Functional.hpp:88-220 of the reference): integrate a density
 // f(psi, grad psi, lap psi, biharm psi) (arity 4/3/2/1, detected at compile time) or
-// f(u, grad u) / f(u) for a Vec3 field.  The derivatives (Order 2) are computed on the
-// device; a general density is a host callable, so the integrand is assembled on the host
-// and reduced on the device.  func::CahnHilliardDensity{kappa} -- the free-energy density
-// -p^2/2 + p^4/4 + kappa/2 |grad p|^2 of examples/common/cahn_hilliard.hpp:64-68 -- is
-// recognised and evaluated by one fused kernel (gradient + density + reduction).
+// f(u, grad u) / f(u) for a Vec3 field.  A density of func/Densities.hpp (value, gradient
+// energy, Cahn-Hilliard, Ginzburg-Landau, phase-field crystal, linear elastic) is evaluated
+// on the device in one fused pass (derivatives + density + reduction), with the bits the
+// host evaluation of the same formula gives.  Any other callable runs on the host: the
+// Order-2 derivatives are computed on the device, the integrand is assembled on the host and
+// reduced on the device.
 #pragma once
 
 #include <cstddef>
@@ -15,6 +16,7 @@
 #include <gridcalc/diff/Biharmonic.hpp>
 #include <gridcalc/diff/Gradient.hpp>
 #include <gridcalc/diff/Laplacian.hpp>
+#include <gridcalc/func/Densities.hpp>
 #include <gridcalc/func/Integrate.hpp>
 
 namespace gridcalc::func {
@@ -27,34 +29,37 @@ template <class Tag>
 constexpr int methodOf() {
   return std::is_same_v<Tag, Kahan> ? GC_KAHAN : GC_PAIRWISE;
 }
-}  // namespace detail
 
-// Cahn-Hilliard free-energy density with gradient coefficient kappa.
-struct CahnHilliardDensity {
-  double kappa;
-  double operator()(double p, const core::Vec3d& g) const {
-    return -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * kappa * g.squaredNorm();
-  }
-};
-
-template <class T, class Tag = Pairwise>
-inline double evaluate(const core::Field<T>& psi, const CahnHilliardDensity& f, Tag tag = {}) {
+template <class Tag>
+inline double evaluateOnDevice(const core::Grid& grid, gc_dtype dt, const gc_density& d,
+                               const void* field) {
   static_assert(std::is_same_v<Tag, Pairwise> || std::is_same_v<Tag, Kahan>,
                 "func::evaluate: Tag must be func::Pairwise or func::Kahan");
-  (void)tag;
-  const auto& grid = psi.getGrid();
-  diff::detail::requireAxisExtent("x", grid.getNx(), 1);
-  diff::detail::requireAxisExtent("y", grid.getNy(), 1);
-  diff::detail::requireAxisExtent("z", grid.getNz(), 1);
   const gc_grid g = hip::toGrid(grid);
-  double F = 0.0;
-  hip::check(gc_ch_free_energy(hip::context(), &g, hip::dtype_of<T>::value, f.kappa,
-                               detail::methodOf<Tag>(), psi.deviceData(), &F));
-  return F;
+  double r = 0.0;
+  hip::check(gc_evaluate(hip::context(), &g, dt, &d, methodOf<Tag>(), field, &r));
+  return r;
+}
+}  // namespace detail
+
+// Recognised scalar densities, FP64 or FP32 fields: one fused device pass.
+template <class T, class D, class Tag = Pairwise,
+          std::enable_if_t<is_device_density_v<D>, int> = 0>
+inline double evaluate(const core::Field<T>& psi, const D& f, Tag tag = {}) {
+  (void)tag;
+  return detail::evaluateOnDevice<Tag>(psi.getGrid(), hip::dtype_of<T>::value, f.desc(),
+                                       psi.deviceData());
 }
 
-template <class F, class Tag = Pairwise,
-          std::enable_if_t<!std::is_same_v<std::decay_t<F>, CahnHilliardDensity>, int> = 0>
+// The linear elastic energy density of a displacement field: one fused device pass.
+template <class Tag = Pairwise>
+inline double evaluate(const core::Field<core::Vec3d>& u, const LinearElasticDensity& f,
+                       Tag tag = {}) {
+  (void)tag;
+  return detail::evaluateOnDevice<Tag>(u.getGrid(), GC_F64, f.desc(), u.deviceData());
+}
+
+template <class F, class Tag = Pairwise, std::enable_if_t<!is_device_density_v<F>, int> = 0>
 inline double evaluate(const core::Field<double>& psi, F&& f, Tag tag = {}) {
   static_assert(std::is_same_v<Tag, Pairwise> || std::is_same_v<Tag, Kahan>,
                 "func::evaluate: Tag must be func::Pairwise or func::Kahan");
@@ -93,7 +98,8 @@ inline double evaluate(const core::Field<double>& psi, F&& f, Tag tag = {}) {
   return integrate(integrand, tag);
 }
 
-template <class F, class Tag = Pairwise>
+template <class F, class Tag = Pairwise,
+          std::enable_if_t<!std::is_same_v<std::decay_t<F>, LinearElasticDensity>, int> = 0>
 inline double evaluate(const core::Field<core::Vec3d>& u, F&& f, Tag tag = {}) {
   static_assert(std::is_same_v<Tag, Pairwise> || std::is_same_v<Tag, Kahan>,
                 "func::evaluate: Tag must be func::Pairwise or func::Kahan");

@@ -149,6 +149,34 @@ gc_status gc_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, int metho
  * allocates). */
 gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
                                const void* in, double* result_dev);
+/* Densities func::evaluate integrates on the device (func/Functional.hpp:88-148, the scalar
+ * overload, and :183-220, the Vec3 overload), each a formula of the Order-2 derivatives the
+ * reference materialises.  The C++ structs in gridcalc/func/Densities.hpp evaluate the same
+ * formulas on the host operation for operation (so a caller's lambda of the same expression
+ * gives the same bits).  a and b are the density's parameters. */
+typedef enum gc_density_kind {
+  GC_DENSITY_VALUE = 0,           /* f(p) = p                                    arity 1 */
+  GC_DENSITY_GRADIENT_ENERGY = 1, /* f(p, g) = |g|^2                             arity 2 */
+  GC_DENSITY_CAHN_HILLIARD = 2,   /* -p^2/2 + p^4/4 + a/2 |g|^2 (a = kappa)       arity 2 */
+  GC_DENSITY_GINZBURG_LANDAU = 3, /* |g|^2/2 + a (p^2 - 1)^2 (a = W)              arity 3 */
+  GC_DENSITY_PFC = 4,             /* (q0^4 - eps)/2 p^2 + q0^2 p lap p + p bih p / 2 + p^4/4
+                                     (a = q0, b = eps)                           arity 4 */
+  GC_DENSITY_LINEAR_ELASTIC = 5   /* Vec3 field u: e = sym(grad u),
+                                     (lambda tr(e)^2 + 2 mu e:e) / 2 (a = lambda, b = mu) */
+} gc_density_kind;
+
+typedef struct gc_density {
+  int32_t kind; /* gc_density_kind */
+  double a, b;
+} gc_density;
+
+/* func::evaluate(field, f, Pairwise|Kahan) for a recognised density f, fused: the derivatives
+ * each site needs, the density and the reduction in one pass (func/Functional.hpp:88-220).
+ * field is scalar, or Vec3 (3 values per site) for GC_DENSITY_LINEAR_ELASTIC.  Extent checks
+ * as the materialised operators: radius 1 for any gradient or Laplacian, radius 2 for the
+ * biharmonic. */
+gc_status gc_evaluate(gc_context* ctx, const gc_grid* g, gc_dtype dt, const gc_density* f,
+                      int method, const void* field, double* result);
 /* Cahn-Hilliard free energy func::evaluate(psi, f(p, grad p)) with
  * f = -p^2/2 + p^4/4 + kappa/2 |grad p|^2 (func/Functional.hpp:122-130,
  * examples/common/cahn_hilliard.hpp:64-68), fused: gradient + density + reduction. */

@@ -415,6 +415,72 @@ int or_gradient_energy(const or_grid* g, int method, const double* psi, double* 
   return OR_OK;
 }
 
+/* func/Functional.hpp:88-148 (scalar field; the Order-2 gradient, Laplacian and biharmonic
+ * the density needs are materialised first, in that order, as the reference does) and
+ * :183-220 (Vec3 field; its Order-2 rank-2 gradient), then the density site by site and
+ * or_integrate.  kind: 0 p; 1 |g|^2; 2 Cahn-Hilliard (a = kappa); 3 Ginzburg-Landau
+ * (a = W); 4 phase-field crystal (a = q0, b = eps); 5 linear elastic (Vec3 field,
+ * a = lambda, b = mu).  The formulas are those of examples/common/cahn_hilliard.hpp:64-76,
+ * test/functional_test.cpp:45-48, test/pfc_functional_test.cpp:44-50 and
+ * examples/common/elastic_energy.hpp:81-88, with |g|^2 = (gx*gx + gy*gy) + gz*gz and the
+ * strain sums over the 3x3 entries in column-major order. */
+int or_evaluate(const or_grid* g, int kind, double a, double b, int method, const double* f,
+                double* result) {
+  const size_t n = gsize(g);
+  if (kind < 0 || kind > 5) return OR_ARG;
+  if (kind != 0 && !extents_ok(g, 1)) return OR_EXTENT;
+  double* dens = (double*)malloc(n * sizeof(double));
+  double* d1 = (double*)malloc(9 * n * sizeof(double));
+  double* d2 = (double*)malloc(n * sizeof(double));
+  double* d4 = (double*)malloc(n * sizeof(double));
+  int rc = (!dens || !d1 || !d2 || !d4) ? OR_NOMEM : OR_OK;
+  if (!rc && kind == 5) rc = or_gradient_vec(g, 2, f, d1);
+  if (!rc && kind >= 1 && kind <= 4) rc = or_gradient(g, 2, f, d1);
+  if (!rc && (kind == 3 || kind == 4)) rc = or_laplacian(g, 2, f, d2);
+  if (!rc && kind == 4) rc = or_biharmonic(g, 2, f, d4);
+  if (!rc) {
+    for (size_t c = 0; c < n; ++c) {
+      if (kind == 5) {
+        const double* G = d1 + 9 * c; /* G(r, col) at r + 3 col */
+        double e[9];
+        for (int col = 0; col < 3; ++col)
+          for (int r = 0; r < 3; ++r) e[r + 3 * col] = 0.5 * (G[r + 3 * col] + G[col + 3 * r]);
+        const double tr = e[0] + e[4] + e[8];
+        double ee = e[0] * e[0];
+        for (int q = 1; q < 9; ++q) ee += e[q] * e[q];
+        dens[c] = 0.5 * (a * tr * tr + 2.0 * b * ee);
+        continue;
+      }
+      const double p = f[c];
+      if (kind == 0) {
+        dens[c] = p;
+        continue;
+      }
+      const double gx = d1[3 * c], gy = d1[3 * c + 1], gz = d1[3 * c + 2];
+      const double sq = gx * gx + gy * gy + gz * gz;
+      if (kind == 1) {
+        dens[c] = sq;
+      } else if (kind == 2) {
+        dens[c] = -0.5 * p * p + 0.25 * p * p * p * p + 0.5 * a * sq;
+      } else if (kind == 3) {
+        const double q = p * p - 1.0;
+        dens[c] = 0.5 * sq + a * q * q;
+      } else {
+        const double q2 = a * a;
+        const double q4 = q2 * q2;
+        const double p2 = p * p;
+        dens[c] = 0.5 * (q4 - b) * p2 + q2 * p * d2[c] + 0.5 * p * d4[c] + 0.25 * p2 * p2;
+      }
+    }
+    *result = or_integrate(g, method, dens);
+  }
+  free(dens);
+  free(d1);
+  free(d2);
+  free(d4);
+  return rc;
+}
+
 /* ---------------- right-hand sides ---------------- */
 
 typedef int (*or_rhs_fn)(const void* ctx, const or_grid* g, const double* in, double* out);

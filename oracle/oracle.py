@@ -98,6 +98,8 @@ def lib():
         L.or_diffuse_field.restype = C.c_int
         L.or_stencil_table.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), dp]
         L.or_stencil_table.restype = C.c_int
+        L.or_evaluate.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp]
+        L.or_evaluate.restype = C.c_int
         L.or_build_info.argtypes = []
         L.or_build_info.restype = C.c_char_p
         _lib = L
@@ -227,6 +229,19 @@ def gradient_energy(g, psi, method=PAIRWISE):
     E = C.c_double()
     _check(lib().or_gradient_energy(C.byref(g), method, _dp(psi), C.byref(E)))
     return E.value
+
+
+DENSITY = dict(value=0, gradient_energy=1, cahn_hilliard=2, ginzburg_landau=3, pfc=4,
+               linear_elastic=5)
+
+
+def evaluate(g, kind, field, a=0.0, b=0.0, method=PAIRWISE):
+    """func::evaluate of density `kind` (a name of DENSITY) on a scalar field, or a Vec3
+    field (..., 3) for linear_elastic."""
+    field = _f64(field)
+    r = C.c_double()
+    _check(lib().or_evaluate(C.byref(g), DENSITY[kind], a, b, method, _dp(field), C.byref(r)))
+    return r.value
 
 
 class _CHParams(C.Structure):

@@ -500,3 +500,82 @@ def test_ch_energy_rise_from_white_noise(orc, gc):
         Fs.append(orc.free_energy(g, c, kappa))
     assert all(b < a for a, b in zip(Es, Es[1:])), Es
     assert Fs[-1] > Fs[0], Fs
+
+
+# ---- func::evaluate densities: exact discrete closed forms ---------------------------------
+
+def _sss(N):
+    h = 2 * np.pi / N
+    x = np.arange(N) * h
+    s, c = np.sin(x), np.cos(x)
+    return h, x, s, c
+
+
+@pytest.mark.parametrize("N", [16, 12])
+def test_evaluate_scalar_densities_closed_form(orc, N):
+    """psi = sin x sin y sin z on [0, 2pi)^3.  The central difference gives
+    g = (sin h / h) (cos x sin y sin z, ...), lap psi = L psi, biharm psi = B psi with the O2
+    symbols L = 3 s2, B = 3 s4 + 6 s2^2 (s2 = (2 cos h - 2)/h^2, s4 = (2 cos 2h - 8 cos h + 6)/h^4),
+    and whole-period grid sums give sum psi^2 = (N/2)^3, sum psi^4 = (3N/8)^3,
+    sum |g|^2 = 3 (sin h/h)^2 (N/2)^3; every density's integral follows exactly."""
+    h, x, s, c = _sss(N)
+    g = orc.Grid.make(N, N, N, h)
+    psi = s[None, None, :] * s[None, :, None] * s[:, None, None]
+    V = h ** 3
+    S2, S4 = (N / 2) ** 3, (3 * N / 8) ** 3
+    G2 = 3 * (np.sin(h) / h) ** 2 * S2
+    s2 = (2 * np.cos(h) - 2) / h ** 2
+    s4 = (2 * np.cos(2 * h) - 8 * np.cos(h) + 6) / h ** 4
+    L, B = 3 * s2, 3 * s4 + 6 * s2 * s2
+    kappa, W, q0, eps = 0.7, 1.3, 1.1, 0.1
+    want = {
+        ("value", 0.0, 0.0): 0.25 * N ** 3 * V,  # on psi + 0.25
+        ("gradient_energy", 0.0, 0.0): G2 * V,
+        ("cahn_hilliard", kappa, 0.0): (-0.5 * S2 + 0.25 * S4 + 0.5 * kappa * G2) * V,
+        ("ginzburg_landau", W, 0.0): (0.5 * G2 + W * (S4 - 2 * S2 + N ** 3)) * V,
+        ("pfc", q0, eps): (0.5 * (q0 ** 4 - eps) * S2 + q0 ** 2 * L * S2 + 0.5 * B * S2
+                           + 0.25 * S4) * V,
+    }
+    for (kind, a, b), w in want.items():
+        f = psi + 0.25 if kind == "value" else psi
+        for method in (orc.PAIRWISE, orc.KAHAN):
+            got = orc.evaluate(g, kind, f, a, b, method)
+            assert abs(got - w) <= 1e-12 * max(abs(w), G2 * V), (kind, got, w)
+    # a plausible slip (PFC without the biharmonic's mixed terms) is far outside that bound
+    wrong = (0.5 * (q0 ** 4 - eps) * S2 + q0 ** 2 * L * S2 + 0.5 * 3 * s4 * S2 + 0.25 * S4) * V
+    assert abs(orc.evaluate(g, "pfc", psi, q0, eps) - wrong) > 1e-3 * abs(wrong)
+
+
+def test_evaluate_linear_elastic_closed_form(orc):
+    """u = (sin x + sin y, sin z, sin x): G = grad u has G00 = s cos x, G01 = s cos y,
+    G12 = s cos z, G20 = s cos x (s = sin h / h); e = sym(G) gives tr e = s cos x and
+    e:e = s^2 (1.5 cos^2 x + 0.5 cos^2 y + 0.5 cos^2 z), so the integral of
+    (lambda tr^2 + 2 mu e:e)/2 is h^3 (lambda s^2 N^3/2 + 2 mu 1.25 s^2 N^3)/2 exactly.  Using
+    G instead of its symmetric part would give 2 mu (1.5 + 1 + 1) s^2 N^3 / 2 instead."""
+    N = 16
+    h, x, sx, cx = _sss(N)
+    g = orc.Grid.make(N, N, N, h)
+    X = np.broadcast_to(x[None, None, :], (N, N, N))
+    Y = np.broadcast_to(x[None, :, None], (N, N, N))
+    Z = np.broadcast_to(x[:, None, None], (N, N, N))
+    u = np.stack([np.sin(X) + np.sin(Y), np.sin(Z), np.sin(X)], axis=-1)
+    lam, mu = 1.7, 0.6
+    s = np.sin(h) / h
+    want = h ** 3 * 0.5 * (lam * s * s * N ** 3 / 2 + 2 * mu * 1.25 * s * s * N ** 3)
+    got = orc.evaluate(g, "linear_elastic", u, lam, mu)
+    assert abs(got - want) <= 1e-12 * abs(want)
+
+
+def test_evaluate_reference_hand_computed(orc):
+    """The reference tests' continuum answers at N = 64 (functional_test.cpp:27-56,
+    pfc_functional_test.cpp:52-101) within their own tolerances."""
+    N = 64
+    h, x, s, c = _sss(N)
+    g = orc.Grid.make(N, N, N, h)
+    psi = s[None, None, :] * s[None, :, None] * s[:, None, None]
+    pi3 = np.pi ** 3
+    gl = orc.evaluate(g, "ginzburg_landau", psi, 1.0)
+    assert abs(gl - 507 / 64 * pi3) < 1e-2 * 507 / 64 * pi3
+    pfc = orc.evaluate(g, "pfc", psi, 1.0, 0.1)
+    assert abs(pfc - 2.05546875 * pi3) < 2e-2 * 2.05546875 * pi3
+    assert abs(orc.evaluate(g, "gradient_energy", psi) / 2 - 1.5 * pi3) < 1e-2 * 1.5 * pi3
