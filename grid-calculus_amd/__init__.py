@@ -79,6 +79,16 @@ class Density(C.Structure):
     _fields_ = [("kind", C.c_int32), ("a", C.c_double), ("b", C.c_double)]
 
 
+class ExprInsn(C.Structure):
+    """gc_expr_insn: one instruction of a per-site tensor-expression program."""
+    _fields_ = [("op", C.c_int32), ("dst", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("nc", C.c_int32), ("c", C.c_double), ("ptr", C.c_void_p)]
+
+
+# gc_expr_op
+(EXPR_LOAD, EXPR_IDENTITY, EXPR_SCALE, EXPR_ADD, EXPR_SUB, EXPR_MUL, EXPR_TRACE, EXPR_SYM,
+ EXPR_MATMUL, EXPR_DDOT) = range(10)
+
 # gc_density_kind
 DENSITY = dict(value=0, gradient_energy=1, cahn_hilliard=2, ginzburg_landau=3, pfc=4,
                linear_elastic=5)
@@ -128,12 +138,14 @@ def lib():
         "gc_divergence": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
         "gc_divergence_mat": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
         "gc_biharmonic": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
+        "gc_mixed_partial": ([vp, gp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp], st),
         "gc_cell_laplacian": ([vp, gp, C.c_int, vp, vp, vp, i64p], st),
         "gc_first_nonpositive": ([vp, gp, C.c_int, vp, i64p], st),
         "gc_max_face_d": ([vp, gp, C.c_int, vp, dp], st),
         "gc_integrate": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_integrate_noalloc": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
         "gc_evaluate": ([vp, gp, C.c_int, C.POINTER(Density), C.c_int, vp, dp], st),
+        "gc_expr_eval": ([vp, gp, C.POINTER(ExprInsn), C.c_int, C.c_int, C.c_int, vp], st),
         "gc_ch_free_energy": ([vp, gp, C.c_int, C.c_double, C.c_int, vp, dp], st),
         "gc_gradient_energy": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_axpy": ([vp, C.c_int, sz, vp, C.c_double, vp, vp], st),
@@ -357,6 +369,14 @@ def biharmonic(ctx, g, f, order=2, out=None):
     return out
 
 
+def mixed_partial(ctx, g, f, nx, ny, nz, order=2, out=None):
+    """d^(nx+ny+nz) f / dx^nx dy^ny dz^nz (1 <= nx+ny+nz <= 4)."""
+    out = out or DeviceArray(g.shape, f.dtype)
+    _check(lib().gc_mixed_partial(ctx.handle, C.byref(g), _dt(f), nx, ny, nz, order, f.ptr,
+                                  out.ptr))
+    return out
+
+
 def cell_laplacian(ctx, g, psi, D, out=None):
     out = out or DeviceArray(g.shape, psi.dtype)
     bad = C.c_int64(-1)
@@ -405,6 +425,15 @@ def evaluate(ctx, g, kind, field, a=0.0, b=0.0, method=PAIRWISE):
     _check(lib().gc_evaluate(ctx.handle, C.byref(g), _dt(field), C.byref(D), method, field.ptr,
                              C.byref(r)))
     return r.value
+
+
+def expr_eval(ctx, g, prog, out_slot, out_nc):
+    """Runs a list of ExprInsn at every site; returns the FP64 result field of out_nc values
+    per site."""
+    out = DeviceArray(g.shape + ((out_nc,) if out_nc > 1 else ()), np.float64)
+    arr = (ExprInsn * len(prog))(*prog)
+    _check(lib().gc_expr_eval(ctx.handle, C.byref(g), arr, len(prog), out_slot, out_nc, out.ptr))
+    return out
 
 
 def gradient_energy(ctx, g, psi, method=PAIRWISE):

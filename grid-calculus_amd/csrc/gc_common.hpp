@@ -59,6 +59,8 @@ gc_status check_ctx(gc_context* ctx);
 gc_status check_grid(const gc_grid* g);
 // requireAxisExtent for every axis (diff/detail/PreconditionAxisExtent.hpp:64-79).
 gc_status check_extent(const gc_grid* g, int radius);
+// requireAxisExtent with a radius per axis (x, y, z checked in that order).
+gc_status check_extent3(const gc_grid* g, int rx, int ry, int rz);
 gc_status check_dtype(gc_dtype dt);
 
 inline size_t sites(const gc_grid* g) {
@@ -124,6 +126,22 @@ template <> struct D4<4> {
            : (m == 1 || m == 5) ? 2.0
            : (m == 2 || m == 4) ? -13.0 / 2.0
                                 : 28.0 / 3.0;
+  }
+};
+
+// stencil/ThirdDerivative.hpp: offsets -R..R
+template <int O> struct D3;
+template <> struct D3<2> {
+  static constexpr int R = 2, N = 5;
+  __host__ __device__ static constexpr double w(int m) {
+    return m == 0 ? -0.5 : m == 1 ? 1.0 : m == 2 ? 0.0 : m == 3 ? -1.0 : 0.5;
+  }
+};
+template <> struct D3<4> {
+  static constexpr int R = 3, N = 7;
+  __host__ __device__ static constexpr double w(int m) {
+    return m == 0 ? 1.0 / 8.0 : m == 1 ? -1.0 : m == 2 ? 13.0 / 8.0 : m == 3 ? 0.0
+           : m == 4 ? -13.0 / 8.0 : m == 5 ? 1.0 : -1.0 / 8.0;
   }
 };
 

@@ -60,6 +60,13 @@ gc_status check_extent(const gc_grid* g, int radius) {
   return GC_OK;
 }
 
+gc_status check_extent3(const gc_grid* g, int rx, int ry, int rz) {
+  GC_TRY(check_axis("x", g->nx, rx));
+  GC_TRY(check_axis("y", g->ny, ry));
+  GC_TRY(check_axis("z", g->nz, rz));
+  return GC_OK;
+}
+
 gc_status check_dtype(gc_dtype dt) {
   if (dt != GC_F64 && dt != GC_F32) return set_error(GC_ERR_INVALID, "gridcalc: bad dtype %d", (int)dt);
   return GC_OK;

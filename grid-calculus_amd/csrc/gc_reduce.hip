@@ -46,9 +46,13 @@ struct SiteIdx {
   int i, j, k;
 };
 
+// Grids hold at most 2^31-1 sites, so site indices fit 32-bit unsigned arithmetic (a 64-bit
+// division costs several times a 32-bit one).
 __device__ __forceinline__ SiteIdx site_of(const Dims& d, size_t c) {
-  const size_t r = c / (size_t)d.nx;
-  return SiteIdx{(int)(c - r * (size_t)d.nx), (int)(r % (size_t)d.ny), (int)(r / (size_t)d.ny)};
+  const unsigned cc = (unsigned)c;
+  const unsigned r = cc / (unsigned)d.nx;
+  const unsigned k = r / (unsigned)d.ny;
+  return SiteIdx{(int)(cc - r * (unsigned)d.nx), (int)(r - k * (unsigned)d.ny), (int)k};
 }
 
 template <class T>
@@ -56,8 +60,9 @@ struct Sampler {  // wrapped read of component `comp` of an NC-component AoS fie
   const T* p;
   Dims d;
   __device__ __forceinline__ double operator()(int i, int j, int k, int NC = 1, int comp = 0) const {
-    const size_t s = (size_t)wrap(k, d.nz) * d.plane + (size_t)wrap(j, d.ny) * d.nx + wrap(i, d.nx);
-    return (double)p[s * NC + comp];
+    const unsigned s = (unsigned)wrap(k, d.nz) * (unsigned)d.plane +
+                       (unsigned)wrap(j, d.ny) * (unsigned)d.nx + (unsigned)wrap(i, d.nx);
+    return (double)p[(size_t)s * NC + comp];
   }
 };
 

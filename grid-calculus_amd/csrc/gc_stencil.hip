@@ -13,130 +13,13 @@
 // stage only the current plane (x or y halo) in LDS.  The operators move 2-12 words per site
 // and are bounded by HBM.
 #include "gc_common.hpp"
+#include "gc_zmarch.hpp"
 
 #include <cstring>
 #include <type_traits>
 
 namespace gc {
 namespace {
-
-// ---- z-marching LDS engine for the scalar-input operators --------------------------------
-//
-// A block owns an OX x OY tile of xy-columns and a chunk of consecutive planes.  It keeps the
-// 2R+1 planes z-R..z+R of every input field (tile + R-site halo, periodic wrap applied while
-// filling) in an LDS ring, so each input value is read from memory once per block plus the
-// halo; the next plane is fetched into registers while the current output plane is formed.
-// The operators read neighbours through an accessor and keep the reference's per-axis
-// accumulation order, so results equal the one-site-at-a-time evaluation bit for bit.
-
-template <int R, int NF, int OX, int OY>
-struct ZTile {
-  static constexpr int NT = 256;
-  static constexpr int PW = OX + 2 * R, PH = OY + 2 * R, PN = PW * PH, NP = 2 * R + 1;
-  static constexpr int PL = (PN + NT - 1) / NT;  // plane slots filled per thread
-  static constexpr int ROWS = NT / OX;
-  static constexpr int OR = OX * OY / NT;        // output sites per thread per plane
-  static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
-};
-
-// Neighbour access into the ring: F(field, dx, dy, dz) at the current output site.
-template <class T, int R, int PN, int NP, int PW>
-struct RingAt {
-  const T* ring;     // [NF][NP][PN]
-  const int* slot;   // slot of plane z + dz at slot[dz + R]
-  int base;          // (oy + R) * PW + (ox + R)
-  __device__ __forceinline__ T operator()(int f, int dx, int dy, int dz) const {
-    return ring[(f * NP + slot[dz + R]) * PN + base + dy * PW + dx];
-  }
-};
-
-// CHECK: every value of field 1 the block stages is tested (!(v > 0)) and the smallest
-// failing flat index goes to *bad by atomicMin (the fvm::cellLaplacian D > 0 precondition,
-// checked in the same pass).
-template <class T, int OX, int OY, class Op, bool CHECK = false>
-__global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
-                                                const T* __restrict__ f1, T* __restrict__ out,
-                                                Dims d, int ntx, int nty, int zlen,
-                                                unsigned long long* __restrict__ bad) {
-  constexpr int R = Op::R, NF = Op::NF;
-  using Z = ZTile<R, NF, OX, OY>;
-  __shared__ T ring[NF * Z::NP * Z::PN];
-  const int t = threadIdx.x;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int tx = (int)(bid % (unsigned)ntx);
-  const unsigned rr = bid / (unsigned)ntx;
-  const int ty = (int)(rr % (unsigned)nty);
-  const int zc = (int)(rr / (unsigned)nty);
-  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
-  const int nout = min(zlen, d.nz - z0);
-
-  unsigned poff[Z::PL];
-  bool pin[Z::PL];
-#pragma unroll
-  for (int s = 0; s < Z::PL; ++s) {
-    const int idx = t + s * Z::NT;
-    pin[s] = idx < Z::PN;
-    const int px = idx % Z::PW, py = idx / Z::PW;
-    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
-                     : 0u;
-  }
-  T reg[NF][Z::PL];
-  unsigned long long first = ~0ull;
-  auto load = [&](int plane) {
-    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
-#pragma unroll
-    for (int s = 0; s < Z::PL; ++s) {
-      if (pin[s]) {
-        reg[0][s] = f0[zo + poff[s]];
-        if constexpr (NF > 1) {
-          reg[NF - 1][s] = f1[zo + poff[s]];
-          if constexpr (CHECK)
-            if (!(reg[NF - 1][s] > T(0))) first = min(first, (unsigned long long)(zo + poff[s]));
-        }
-      }
-    }
-  };
-  auto store = [&](int slot) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int s = 0; s < Z::PL; ++s)
-        if (pin[s]) ring[(f * Z::NP + slot) * Z::PN + t + s * Z::NT] = reg[f][s];
-  };
-
-  // plane z0 - R + i lives in slot i % NP
-  for (int i = 0; i < 2 * R; ++i) {
-    load(z0 - R + i);
-    store(i);
-  }
-  load(z0 + R);
-  const int ox = t % OX, oy0 = t / OX;
-  for (int k = 0; k < nout; ++k) {
-    store((k + 2 * R) % Z::NP);
-    __syncthreads();
-    if (k + 1 < nout) load(z0 + k + 1 + R);
-    int slot[Z::NP];
-#pragma unroll
-    for (int j = 0; j < Z::NP; ++j) slot[j] = (k + j) % Z::NP;
-    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
-#pragma unroll
-    for (int q = 0; q < Z::OR; ++q) {
-      const int oy = oy0 + q * Z::ROWS;
-      if (x0 + ox < d.nx && y0 + oy < d.ny) {
-        const RingAt<T, R, Z::PN, Z::NP, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
-        op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
-      }
-    }
-    __syncthreads();
-  }
-  if constexpr (CHECK) {
-    for (int st = 32; st >= 1; st >>= 1) {
-      const unsigned long long o = __shfl_xor(first, st);
-      first = o < first ? o : first;
-    }
-    if ((t & 63) == 0 && first != ~0ull) atomicMin(bad, first);
-  }
-}
 
 // diff/Laplacian.hpp:56-71
 template <class T, int O>
@@ -250,7 +133,6 @@ struct OpCell {
   }
 };
 
-constexpr int kZOX = 64, kZOY = 8, kZLen = 32;
 
 // fvm::cellLaplacian (fvm/CellLaplacian.hpp:115-141) on the z-march with shared face means and the D > 0 check fused in.
 // The harmonic mean of a face depends on D only, so each plane's x and y faces are formed
@@ -371,14 +253,6 @@ __global__ __launch_bounds__(256) void k_cell_zf(T ihx2, T ihy2, T ihz2,
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
-template <class T, class Op, bool CHECK = false>
-void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, const T* f1,
-                   T* out, unsigned long long* bad = nullptr) {
-  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + kZOY - 1) / kZOY;
-  const int nzc = (d.nz + kZLen - 1) / kZLen;
-  hipLaunchKernelGGL((k_zmarch<T, kZOX, kZOY, Op, CHECK>), dim3((unsigned)(ntx * nty * nzc)),
-                     dim3(256), 0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen, bad);
-}
 
 // ---- axis stencils: z in register queues (used for the order-4 Laplacian) --------------
 //

@@ -115,6 +115,14 @@ gc_status gc_divergence_mat(gc_context* ctx, const gc_grid* g, gc_dtype dt, int 
 /* diff::biharmonic<Order> / diff::d4 (diff/Biharmonic.hpp:48-119). */
 gc_status gc_biharmonic(gc_context* ctx, const gc_grid* g, gc_dtype dt, int order,
                         const void* in, void* out);
+/* diff::detail::mixedDerivative<Nx, Ny, Nz, Order> (diff/detail/MultiIndexDerivative.hpp:
+ * 96-136): d^(nx+ny+nz) psi / dx^nx dy^ny dz^nz with the tensor product of one axis stencil
+ * per axis (first to fourth central difference, Order 2 or 4), 1 <= nx+ny+nz <= 4; behind the
+ * named partials of diff/MixedPartial.hpp (dxx .. dyz), diff/ThirdOrder.hpp (d3dx3 ..
+ * d3dxdydz) and diff/FourthOrder.hpp (d4dx4 .. d4dxdydz2).  Extent check per axis with that
+ * axis's stencil radius. */
+gc_status gc_mixed_partial(gc_context* ctx, const gc_grid* g, gc_dtype dt, int nx, int ny, int nz,
+                           int order, const void* in, void* out);
 /* fvm::cellLaplacian(psi, D) (fvm/CellLaplacian.hpp:91-143).  Checks D > 0
  * (CellLaplacian.hpp:99-109) inside the operator's kernel: on a violation returns
  * GC_ERR_NONPOS_D (out then holds no meaningful values) and, when first_bad is not NULL,
@@ -149,6 +157,38 @@ gc_status gc_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, int metho
  * allocates). */
 gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
                                const void* in, double* result_dev);
+/* ---- tensor contraction expressions ------------------------------------------ */
+
+/* One instruction of a per-site register program (tensor::expr of tensor/Expressions.hpp,
+ * lowered by gridcalc/tensor/Expressions.hpp).  Slots 0..7 hold nc = 1 (scalar), 3 (Vec3) or
+ * 9 (Mat3, column-major) doubles.  dst = op(a, b); LOAD reads nc doubles per site from the FP64
+ * field at ptr; SCALE multiplies by c. */
+typedef enum gc_expr_op {
+  GC_EXPR_LOAD = 0,     /* dst = field(ptr)                              */
+  GC_EXPR_IDENTITY = 1, /* dst = Mat3 identity                           */
+  GC_EXPR_SCALE = 2,    /* dst = c * a                                   */
+  GC_EXPR_ADD = 3,      /* dst = a + b                                   */
+  GC_EXPR_SUB = 4,      /* dst = a - b                                   */
+  GC_EXPR_MUL = 5,      /* dst = a * b (scalars)                         */
+  GC_EXPR_TRACE = 6,    /* dst = (a00 + a11) + a22                       */
+  GC_EXPR_SYM = 7,      /* dst = 0.5 * (a + a^T)                         */
+  GC_EXPR_MATMUL = 8,   /* dst = a * b (3x3 matrix product)              */
+  GC_EXPR_DDOT = 9      /* dst = sum of a_i * b_i over the 9 entries     */
+} gc_expr_op;
+
+typedef struct gc_expr_insn {
+  int32_t op, dst, a, b, nc;
+  double c;
+  const void* ptr;
+} gc_expr_insn;
+
+/* Runs a program of n_insn <= 32 instructions at every site and writes slot out_slot
+ * (out_nc doubles per site) to the FP64 field out (tensor::expr::materialize,
+ * tensor/Expressions.hpp:628-651; the eager tensor::trace / singleContract / doubleContract
+ * of tensor/Contraction.hpp:33-101 are one-node programs). */
+gc_status gc_expr_eval(gc_context* ctx, const gc_grid* g, const gc_expr_insn* prog, int n_insn,
+                       int out_slot, int out_nc, void* out);
+
 /* Densities func::evaluate integrates on the device (func/Functional.hpp:88-148, the scalar
  * overload, and :183-220, the Vec3 overload), each a formula of the Order-2 derivatives the
  * reference materialises.  The C++ structs in gridcalc/func/Densities.hpp evaluate the same

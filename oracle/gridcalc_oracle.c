@@ -83,17 +83,25 @@ static int stencil_d1(int order, or_stencil* s) {
   if (order == 4) { s->n = 5; s->radius = 2; s->off = OFF5; s->w = D1_O4; return 0; }
   return -1;
 }
+/* stencil/ThirdDerivative.hpp */
+static const double D3_O2[5] = {-1.0 / 2.0, 1.0, 0.0, -1.0, 1.0 / 2.0};
+static const double D3_O4[7] = {1.0 / 8.0, -1.0, 13.0 / 8.0, 0.0, -13.0 / 8.0, 1.0, -1.0 / 8.0};
+static int stencil_d3(int order, or_stencil* s) {
+  if (order == 2) { s->n = 5; s->radius = 2; s->off = OFF5; s->w = D3_O2; return 0; }
+  if (order == 4) { s->n = 7; s->radius = 3; s->off = OFF7; s->w = D3_O4; return 0; }
+  return -1;
+}
 static int stencil_d4(int order, or_stencil* s) {
   if (order == 2) { s->n = 5; s->radius = 2; s->off = OFF5; s->w = D4_O2; return 0; }
   if (order == 4) { s->n = 7; s->radius = 3; s->off = OFF7; s->w = D4_O4; return 0; }
   return -1;
 }
 
-/* Weight-table accessor for tests: kind 1/2/4 = first/second/fourth derivative. */
+/* Weight-table accessor for tests: kind 1/2/3/4 = first/second/third/fourth derivative. */
 int or_stencil_table(int kind, int order, int* n, int* radius, int* offsets, double* weights) {
   or_stencil s;
   int rc = kind == 1 ? stencil_d1(order, &s) : kind == 2 ? stencil_d2(order, &s)
-         : kind == 4 ? stencil_d4(order, &s) : -1;
+         : kind == 3 ? stencil_d3(order, &s) : kind == 4 ? stencil_d4(order, &s) : -1;
   if (rc) return OR_ARG;
   *n = s.n; *radius = s.radius;
   for (int m = 0; m < s.n; ++m) { offsets[m] = s.off[m]; weights[m] = s.w[m]; }
@@ -280,6 +288,53 @@ int or_biharmonic(const or_grid* g, int order, const double* f, double* out) {
 
 /* fvm/CellLaplacian.hpp:48-50 */
 static inline double hmean(double a, double b) { return 2.0 * a * b / (a + b); }
+
+/* diff/detail/MultiIndexDerivative.hpp:46-84: the axis stencil of an n-th derivative
+ * (order 0 is the identity) */
+static const int OFF1[1] = {0};
+static const double D0_W[1] = {1.0};
+static int axis_stencil(int n, int order, or_stencil* s) {
+  if (n == 0) { s->n = 1; s->radius = 0; s->off = OFF1; s->w = D0_W; return 0; }
+  if (n == 1) return stencil_d1(order, s);
+  if (n == 2) return stencil_d2(order, s);
+  if (n == 3) return stencil_d3(order, s);
+  if (n == 4) return stencil_d4(order, s);
+  return -1;
+}
+
+/* diff/detail/MultiIndexDerivative.hpp:96-136 */
+int or_mixed_partial(const or_grid* g, int nx, int ny, int nz, int order, const double* f,
+                     double* out) {
+  or_stencil sx, sy, sz;
+  if (nx < 0 || ny < 0 || nz < 0 || nx + ny + nz < 1 || nx + ny + nz > 4) return OR_ARG;
+  if (axis_stencil(nx, order, &sx) || axis_stencil(ny, order, &sy) ||
+      axis_stencil(nz, order, &sz))
+    return OR_ARG;
+  if (!axis_ok(g->nx, sx.radius) || !axis_ok(g->ny, sy.radius) || !axis_ok(g->nz, sz.radius))
+    return OR_EXTENT;
+  double scale = 1.0;
+  for (int n = 0; n < nx; ++n) scale /= g->hx;
+  for (int n = 0; n < ny; ++n) scale /= g->hy;
+  for (int n = 0; n < nz; ++n) scale /= g->hz;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        double sum = 0.0;
+        for (int mz = 0; mz < sz.n; ++mz) {
+          const int oz = sz.off[mz];
+          const double wz = sz.w[mz];
+          for (int my = 0; my < sy.n; ++my) {
+            const int oy = sy.off[my];
+            const double wy = sy.w[my];
+            const double wyz = wy * wz;
+            for (int mx = 0; mx < sx.n; ++mx)
+              sum += sx.w[mx] * wyz * f[at(g, i + sx.off[mx], j + oy, k + oz)];
+          }
+        }
+        out[at(g, i, j, k)] = sum * scale;
+      }
+  return OR_OK;
+}
 
 /* fvm/CellLaplacian.hpp:91-143.  On a non-positive D returns OR_NONPOS_D and
  * stores the first violating flat index in *bad_index (CellLaplacian.hpp:99-109). */

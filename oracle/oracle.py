@@ -98,6 +98,8 @@ def lib():
         L.or_diffuse_field.restype = C.c_int
         L.or_stencil_table.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), dp]
         L.or_stencil_table.restype = C.c_int
+        L.or_mixed_partial.argtypes = [gp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.or_mixed_partial.restype = C.c_int
         L.or_evaluate.argtypes = [gp, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp]
         L.or_evaluate.restype = C.c_int
         L.or_build_info.argtypes = []
@@ -184,6 +186,15 @@ def divergence_mat(g, m, order=2):
 
 def biharmonic(g, f, order=2):
     return _unary("or_biharmonic", g, order, f, 1, 1)
+
+
+def mixed_partial(g, f, nx, ny, nz, order=2):
+    f = _f64(f)
+    if f.size != g.size:
+        raise ValueError("input size does not match grid")
+    out = np.empty(g.shape, dtype=np.float64)
+    _check(lib().or_mixed_partial(C.byref(g), nx, ny, nz, order, _dp(f), _dp(out)))
+    return out
 
 
 def cell_laplacian(g, psi, D):
@@ -282,3 +293,43 @@ def diffuse_field(g, psi, D, dt, n_steps, method=EULER):
         raise OracleError(rc, f"first violation at flat index {bad.value}")
     _check(rc)
     return out
+
+
+# ---- tensor contractions (tensor/Expressions.hpp, tensor/Contraction.hpp) -------------------
+# Per-site restatements on (..., 9) column-major Mat3 arrays, M(r, c) at r + 3c, in the
+# operation order the reference's evalAt nodes use (left-to-right sums).
+
+def mat_trace(m):
+    """Trace::evalAt (Expressions.hpp:383-387): (m00 + m11) + m22."""
+    return (m[..., 0] + m[..., 4]) + m[..., 8]
+
+
+def mat_sym(m):
+    """Sym::evalAt (Expressions.hpp:421-425): 0.5 * (m + m^T)."""
+    out = np.empty_like(m)
+    for c in range(3):
+        for r in range(3):
+            out[..., r + 3 * c] = 0.5 * (m[..., r + 3 * c] + m[..., c + 3 * r])
+    return out
+
+
+def mat_mul(a, b):
+    """SingleContract::evalAt (Expressions.hpp:479-483): (a b)(r, c) = sum_k a(r, k) b(k, c),
+    k = 0, 1, 2 left to right."""
+    out = np.empty_like(a)
+    for c in range(3):
+        for r in range(3):
+            s = a[..., r] * b[..., 3 * c]
+            s = s + a[..., r + 3] * b[..., 1 + 3 * c]
+            s = s + a[..., r + 6] * b[..., 2 + 3 * c]
+            out[..., r + 3 * c] = s
+    return out
+
+
+def mat_ddot(a, b):
+    """DoubleContract::evalAt (Expressions.hpp:519-525): sum of a_i b_i over the 9 entries in
+    storage order."""
+    s = a[..., 0] * b[..., 0]
+    for q in range(1, 9):
+        s = s + a[..., q] * b[..., q]
+    return s

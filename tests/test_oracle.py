@@ -15,6 +15,8 @@ EXACT = {
     (2, 4): [Fr(-1, 12), Fr(4, 3), Fr(-5, 2), Fr(4, 3), Fr(-1, 12)],
     (1, 2): [Fr(-1, 2), Fr(0), Fr(1, 2)],
     (1, 4): [Fr(1, 12), Fr(-2, 3), Fr(0), Fr(2, 3), Fr(-1, 12)],
+    (3, 2): [Fr(-1, 2), Fr(1), Fr(0), Fr(-1), Fr(1, 2)],
+    (3, 4): [Fr(1, 8), Fr(-1), Fr(13, 8), Fr(0), Fr(-13, 8), Fr(1), Fr(-1, 8)],
     (4, 2): [Fr(1), Fr(-4), Fr(6), Fr(-4), Fr(1)],
     (4, 4): [Fr(-1, 6), Fr(2), Fr(-13, 2), Fr(28, 3), Fr(-13, 2), Fr(2), Fr(-1, 6)],
 }
@@ -28,7 +30,7 @@ def test_stencil_tables_exact(orc, kind, order):
     assert w == [float(x) for x in ex]  # the reference's EXPECT_DOUBLE_EQ audits
     # moment conditions of a derivative of order `kind` with accuracy `order`:
     # sum w m^p = kind! for p = kind, 0 for other p < kind + order
-    fact = {1: 1, 2: 2, 4: 24}[kind]
+    fact = {1: 1, 2: 2, 3: 6, 4: 24}[kind]
     for p in range(kind + order):
         mom = sum(wi * Fr(m) ** p for wi, m in zip(ex, offs))
         assert mom == (fact if p == kind else 0), (p, mom)
@@ -579,3 +581,79 @@ def test_evaluate_reference_hand_computed(orc):
     pfc = orc.evaluate(g, "pfc", psi, 1.0, 0.1)
     assert abs(pfc - 2.05546875 * pi3) < 2e-2 * 2.05546875 * pi3
     assert abs(orc.evaluate(g, "gradient_energy", psi) / 2 - 1.5 * pi3) < 1e-2 * 1.5 * pi3
+
+
+# ---- mixed partial derivatives ---------------------------------------------------------------
+
+# every multi-index with 1 <= nx + ny + nz <= 4 (the 31 named partials plus the three first
+# derivatives)
+MULTI = [(a, b, c) for a in range(5) for b in range(5) for c in range(5) if 1 <= a + b + c <= 4]
+
+
+def _axis_symbol(n, order, t):
+    """sum_m w_m e^{i m t} of the n-th derivative stencil (identity for n = 0)."""
+    if n == 0:
+        return 1.0 + 0j
+    w = EXACT[(n, order)]
+    r = (len(w) - 1) // 2
+    return sum(float(wm) * np.exp(1j * (m - r) * t) for m, wm in enumerate(w))
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("multi", MULTI, ids=lambda m: "".join(map(str, m)))
+def test_mixed_partial_symbol(orc, order, multi):
+    """On sin(th), th = k.x + phase, the tensor-product stencil acts as the product of the
+    per-axis complex symbols S_a = sum_m w_m e^{i m k_a h_a}: the result is
+    Im(prod_a S_a / h_a^n_a * e^{i th}) exactly (to rounding)."""
+    (nx, ny, nz), h, k = (14, 11, 9), (0.3, 0.2, 0.5), (2, 1, 3)
+    g = orc.Grid.make(nx, ny, nz, h)
+    arg, th = _mode(g.shape, h, k)
+    S = 1.0 + 0j
+    for a in range(3):
+        S *= _axis_symbol(multi[a], order, th[a]) / h[a] ** multi[a]
+    want = np.imag(S * np.exp(1j * arg))
+    got = orc.mixed_partial(g, np.sin(arg), *multi, order)
+    assert np.abs(got - want).max() < 1e-10 * max(1.0, abs(S)), (multi, order)
+
+
+def test_mixed_partial_known_values_and_extents(orc):
+    """Closed forms of the named partials the reference tests sweep: on
+    psi = sin x sin y sin z, d2/dxdy at O2 gives (sin h / h)^2 cos x cos y sin z; the
+    third-derivative stencil's symbol at O2 is i (sin 2t - 2 sin t) / h^3; and the per-axis
+    extent rule uses that axis's radius (an axis of order 0 accepts any N)."""
+    N = 16
+    h = 2 * np.pi / N
+    g = orc.Grid.make(N, N, N, h)
+    x = np.arange(N) * h
+    s, c = np.sin(x), np.cos(x)
+    psi = s[None, None, :] * s[None, :, None] * s[:, None, None]
+    want = (np.sin(h) / h) ** 2 * c[None, None, :] * c[None, :, None] * s[:, None, None]
+    assert np.abs(orc.mixed_partial(g, psi, 1, 1, 0, 2) - want).max() < 1e-12
+    want3 = (np.sin(2 * h) - 2 * np.sin(h)) / h ** 3 * c[None, None, :] * s[None, :, None] * s[:, None, None]
+    assert np.abs(orc.mixed_partial(g, psi, 3, 0, 0, 2) - want3).max() < 1e-9
+    g2 = orc.Grid.make(5, 2, 16)  # y of extent 2 cannot take any radius >= 1
+    orc.mixed_partial(g2, np.zeros(g2.shape), 2, 0, 2, 2)
+    with pytest.raises(orc.OracleError):
+        orc.mixed_partial(g2, np.zeros(g2.shape), 1, 1, 0, 2)
+    with pytest.raises(orc.OracleError):  # third derivative O4 has radius 3: N = 5 too small
+        orc.mixed_partial(g2, np.zeros(g2.shape), 3, 0, 0, 4)
+
+
+# ---- tensor contraction restatements ----------------------------------------------------------
+
+def test_tensor_restatements_pinned(orc):
+    """The per-site trace, symmetric part, matrix product and double contraction of the oracle
+    (column-major storage) against numpy's own trace, transpose, matmul and einsum."""
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal((2, 50, 9))
+    A = a.reshape(50, 3, 3).transpose(0, 2, 1)  # column-major storage -> A[r, c]
+    B = b.reshape(50, 3, 3).transpose(0, 2, 1)
+    assert np.allclose(orc.mat_trace(a), np.trace(A, axis1=1, axis2=2), rtol=0, atol=1e-14)
+    assert np.allclose(orc.mat_sym(a).reshape(50, 3, 3).transpose(0, 2, 1),
+                       0.5 * (A + A.transpose(0, 2, 1)), rtol=0, atol=1e-15)
+    assert np.allclose(orc.mat_mul(a, b).reshape(50, 3, 3).transpose(0, 2, 1), A @ B, rtol=0,
+                       atol=1e-14)
+    assert np.allclose(orc.mat_ddot(a, b), np.einsum("nij,nij->n", A, B), rtol=0, atol=1e-14)
+    # a transposed operand in the product would not pass
+    assert not np.allclose(orc.mat_mul(a, b).reshape(50, 3, 3).transpose(0, 2, 1),
+                           A.transpose(0, 2, 1) @ B, atol=1e-6)
