@@ -22,6 +22,7 @@ a bounded prefix of the same workload, with the host CPU model and the oracle's 
   python bench.py --workload tune [--n 256]       # autotuner: ms/step of every tile shape
   python bench.py --workload diffuse [--n 256]    # constant- and heterogeneous-D diffusion RK4
   python bench.py --workload reduce [--n 256]     # integrate() and the fused free energy
+  python bench.py --workload rhs                  # RK4 right-hand sides: host lambda, expr, fused
 """
 from __future__ import annotations
 
@@ -51,7 +52,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
-                                                          "learned", "tune", "diffuse", "reduce"])
+                                                          "learned", "tune", "diffuse", "reduce",
+                                                          "rhs"])
     p.add_argument("--n-ic", type=int, default=4)
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--diag-every", type=int, default=500)
@@ -668,6 +670,33 @@ def reduce(a):
     return {"metric": "reduction ms per call", "grid": [n, n, n], "rows": rows}
 
 
+def rhs(a):
+    """Right-hand-side paths of solve::integrate(..., RK4) through the C++ header API
+    (grid-calculus_amd/cpp/examples/rhs_paths.cpp, built in-tree): the caller's host-loop lambda
+    of the Cahn-Hilliard example, the same lambda written with tensor::expr expressions, and the
+    fused CahnHilliardRhs, on 128^2 and 64^3 (ms per RK4 step, field transfers per step,
+    identical bits); plus the wall clock of the unchanged reference example program
+    (cpp/bin/ref_examples/cahn_hilliard, 64^3, 100 RK4 steps, one diagnostic line and one
+    ASCII VTK snapshot at the end) when it was built."""
+    import subprocess
+    import tempfile
+    cpp = os.path.join(ROOT, "grid-calculus_amd", "cpp", "bin")
+    r = subprocess.run([os.path.join(cpp, "rhs_paths")], capture_output=True, text=True,
+                       timeout=600, check=True)
+    rows = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    out = {"metric": "RK4 ms per step by right-hand side", "rows": rows}
+    ex = os.path.join(cpp, "ref_examples", "cahn_hilliard")
+    if os.path.exists(ex):
+        with tempfile.TemporaryDirectory() as tmp:
+            r = subprocess.run([ex, "--n-x", "64", "--n-steps", "100", "--snapshot-every", "100",
+                                "--out-dir", tmp], capture_output=True, text=True, timeout=600,
+                               check=True)
+        wall = [float(x.split("=")[1].rstrip("s")) for x in r.stdout.split()
+                if x.startswith("wall_clock=")]
+        out["reference_example_64cubed_100_steps_s"] = wall[0] if wall else None
+    return out
+
+
 def copy(a):
     """Plain device-to-device copy of a known size (256 MiB): the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -695,7 +724,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     out = {"ch3d": ch3d, "sweep": sweep, "copy": copy, "ensemble": ensemble,
-           "learned": learned, "tune": tune, "diffuse": diffuse, "reduce": reduce}[a.workload](a)
+           "learned": learned, "tune": tune, "diffuse": diffuse, "reduce": reduce,
+           "rhs": rhs}[a.workload](a)
     print(json.dumps(out))
 
 

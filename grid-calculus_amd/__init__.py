@@ -127,6 +127,8 @@ def lib():
         "gc_context_scratch_bytes": ([vp], sz),
         "gc_malloc": ([C.POINTER(vp), sz], st),
         "gc_free": ([vp], st),
+        "gc_host_malloc": ([C.POINTER(vp), sz], st),
+        "gc_host_free": ([vp], st),
         "gc_memcpy_htod": ([vp, vp, vp, sz], st),
         "gc_memcpy_dtoh": ([vp, vp, vp, sz], st),
         "gc_memcpy_dtod": ([vp, vp, vp, sz], st),

@@ -24,6 +24,7 @@
 #include <gridcalc/solve/CahnHilliard.hpp>
 #include <gridcalc/solve/Diffusion.hpp>
 #include <gridcalc/solve/Integrator.hpp>
+#include <gridcalc/tensor/Expressions.hpp>
 
 #include "check.hpp"
 
@@ -312,6 +313,36 @@ TEST(CahnHilliard, CoarseningAcceptanceLambdaPath) {
   EXPECT_LT(std::abs(means.back() - means.front()), 1e-10);
   // the fused density gives the lambda density's value
   EXPECT_EQ(func::evaluate(psi, func::CahnHilliardDensity{0.5}), Fs.back());
+}
+
+// ---- right-hand sides that stay on the device ------------------------------------------------
+
+TEST(DeviceResidentRhs, ExpressionLambdaMatchesHostLoopsWithoutTransfers) {
+  // the example's right-hand side with its loops written as tensor::expr expressions: the
+  // same bits as the host loops, and not a single field transfer per step
+  namespace expr = gridcalc::tensor::expr;
+  for (const Grid& g : {Grid(32, 24, 20, Vec3d(1.0, 0.9, 1.1)), Grid(64, 48, 1, Vec3d(1.0, 1.0, 1.0))}) {
+    const auto psi0 = noiseField(g, 3, 0.1);
+    auto exprRhs = [](const Field<double>& y) {
+      const auto lap = diff::laplacian(y);
+      const auto p = expr::field(y);
+      const auto mu = expr::materialize(-p + p * p * p - 0.5 * expr::field(lap));
+      const auto r = diff::laplacian(mu);
+      return expr::materialize(1.3 * expr::field(r));
+    };
+    auto a = psi0;
+    auto b = psi0;
+    auto c = psi0;
+    gridcalc::hip::resetTransferStats();
+    integrate(a, exprRhs, 0.01, 6, RK4{});
+    const auto st = gridcalc::hip::transferStats();
+    EXPECT_EQ(st.uploads, std::size_t(1));  // psi0's host copy, once
+    EXPECT_EQ(st.downloads, std::size_t(0));
+    integrate(b, [](const Field<double>& y) { return lambdaChRhs(y, 1.3, 0.5); }, 0.01, 6, RK4{});
+    integrate(c, CahnHilliardRhs{1.3, 0.5}, 0.01, 6, RK4{});
+    EXPECT_TRUE(bitEqual(a, b));
+    EXPECT_TRUE(bitEqual(a, c));
+  }
 }
 
 // ---- context switch in the middle of a computation ------------------------------------------

@@ -194,6 +194,25 @@ gc_status gc_free(void* ptr) {
   return GC_OK;
 }
 
+gc_status gc_host_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return set_error(GC_ERR_INVALID, "gridcalc: null pointer");
+  *ptr = nullptr;
+  if (bytes == 0) return GC_OK;
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    (void)hipGetLastError();
+    return set_error(GC_ERR_NOMEM, "gridcalc: hipHostMalloc(%zu) failed: %s", bytes,
+                     hipGetErrorString(e));
+  }
+  return GC_OK;
+}
+
+gc_status gc_host_free(void* ptr) {
+  if (ptr) GC_HIP(hipHostFree(ptr));
+  return GC_OK;
+}
+
 gc_status gc_memcpy_htod(gc_context* ctx, void* dst, const void* src, size_t bytes) {
   GC_TRY(check_ctx(ctx));
   if (bytes == 0) return GC_OK;

@@ -1612,11 +1612,36 @@ struct CfgKey {
 std::mutex g_cfg_mu;
 std::map<CfgKey, int> g_cfg;
 
+inline long tile_blocks(const Dims& d, int ox, int oy, int zc) {
+  return (long)((d.nx + ox - 1) / ox) * ((d.ny + oy - 1) / oy) * ((d.nz + zc - 1) / zc);
+}
+
+// The shape used before (or without) gc_ch_tune.  3-D: the two-stage default measured best on
+// 256^3 while it gives every CU a workgroup; smaller grids are latency-bound (a workgroup
+// marches its z-chunk plane by plane), so they take the one-stage shape with the most
+// workgroups and, among those, the shortest z-chunk (measured: 64x8x8 is the fastest
+// candidate on 64^3 and 128^3 in both precisions, 3-5x the two-stage default there).
+int default_config(const Dims& d, int dt) {
+  if (d.nz == 1) return kDefault2;
+  const int pair = dt == GC_F64 ? kDefault3F64 : kDefault3F32;
+  const PairTile& pt = kChPairCfgs[pair - kN3];
+  const int poy = (dt == GC_F64 && pt.q) ? 16 : pt.oy;  // FP64 queue kernels run 64 x 16
+  if (tile_blocks(d, pt.ox, poy, pt.zc) >= 256) return pair;
+  int best = 0;
+  for (int i = 1; i < kN3; ++i) {
+    const TileCfg& c = kCh3Cfgs[i];
+    const TileCfg& b = kCh3Cfgs[best];
+    const long nc = tile_blocks(d, c.ox, c.oy, c.zc), nb = tile_blocks(d, b.ox, b.oy, b.zc);
+    if (nc > nb || (nc == nb && c.zc < b.zc)) best = i;
+  }
+  return best;
+}
+
 int ch_config(const Dims& d, int dt) {
   std::lock_guard<std::mutex> lock(g_cfg_mu);
   auto it = g_cfg.find(CfgKey{d.nx, d.ny, d.nz, dt});
   if (it != g_cfg.end()) return it->second;
-  return d.nz == 1 ? kDefault2 : (dt == GC_F64 ? kDefault3F64 : kDefault3F32);
+  return default_config(d, dt);
 }
 
 template <class T>

@@ -174,7 +174,7 @@ class Field {
   }
 
   Grid _grid;
-  mutable std::vector<T> _data;
+  mutable std::vector<T, hip::HostAllocator<T>> _data;  // page-locked host copy
   mutable void* _dev = nullptr;
   mutable State _state;
 };

@@ -11,8 +11,11 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <mutex>
+#include <new>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -40,6 +43,10 @@ struct State {
   gc_context* user_ctx = nullptr;
   std::multimap<std::size_t, void*> free_blocks;  // cached device blocks by size
   std::size_t cached_bytes = 0;
+  std::size_t uploads = 0, downloads = 0, upload_bytes = 0, download_bytes = 0;
+  std::multimap<std::size_t, void*> free_host;  // cached page-locked host blocks by size
+  std::size_t cached_host_bytes = 0;
+  std::set<void*> pageable;  // host blocks that fell back to malloc (pinning refused)
 };
 
 // Never destroyed: device memory and the context are reclaimed with the process, so no
@@ -112,11 +119,98 @@ inline void release(void* p, std::size_t bytes) noexcept {
   s.cached_bytes += bytes;
 }
 
+// Host <-> device field transfers made on behalf of Field (counted: transferStats()).
+// Host storage of Field: page-locked blocks (gc_host_malloc) cached by exact size, so the
+// transfers a caller's host loops cause run as DMA from pinned memory; up to kHostCacheBytes of
+// released blocks stay cached.  When the system refuses to pin more memory, blocks come from
+// malloc instead (still correct, transfers staged).
+constexpr std::size_t kHostCacheBytes = std::size_t(1) << 30;
+
+inline void* host_allocate(std::size_t bytes) {
+  if (bytes == 0) return nullptr;
+  auto& s = detail::state();
+  {
+    std::lock_guard<std::mutex> lock(s.mu);
+    auto it = s.free_host.find(bytes);
+    if (it != s.free_host.end()) {
+      void* p = it->second;
+      s.free_host.erase(it);
+      s.cached_host_bytes -= bytes;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (gc_host_malloc(&p, bytes) == GC_OK && p) return p;
+  p = std::malloc(bytes);
+  if (!p) throw std::bad_alloc();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.pageable.insert(p);
+  return p;
+}
+
+inline void host_release(void* p, std::size_t bytes) noexcept {
+  if (!p) return;
+  auto& s = detail::state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  auto pg = s.pageable.find(p);
+  if (pg != s.pageable.end()) {
+    s.pageable.erase(pg);
+    std::free(p);
+    return;
+  }
+  if (s.cached_host_bytes + bytes <= kHostCacheBytes) {
+    s.free_host.emplace(bytes, p);
+    s.cached_host_bytes += bytes;
+  } else {
+    (void)gc_host_free(p);
+  }
+}
+
+// std::allocator replacement for Field's host vector.
+template <class T>
+struct HostAllocator {
+  using value_type = T;
+  HostAllocator() noexcept = default;
+  template <class U>
+  HostAllocator(const HostAllocator<U>&) noexcept {}
+  T* allocate(std::size_t n) { return static_cast<T*>(host_allocate(n * sizeof(T))); }
+  void deallocate(T* p, std::size_t n) noexcept { host_release(p, n * sizeof(T)); }
+  template <class U>
+  bool operator==(const HostAllocator<U>&) const noexcept { return true; }
+  template <class U>
+  bool operator!=(const HostAllocator<U>&) const noexcept { return false; }
+};
+
 inline void upload(void* dev, const void* host, std::size_t bytes) {
   check(gc_memcpy_htod(context(), dev, host, bytes));
+  auto& s = detail::state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  ++s.uploads;
+  s.upload_bytes += bytes;
 }
 inline void download(void* host, const void* dev, std::size_t bytes) {
   check(gc_memcpy_dtoh(context(), host, dev, bytes));
+  auto& s = detail::state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  ++s.downloads;
+  s.download_bytes += bytes;
+}
+
+struct TransferStats {
+  std::size_t uploads, downloads, upload_bytes, download_bytes;
+};
+
+// Field transfers since the process started (or the last resetTransferStats()): how often a
+// caller's host loops made the library move a field between host and device.
+inline TransferStats transferStats() {
+  auto& s = detail::state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  return TransferStats{s.uploads, s.downloads, s.upload_bytes, s.download_bytes};
+}
+inline void resetTransferStats() {
+  auto& s = detail::state();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.uploads = s.downloads = s.upload_bytes = s.download_bytes = 0;
 }
 inline void copyDevice(void* dst, const void* src, std::size_t bytes) {
   check(gc_memcpy_dtod(context(), dst, src, bytes));

@@ -85,6 +85,11 @@ size_t gc_context_scratch_bytes(gc_context* ctx);
 
 gc_status gc_malloc(void** ptr, size_t bytes);
 gc_status gc_free(void* ptr);
+/* Page-locked host memory (hipHostMalloc) for host copies of fields: transfers to and from
+ * it run as DMA without staging.  gc_host_malloc fails with GC_ERR_NOMEM when the system
+ * cannot pin more memory; gc_host_free releases what it returned. */
+gc_status gc_host_malloc(void** ptr, size_t bytes);
+gc_status gc_host_free(void* ptr);
 /* Copies ordered on the context's stream; host-side copies complete before return. */
 gc_status gc_memcpy_htod(gc_context* ctx, void* dst, const void* src, size_t bytes);
 gc_status gc_memcpy_dtoh(gc_context* ctx, void* dst, const void* src, size_t bytes);
