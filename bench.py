@@ -126,7 +126,8 @@ def cpu_record(orc, extra_sample):
     """The oracle's build and this host's CPU, recorded beside every CPU timing."""
     info = orc.host_info()
     return {"cores": 1, "kind": "port", "cpu_model": info["cpu_model"],
-            "host_nproc": info["nproc"], "build": info["oracle_build"],
+            "host_cores_available": _HOST_CORES if _HOST_CORES is not None else info["nproc"],
+            "host_logical_cpus": os.cpu_count(), "build": info["oracle_build"],
             "sample": extra_sample}
 
 
@@ -345,9 +346,16 @@ def sweep(a):
     return {"metric": "operator GB/s (algorithmic bytes)", "n": n, "rows": rows, "cpu_baseline": cpu}
 
 
+_HOST_CORES = None  # cores this process could use before it pinned itself to one
+
+
 def _pin_one_core():
+    global _HOST_CORES
     try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        avail = sorted(os.sched_getaffinity(0))
+        if _HOST_CORES is None:
+            _HOST_CORES = len(avail)
+        os.sched_setaffinity(0, {avail[0]})
     except (AttributeError, OSError):
         pass
 

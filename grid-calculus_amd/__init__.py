@@ -148,6 +148,10 @@ def lib():
         "gc_integrate_noalloc": ([vp, gp, C.c_int, C.c_int, vp, vp], st),
         "gc_evaluate": ([vp, gp, C.c_int, C.POINTER(Density), C.c_int, vp, dp], st),
         "gc_expr_eval": ([vp, gp, C.POINTER(ExprInsn), C.c_int, C.c_int, C.c_int, vp], st),
+        "gc_snapshot_create": ([vp, sz, C.POINTER(vp)], st),
+        "gc_snapshot_capture": ([vp, vp, sz], st),
+        "gc_snapshot_wait": ([vp, C.POINTER(vp)], st),
+        "gc_snapshot_destroy": ([vp], st),
         "gc_ch_free_energy": ([vp, gp, C.c_int, C.c_double, C.c_int, vp, dp], st),
         "gc_gradient_energy": ([vp, gp, C.c_int, C.c_int, vp, dp], st),
         "gc_axpy": ([vp, C.c_int, sz, vp, C.c_double, vp, vp], st),
@@ -528,6 +532,40 @@ def ch_tune(ctx, g, dtype):
     best = C.c_int()
     _check(lib().gc_ch_tune(ctx.handle, C.byref(g), dtype, C.byref(best), ms))
     return best.value, [ms[i] for i in range(n)]
+
+
+# ---- asynchronous snapshots ---------------------------------------------------------------
+
+class Snapshot:
+    """gc_snapshot: capture() copies a device array into the slot on the context's stream and
+    queues its device-to-host copy on the slot's own stream; numpy() waits for that copy."""
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._h = C.c_void_p()
+        _check(lib().gc_snapshot_create(ctx.handle, self.nbytes, C.byref(self._h)))
+        ctx._register(self)
+
+    def capture(self, arr):
+        _check(lib().gc_snapshot_capture(self._h, arr.ptr, arr.nbytes))
+
+    def numpy(self):
+        p = C.c_void_p()
+        _check(lib().gc_snapshot_wait(self._h, C.byref(p)))
+        buf = (C.c_char * self.nbytes).from_address(p.value)
+        return np.frombuffer(buf, dtype=self.dtype).reshape(self.shape).copy()
+
+    def close(self):
+        if self._h:
+            lib().gc_snapshot_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- batched 2-D Cahn-Hilliard ensembles --------------------------------------------------

@@ -254,4 +254,76 @@ gc_status gc_timer_stop(gc_context* ctx, float* ms) {
   return GC_OK;
 }
 
+// ---- asynchronous snapshots ------------------------------------------------------------
+
+struct gc_snapshot {
+  gc_context* ctx = nullptr;
+  hipStream_t copy = nullptr;
+  hipEvent_t ready = nullptr, done = nullptr;
+  void* dev = nullptr;
+  void* host = nullptr;
+  size_t bytes = 0;
+  bool pending = false;
+};
+
+gc_status gc_snapshot_destroy(gc_snapshot* s) {
+  if (!s) return GC_OK;
+  (void)hipSetDevice(s->ctx->device);
+  if (s->copy) (void)hipStreamSynchronize(s->copy);
+  if (s->ready) (void)hipEventDestroy(s->ready);
+  if (s->done) (void)hipEventDestroy(s->done);
+  if (s->copy) (void)hipStreamDestroy(s->copy);
+  if (s->dev) (void)hipFree(s->dev);
+  if (s->host) (void)hipHostFree(s->host);
+  delete s;
+  return GC_OK;
+}
+
+gc_status gc_snapshot_create(gc_context* ctx, size_t bytes, gc_snapshot** out) {
+  if (!out) return set_error(GC_ERR_INVALID, "gridcalc: null snapshot pointer");
+  *out = nullptr;
+  GC_TRY(check_ctx(ctx));
+  if (bytes == 0) return set_error(GC_ERR_INVALID, "gridcalc: empty snapshot");
+  gc_snapshot* s = new gc_snapshot();
+  s->ctx = ctx;
+  s->bytes = bytes;
+  if (hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->done, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&s->dev, bytes) != hipSuccess ||
+      hipHostMalloc(&s->host, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    gc_snapshot_destroy(s);
+    return set_error(GC_ERR_NOMEM, "gridcalc: snapshot set-up of %zu bytes failed", bytes);
+  }
+  *out = s;
+  return GC_OK;
+}
+
+gc_status gc_snapshot_capture(gc_snapshot* s, const void* src_dev, size_t bytes) {
+  if (!s) return set_error(GC_ERR_INVALID, "gridcalc: null snapshot");
+  GC_TRY(check_ctx(s->ctx));
+  if (!src_dev) return set_error(GC_ERR_INVALID, "gridcalc: null snapshot source");
+  if (bytes != s->bytes)
+    return set_error(GC_ERR_INVALID, "gridcalc: snapshot of %zu bytes into a slot of %zu", bytes,
+                     s->bytes);
+  if (s->pending) GC_HIP(hipEventSynchronize(s->done));  // the staging buffer is free again
+  GC_HIP(hipMemcpyAsync(s->dev, src_dev, bytes, hipMemcpyDeviceToDevice, s->ctx->stream));
+  GC_HIP(hipEventRecord(s->ready, s->ctx->stream));
+  GC_HIP(hipStreamWaitEvent(s->copy, s->ready, 0));
+  GC_HIP(hipMemcpyAsync(s->host, s->dev, bytes, hipMemcpyDeviceToHost, s->copy));
+  GC_HIP(hipEventRecord(s->done, s->copy));
+  s->pending = true;
+  return GC_OK;
+}
+
+gc_status gc_snapshot_wait(gc_snapshot* s, const void** host) {
+  if (!s || !host) return set_error(GC_ERR_INVALID, "gridcalc: null snapshot argument");
+  GC_TRY(check_ctx(s->ctx));
+  if (s->pending) GC_HIP(hipEventSynchronize(s->done));
+  s->pending = false;
+  *host = s->host;
+  return GC_OK;
+}
+
 }  // extern "C"

@@ -162,6 +162,22 @@ gc_status gc_integrate(gc_context* ctx, const gc_grid* g, gc_dtype dt, int metho
  * allocates). */
 gc_status gc_integrate_noalloc(gc_context* ctx, const gc_grid* g, gc_dtype dt, int method,
                                const void* in, double* result_dev);
+/* ---- asynchronous snapshots ------------------------------------------------------ */
+
+/* A snapshot slot for writing fields out while stepping continues (the VTK snapshots of
+ * examples/common/vtk_writer.hpp:34-67 and examples/cahn_hilliard.cpp:130-135).  It owns a
+ * device staging buffer, a page-locked host buffer of `bytes`, and a copy stream of its own
+ * (default priority).  gc_snapshot_capture copies the field into the staging buffer on the
+ * context's stream (ordered after the work that produced it) and queues the device-to-host
+ * copy on the copy stream, so later kernels on the context's stream do not wait for it;
+ * gc_snapshot_wait blocks until that copy is done and returns the host buffer.  A slot
+ * holds one capture at a time (capture waits for the previous one). */
+typedef struct gc_snapshot gc_snapshot;
+gc_status gc_snapshot_create(gc_context* ctx, size_t bytes, gc_snapshot** out);
+gc_status gc_snapshot_capture(gc_snapshot* s, const void* src_dev, size_t bytes);
+gc_status gc_snapshot_wait(gc_snapshot* s, const void** host);
+gc_status gc_snapshot_destroy(gc_snapshot* s);
+
 /* ---- tensor contraction expressions ------------------------------------------ */
 
 /* One instruction of a per-site register program (tensor::expr of tensor/Expressions.hpp,

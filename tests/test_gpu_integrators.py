@@ -132,3 +132,23 @@ def test_pointwise_steps(gc, ctx, dtype):
     gc.rk4_combine(ctx, da, dt, db, dc, de, df)
     sixth, third = dtype(dt / 6.0), dtype(dt / 3.0)
     assert np.array_equal(da.numpy(ctx), a1 + (sixth * (b + f) + third * (c + e)))
+
+
+def test_snapshot_capture_while_stepping(gc, orc, ctx):
+    """gc_snapshot: each capture holds the state at the time of the call, while the stepping
+    queued right after it (on the context's stream) changes the field."""
+    g = gc.Grid.make(40, 24, 16)
+    og = orc.Grid.make(40, 24, 16)
+    c0 = gc.inputs.ch_initial(40, 24, 16, 0.1, seed=8)
+    d = gc.DeviceArray.from_numpy(ctx, c0)
+    snaps = [gc.Snapshot(ctx, g.shape) for _ in range(3)]
+    for s in snaps:
+        s.capture(d)
+        gc.ch_integrate(ctx, g, d, 1.0, 0.5, 0.01, 5, gc.RK4)
+    want = c0
+    for s in snaps:
+        assert np.array_equal(s.numpy(), want)
+        want = orc.ch_integrate(og, want, 1.0, 0.5, 0.01, 5, orc.RK4)
+    assert np.array_equal(d.numpy(ctx), want)
+    for s in snaps:
+        s.close()
