@@ -187,6 +187,7 @@ def lib():
         "gc_cnn_forward": ([vp, C.c_int, vp, C.c_int, vp], st),
         "gc_cnn_forward_layers": ([vp, C.c_int, vp, C.c_int, vp], st),
         "gc_cnn_loss_grad": ([vp, C.c_int, vp, vp, ip, C.c_int, dp], st),
+        "gc_cnn_loss_grad_chain": ([vp, C.c_int, vp, C.c_size_t, ip, C.c_int, C.c_int, dp, dp], st),
         "gc_cnn_adam": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], st),
         "gc_cnn_predict": ([vp, C.c_int, vp, vp, C.c_int], st),
         "gc_cast": ([vp, C.c_int, C.c_int, sz, vp, vp], st),
@@ -699,6 +700,23 @@ class Corrector:
             ip = None
         loss = C.c_double()
         _check(lib().gc_cnn_loss_grad(self._h, c.gc_dtype, c.ptr, target.ptr, ip, B, C.byref(loss)))
+        return loss.value
+
+    def chain_loss_grad(self, data, idx, K, weights=None):
+        """Unrolled-chain loss and gradient (gc_cnn_loss_grad_chain): chains start at images
+        idx of device array data (images along its leading axes, H x W each); targets are the
+        K images after each start; weights (K values, default 1/K each) weigh the steps."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        n_images = int(np.prod(data.shape)) // (self.H * self.W)
+        wp = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            assert w.size == K
+            wp = w.ctypes.data_as(C.POINTER(C.c_double))
+        loss = C.c_double()
+        _check(lib().gc_cnn_loss_grad_chain(self._h, data.gc_dtype, data.ptr, n_images,
+                                            idx.ctypes.data_as(C.POINTER(C.c_int)), idx.size, K,
+                                            wp, C.byref(loss)))
         return loss.value
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):

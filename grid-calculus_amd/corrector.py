@@ -52,10 +52,23 @@ CONFIG = dict(H=256, W=256, kappa=1.0, M=1.0, dt=0.01, substeps=16, n_traj=2000,
 # (snapshot m, snapshot m + 1): the network learns to advance the states its own errors lead
 # to, which cuts the error that accumulates over the slow late phase (the pushforward trick of
 # Brandstetter, Worrall and Welling, "Message passing neural PDE solvers", ICLR 2022).
+# A seventh stage entry K > 1 trains instead on unrolled chains: from snapshot m, K
+# applications of the residual update, the loss on every one of snapshots m + 1 .. m + K
+# (weight 1/K each) and its gradient carried back through all K applications
+# (gc_cnn_loss_grad_chain), so the weights are fitted to the error after several steps of
+# their own, not only to the one-step error.
+# The shipped parameters were made by these stages; each phase's json file records, per stage,
+# the forward arithmetic of its 32 -> 32 layers (the first stages ran when those layers were
+# evaluated in BF16; the later ones, with chains, re-fitted the weights to the FP16 forward
+# pass the kernels use now).
 WARM = 1
-PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666, 0),)),
+PHASES = (("early", 1, 40, ((12, 2000, 3e-4, 3e-6, 66666, 0), (8, 2000, 1e-4, 1e-6, 31001, 0, 1),
+                            (8, 2000, 3e-5, 3e-7, 32001, 0, 3), (8, 2000, 2e-5, 2e-7, 33001, 0, 4))),
           ("late", 40, 125, ((12, 2000, 3e-4, 3e-6, 55555, 0), (12, 2000, 1e-4, 1e-6, 77777, 0),
-                             (20, 2000, 1e-4, 1e-6, 133331, 2), (16, 2000, 5e-5, 5e-7, 155551, 2))))
+                             (20, 2000, 1e-4, 1e-6, 133331, 2), (16, 2000, 5e-5, 5e-7, 155551, 2),
+                             (8, 2000, 5e-5, 5e-7, 31002, 0, 1), (8, 2000, 3e-5, 3e-7, 31003, 0, 3),
+                             (8, 2000, 2e-5, 2e-7, 32002, 0, 4),
+                             (10, 2000, 1.5e-5, 1.5e-7, 33002, 0, 4))))
 
 
 def params_path(H, W, phase=None):
@@ -214,8 +227,9 @@ def train(cfg=CONFIG, log=print, out_dir=DATA_DIR):
 def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
     """Continues training `params` on pairs whose first snapshot is macro step m_lo..m_hi-1
     (uniformly), through `stages` = ((batches, iterations per batch, lr max, lr min, seed,
-    pushforward period), ...), each a cosine schedule over fresh trajectories (see PHASES for
-    the pushforward iterations).  Returns (params, loss history)."""
+    pushforward period[, chain length K]), ...), each a cosine schedule over fresh
+    trajectories (see PHASES for the pushforward and chain iterations).  Returns (params,
+    loss history)."""
     H, W, mb = cfg["H"], cfg["W"], cfg["minibatch"]
     hw = H * W
     history = []
@@ -230,7 +244,8 @@ def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
                 _check(lib().gc_memcpy_dtod(ctx.handle, C.c_void_p(dst.ptr.value + b * hw * 4),
                                             C.c_void_p(data.ptr.value + int(r) * hw * 4), hw * 4))
 
-        for n_batches, iters, lr_max, lr_min, seed, pf in stages:
+        for n_batches, iters, lr_max, lr_min, seed, pf, *chain in stages:
+            K = chain[0] if chain else 1
             net.set_params(p, scale)  # fresh Adam state per stage
             rng = np.random.default_rng(seed)
             total = n_batches * iters
@@ -249,7 +264,10 @@ def finetune(params, scale, m_lo, m_hi, stages, cfg=CONFIG, log=print):
                     frac = it / max(1, total - 1)
                     lr = lr_min + 0.5 * (lr_max - lr_min) * (1 + math.cos(math.pi * frac))
                     tr = rng.integers(0, cfg["batch_traj"], mb)
-                    if pf and it % pf:  # pushforward: net(snapshot m) -> snapshot m + 2
+                    if K > 1:  # unrolled chain: snapshot m through K applications
+                        m = rng.integers(m_lo, min(m_hi, n_macro - K + 1), mb)
+                        losses.append(net.chain_loss_grad(data, tr * per_traj + m, K))
+                    elif pf and it % pf:  # pushforward: net(snapshot m) -> snapshot m + 2
                         m = rng.integers(m_lo, min(m_hi, n_macro - 1), mb)
                         gather(xs, data, tr * per_traj + m)
                         gather(ts, data, tr * per_traj + m + 2)
@@ -286,13 +304,17 @@ def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR, only=None):
         if only is not None and name not in only:
             continue
         t0 = time.perf_counter()
-        start, done, hist0, sec0 = base, 0, [], 0.0
+        start, done, hist0, sec0, arith0 = base, 0, [], 0.0, []
         prev, pmeta = load_params(H, W, name)
         if prev is not None and pmeta.get("config") == cfg:
             k = len(pmeta["stages"])
-            if 0 < k < len(stages) and pmeta["stages"] == [list(st) for st in stages[:k]]:
+            if 0 < k <= len(stages) and pmeta["stages"] == [list(st) for st in stages[:k]]:
+                if k == len(stages):
+                    log(f"  {name}: every stage done")
+                    continue
                 start, done, hist0 = prev, k, list(pmeta["history"])
                 sec0 = float(pmeta.get("seconds", 0.0))
+                arith0 = list(pmeta.get("forward_arithmetic", ["bf16"] * k))
                 log(f"  {name}: continuing after stage {k} of {len(stages)}")
         p, history = finetune(start, meta["scale"], m_lo, m_hi, stages[done:], cfg, log)
         history = hist0 + history
@@ -300,8 +322,9 @@ def train_phases(cfg=CONFIG, log=print, out_dir=DATA_DIR, only=None):
         np.save(path, p)
         json.dump(dict(scale=meta["scale"], history=history, macro_steps=[m_lo, m_hi],
                        stages=[list(st) for st in stages], config=cfg,
-                       seconds=sec0 + time.perf_counter() - t0), open(path[:-4] + ".json", "w"),
-                  indent=1)
+                       seconds=sec0 + time.perf_counter() - t0,
+                       forward_arithmetic=arith0 + ["fp16"] * (len(stages) - done)),
+                  open(path[:-4] + ".json", "w"), indent=1)
         log(f"  wrote {path}")
 
 

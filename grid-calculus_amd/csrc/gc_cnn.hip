@@ -6,14 +6,19 @@
 //   out = conv(a5, W6) + b6,      prediction = c + scale * (out - mean(out))
 //
 // conv(x, W)[y, x] = sum_tap W[tap] x[(y+dy) mod H, (x+dx) mod W], tap = 3(dy+1) + (dx+1).
-// The 32 -> 32 layers take BF16 weights and activations; the 1 -> 32 and 32 -> 1 layers
-// take the state and their weights in FP32; every accumulation is FP32.  The four 32 -> 32 layers
-// carry 98 % of the FLOPs and run as implicit GEMMs on the matrix cores
-// (v_mfma_f32_32x32x16_bf16): M = 32 output channels, N = 32 sites, K = 9 taps x 32 input
-// channels (18 MFMAs per 32 x 32 output tile; a lane ends with 16 channels of one site).
-// Backward-data is the same convolution with the taps flipped and the weight matrix
-// transposed; backward-weights is a GEMM over sites whose operands are staged channel-major
-// in LDS.  The 1 -> 32 and 32 -> 1 layers (288 MACs per site) run on the vector ALUs.
+// In the forward pass the 32 -> 32 layers take FP16 weights and activations (10-bit
+// significands: with BF16's 7 the roundings alone left the trained network's one-step error
+// several times its exact-arithmetic error); the 1 -> 32 and 32 -> 1 layers take the state
+// and their weights in FP32; every accumulation is FP32.  The four 32 -> 32 layers carry 98 %
+// of the FLOPs and run as implicit GEMMs on the matrix cores (v_mfma_f32_32x32x16_f16):
+// M = 32 output channels, N = 32 sites, K = 9 taps x 32 input channels (18 MFMAs per 32 x 32
+// output tile; a lane ends with 16 channels of one site).  The backward pass runs in BF16
+// (v_mfma_f32_32x32x16_bf16), whose FP32 exponent range holds the small loss gradients that
+// FP16 would flush: backward-data is the same convolution with the taps flipped and the
+// weight matrix transposed (BF16 weights, BF16 pre-activation gradients); backward-weights is
+// a GEMM over sites whose operands are staged channel-major in LDS, the FP16 activations
+// rounded to BF16 there.  The 1 -> 32 and 32 -> 1 layers (288 MACs per site) run on the
+// vector ALUs.
 //
 // Parameters are one FP32 vector (37601 values): W1[co][tap], b1, {W_l[co][tap][ci], b_l}
 // for l = 2..5, W6[tap][ci], b6.  Activations are NHWC (32 channels contiguous per site).
@@ -34,9 +39,19 @@ namespace {
 
 using namespace gc;
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+// One 32 x 32 x 16 matrix-core step in either 16-bit format.
+__device__ __forceinline__ f32x16 mfma16(const f16x8& a, const f16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma16(const bf16x8& a, const bf16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 
 constexpr int kC = 32;            // channels
 constexpr int kMidW = 9 * 32 * 32;  // weights of a 32 -> 32 layer
@@ -100,7 +115,7 @@ template <class TS>
 __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const int* __restrict__ idx,
                                                const float* __restrict__ w1,
                                                const float* __restrict__ b1, float* __restrict__ z1,
-                                               __bf16* __restrict__ a1, int H, int W, int B) {
+                                               _Float16* __restrict__ a1, int H, int W, int B) {
   const int HW = H * W;
   const int lane = threadIdx.x & 63, j = lane & 31, cq = (lane >> 5) * 4;
   const long s0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
@@ -116,29 +131,44 @@ __global__ __launch_bounds__(256) void k_conv1(const TS* __restrict__ c, const i
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     float v[4];
-    bf16x4 av;
+    f16x4 av;
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = acc[4 * g + i] + b1[g * 8 + cq + i];
     if (z1) *(float4*)(z1 + (size_t)s * kC + g * 8 + cq) = make_float4(v[0], v[1], v[2], v[3]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) av[i] = (__bf16)gelu_f(v[i]);
-    *(bf16x4*)(a1 + (size_t)s * kC + g * 8 + cq) = av;
+    for (int i = 0; i < 4; ++i) av[i] = (_Float16)gelu_f(v[i]);
+    *(f16x4*)(a1 + (size_t)s * kC + g * 8 + cq) = av;
   }
 }
 
 // ---- 32 -> 32 layers, matrix cores --------------------------------------------------------
-// MODE 0 (forward): out = bf16(gelu(acc + bias)), zout = acc + bias (optional).
-// MODE 1 (backward-data): out = bf16(acc * gelu'(zprev)).
+// MODE 0 (forward, FP16): out = fp16(gelu(acc + bias)), zout = acc + bias (optional).
+// MODE 1 (backward-data, BF16): out = bf16(acc * gelu'(zprev)).
 // Block: 256 threads, one kTR x kTC site tile of one image; wave w computes rows 2w, 2w+1.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
-                                                  const __bf16* __restrict__ wt,
+struct MidTypes {
+  typedef _Float16 T;
+  typedef f16x8 T8;
+  typedef f16x4 T4;
+};
+template <>
+struct MidTypes<1> {
+  typedef __bf16 T;
+  typedef bf16x8 T8;
+  typedef bf16x4 T4;
+};
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_mid(const typename MidTypes<MODE>::T* __restrict__ in,
+                                                  const typename MidTypes<MODE>::T* __restrict__ wt,
                                                   const float* __restrict__ bias,
                                                   const float* __restrict__ zprev,
                                                   float* __restrict__ zout,
-                                                  __bf16* __restrict__ out, int H, int W,
-                                                  int tiles_x, int tiles_y) {
-  __shared__ __attribute__((aligned(16))) __bf16 xs[(kTR + 2) * (kTC + 2) * kSP];
+                                                  typename MidTypes<MODE>::T* __restrict__ out,
+                                                  int H, int W, int tiles_x, int tiles_y) {
+  typedef typename MidTypes<MODE>::T T;
+  typedef typename MidTypes<MODE>::T8 T8;
+  typedef typename MidTypes<MODE>::T4 T4;
+  __shared__ __attribute__((aligned(16))) T xs[(kTR + 2) * (kTC + 2) * kSP];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
@@ -150,12 +180,12 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
     const int site = q >> 2, ch = (q & 3) * 8;
     const int ly = site / (kTC + 2), lx = site % (kTC + 2);
     const int gy = wrap1(y0 + ly - 1, H), gx = wrap1(x0 + lx - 1, W);
-    *(bf16x8*)(xs + site * kSP + ch) = *(const bf16x8*)(in + (img + (size_t)gy * W + gx) * kC + ch);
+    *(T8*)(xs + site * kSP + ch) = *(const T8*)(in + (img + (size_t)gy * W + gx) * kC + ch);
   }
   const int j = lane & 31, kh = (lane >> 5) * 8;
-  bf16x8 wreg[18];
+  T8 wreg[18];
 #pragma unroll
-  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8);
+  for (int kc = 0; kc < 18; ++kc) wreg[kc] = *(const T8*)(wt + (size_t)(kc * 64 + lane) * 8);
   __syncthreads();
 #pragma unroll
   for (int rr = 0; rr < 2; ++rr) {
@@ -165,8 +195,8 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
     for (int kc = 0; kc < 18; ++kc) {
       const int tap = kc >> 1, dy = tap / 3 - 1, dx = tap % 3 - 1;
       const int site = (r + 1 + dy) * (kTC + 2) + (j + 1 + dx);
-      const bf16x8 a = *(const bf16x8*)(xs + site * kSP + (kc & 1) * 16 + kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wreg[kc], a, acc, 0, 0, 0);
+      const T8 a = *(const T8*)(xs + site * kSP + (kc & 1) * 16 + kh);
+      acc = mfma16(wreg[kc], a, acc);
     }
     // lane: site x0 + j of the row, channels 8 g + 4 (lane / 32) + i in acc[4 g + i]
     const size_t o0 = (img + (size_t)(y0 + r) * W + x0 + j) * kC + (lane >> 5) * 4;
@@ -174,7 +204,7 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
     for (int g = 0; g < 4; ++g) {
       const size_t o = o0 + g * 8;
       float v[4];
-      bf16x4 ov;
+      T4 ov;
 #pragma unroll
       for (int i = 0; i < 4; ++i) v[i] = acc[4 * g + i];
       if (MODE == 0) {
@@ -182,15 +212,15 @@ __global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in,
         for (int i = 0; i < 4; ++i) v[i] += bias[g * 8 + (lane >> 5) * 4 + i];
         if (zout) *(float4*)(zout + o) = make_float4(v[0], v[1], v[2], v[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(v[i]);
+        for (int i = 0; i < 4; ++i) ov[i] = (T)gelu_f(v[i]);
       } else {
         const float4 zp = *(const float4*)(zprev + o);
-        ov[0] = (__bf16)(v[0] * gelu_grad_f(zp.x));
-        ov[1] = (__bf16)(v[1] * gelu_grad_f(zp.y));
-        ov[2] = (__bf16)(v[2] * gelu_grad_f(zp.z));
-        ov[3] = (__bf16)(v[3] * gelu_grad_f(zp.w));
+        ov[0] = (T)(v[0] * gelu_grad_f(zp.x));
+        ov[1] = (T)(v[1] * gelu_grad_f(zp.y));
+        ov[2] = (T)(v[2] * gelu_grad_f(zp.z));
+        ov[3] = (T)(v[3] * gelu_grad_f(zp.w));
       }
-      *(bf16x4*)(out + o) = ov;
+      *(T4*)(out + o) = ov;
     }
   }
 }
@@ -302,7 +332,7 @@ __device__ __forceinline__ void mass_fe_reduce(const Comp2* __restrict__ part, i
 // A block owns an 8 x 32 output tile.  Kernel A evaluates layers 1..3 on it, each on the tile
 // plus the halo the later layers of the kernel need (layer 1 on a 2-site halo, layer 2 on a
 // 1-site halo, layer 3 on the tile), from the state region (halo 3) staged in LDS as FP32, and
-// writes layer 3's activations (BF16, site-major, like the per-layer path) to global memory;
+// writes layer 3's activations (FP16, site-major, like the per-layer path) to global memory;
 // kernel B reads them on a 3-site halo and evaluates layers 4..6 the same way.  Activations
 // ping-pong between two LDS buffers (site stride kSP).  Every value is formed with the
 // per-layer kernels' arithmetic (same FMA order, same MFMA operand fragments and K order), so
@@ -340,18 +370,18 @@ struct Chain {
 // One 32 -> 32 layer (index l of the four) on the matrix cores, output region = tile + halo
 // HO, input (in, LDS, site stride kSP) one site wider.  The output region's sites, row-major,
 // are cut into chunks of 32, one per wave (16 waves; every region here has at most 14);
-// store(site, g, v) receives channels 8 g + cq .. 8 g + cq + 3 of an output site as BF16.
+// store(site, g, v) receives channels 8 g + cq .. 8 g + cq + 3 of an output site as FP16.
 // The weight fragments (fragment-major, one contiguous 1 KiB per load) are read as the MFMAs
 // need them: four waves per SIMD hide the load latency.
 template <int HO, class Store>
-__device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
-                                          const __bf16* __restrict__ wmid,
+__device__ __forceinline__ void mid_layer(const _Float16* __restrict__ in, int l,
+                                          const _Float16* __restrict__ wmid,
                                           const float* __restrict__ p, int wave, int lane,
                                           Store store) {
   constexpr int Wo = kFC + 2 * HO, Ho = kFR + 2 * HO, Wi = Wo + 2, NS = Wo * Ho;
   constexpr int nch = (NS + 31) / 32;
   const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
-  const __bf16* wt = wmid + (size_t)l * kMidW;
+  const _Float16* wt = wmid + (size_t)l * kMidW;
   float bj[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) bj[i] = p[kOffMid + l * kMidStride + kMidW + (i / 4) * 8 + cq + i % 4];
@@ -363,16 +393,15 @@ __device__ __forceinline__ void mid_layer(const __bf16* __restrict__ in, int l,
     for (int kc = 0; kc < 18; ++kc) {
       const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
       const int ko = (kc & 1) * 16 + kh;
-      const bf16x8 a = *(const bf16x8*)(in + (i0 + off) * kSP + ko);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-          *(const bf16x8*)(wt + (size_t)(kc * 64 + lane) * 8), a, acc, 0, 0, 0);
+      const f16x8 a = *(const f16x8*)(in + (i0 + off) * kSP + ko);
+      acc = mfma16(*(const f16x8*)(wt + (size_t)(kc * 64 + lane) * 8), a, acc);
     }
     if (c * 32 + j < NS) {
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        bf16x4 ov;
+        f16x4 ov;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ov[i] = (__bf16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
+        for (int i = 0; i < 4; ++i) ov[i] = (_Float16)gelu_f(acc[4 * g + i] + bj[4 * g + i]);
         store(c * 32 + j, g * 8 + cq, ov);
       }
     }
@@ -384,15 +413,15 @@ template <class TS, bool APPLY>
 __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
                                                const float* __restrict__ w1,
                                                const float* __restrict__ b1,
-                                               const __bf16* __restrict__ wmid,
+                                               const _Float16* __restrict__ wmid,
                                                const float* __restrict__ p,
-                                               __bf16* __restrict__ a3, int H, int W,
+                                               _Float16* __restrict__ a3, int H, int W,
                                                int tiles_x, int tiles_y, Chain<TS> ch) {
   __shared__ float cs[kAH0 * kAW0];
   __shared__ Comp2 msh[kFR * kFC / 64];
   __shared__ float w1s[288 + 32];
-  __shared__ __attribute__((aligned(16))) __bf16 bufA[kAN1 * kSP];
-  __shared__ __attribute__((aligned(16))) __bf16 bufB[kAN2 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 bufA[kAN1 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 bufB[kAN2 * kSP];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
@@ -487,30 +516,30 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
       if (sr < RN) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-          bf16x4 av;
+          f16x4 av;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) av[i] = (__bf16)gelu_f(acc[4 * g + i] + w1s[288 + g * 8 + cq + i]);
-          *(bf16x4*)(bufA + sr * kSP + g * 8 + cq) = av;
+          for (int i = 0; i < 4; ++i) av[i] = (_Float16)gelu_f(acc[4 * g + i] + w1s[288 + g * 8 + cq + i]);
+          *(f16x4*)(bufA + sr * kSP + g * 8 + cq) = av;
         }
       }
     };
     for (int cc = wave; cc < nch; cc += kFNW) emit1(chunk(cc), cc * 32 + j);
   }
   __syncthreads();
-  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
-    *(bf16x4*)(bufB + s * kSP + ch0) = v;
+  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+    *(f16x4*)(bufB + s * kSP + ch0) = v;
   });
   __syncthreads();
   const size_t img = (size_t)b * H * W;
-  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
-    *(bf16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
+  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+    *(f16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
   });
 }
 
 // Kernel B: layers 4..6 from layer 3's activations; output to out, the tile's output sum to
 // part[block].
-__global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
-                                               const __bf16* __restrict__ wmid,
+__global__ __launch_bounds__(kFNT) void k_fwd_b(const _Float16* __restrict__ a3,
+                                               const _Float16* __restrict__ wmid,
                                                const float* __restrict__ p,
                                                const float* __restrict__ w6,
                                                const float* __restrict__ b6,
@@ -519,8 +548,8 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
                                                int tiles_x, int tiles_y) {
   __shared__ double psum[kFR * kFC / 32];
   __shared__ float w6s[288];
-  __shared__ __attribute__((aligned(16))) __bf16 bufA[kBN3 * kSP];
-  __shared__ __attribute__((aligned(16))) __bf16 bufB[kBN4 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 bufA[kBN3 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 bufB[kBN4 * kSP];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
@@ -529,29 +558,29 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
   const size_t img = (size_t)b * H * W;
   // layer 3's activations on the halo-3 region: every load issued before the first store
   constexpr int NQ = (kBN3 * 4 + kFNT - 1) / kFNT;
-  bf16x8 rv[NQ];
+  f16x8 rv[NQ];
 #pragma unroll
   for (int r = 0; r < NQ; ++r) {
     const int q = t + r * kFNT;
     if (q < kBN3 * 4) {
       const int site = q >> 2, ch = (q & 3) * 8;
       const int gy = wrap1(y0 + site / (kFC + 6) - 3, H), gx = wrap1(x0 + site % (kFC + 6) - 3, W);
-      rv[r] = *(const bf16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
+      rv[r] = *(const f16x8*)(a3 + (img + (size_t)gy * W + gx) * kC + ch);
     }
   }
 #pragma unroll
   for (int r = 0; r < NQ; ++r) {
     const int q = t + r * kFNT;
-    if (q < kBN3 * 4) *(bf16x8*)(bufA + (q >> 2) * kSP + (q & 3) * 8) = rv[r];
+    if (q < kBN3 * 4) *(f16x8*)(bufA + (q >> 2) * kSP + (q & 3) * 8) = rv[r];
   }
   for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
   __syncthreads();
-  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
-    *(bf16x4*)(bufB + s * kSP + ch0) = v;
+  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+    *(f16x4*)(bufB + s * kSP + ch0) = v;
   });
   __syncthreads();
-  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const bf16x4& v) {
-    *(bf16x4*)(bufA + s * kSP + ch0) = v;
+  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+    *(f16x4*)(bufA + s * kSP + ch0) = v;
   });
   __syncthreads();
   // layer 6 on the tile from the layer-5 region (halo 1, in bufA): lanes 0..31 of wave w
@@ -566,7 +595,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
       const int site = (ry + tp / 3) * Wi + rx + tp % 3;
 #pragma unroll
       for (int ch = 16 * h; ch < 16 * h + 16; ch += 8) {
-        const bf16x8 v = *(const bf16x8*)(bufA + site * kSP + ch);
+        const f16x8 v = *(const f16x8*)(bufA + site * kSP + ch);
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc = fmaf(w6s[tp * kC + ch + u], (float)v[u], acc);
       }
@@ -587,11 +616,11 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const __bf16* __restrict__ a3,
 }
 
 // Backward-weights of a 32 -> 32 layer: dW[co][tap][ci] = sum_s dz[s][co] x[s+off][ci],
-// db[co] = sum_s dz[s][co].  Block: one tile; operands staged channel-major in LDS (x in
-// three copies shifted by dx so every 16-byte read is aligned).  Per-block partial sums go
+// db[co] = sum_s dz[s][co].  Block: one tile; operands staged channel-major in LDS (x, FP16,
+// rounded to BF16 there, in three copies shifted by dx so every 16-byte read is aligned).  Per-block partial sums go
 // to part[block][9248] (reduced later in a fixed order).
 __global__ __launch_bounds__(256) void k_conv_mid_bwdw(const __bf16* __restrict__ dz,
-                                                       const __bf16* __restrict__ x,
+                                                       const _Float16* __restrict__ x,
                                                        float* __restrict__ part, int H, int W,
                                                        int tiles_x, int tiles_y, int ntiles) {
   constexpr int CS = kTC + 8;  // column stride of the channel-major planes (bf16)
@@ -622,7 +651,10 @@ __global__ __launch_bounds__(256) void k_conv_mid_bwdw(const __bf16* __restrict_
       const int site = q >> 2, ch = (q & 3) * 8;
       const int ly = site / (kTC + 2), lx = site % (kTC + 2);  // lx: column x0 + lx - 1
       const int gy = wrap1(y0 + ly - 1, H), gx = wrap1(x0 + lx - 1, W);
-      const bf16x8 v = *(const bf16x8*)(x + (img + (size_t)gy * W + gx) * kC + ch);
+      const f16x8 v16 = *(const f16x8*)(x + (img + (size_t)gy * W + gx) * kC + ch);
+      bf16x8 v;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (__bf16)(float)v16[u];
 #pragma unroll
       for (int sh = 0; sh < 3; ++sh) {  // copy sh holds column (x0 + c + sh - 1) at c
         const int c = lx - sh;
@@ -681,7 +713,7 @@ __global__ __launch_bounds__(256) void k_conv_mid_bwdw(const __bf16* __restrict_
 }
 
 // ---- layer 6 (32 -> 1), vector ALU, FP32 weights ------------------------------------------
-__global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
+__global__ __launch_bounds__(256) void k_conv6(const _Float16* __restrict__ a5,
                                                const float* __restrict__ w6,
                                                const float* __restrict__ b6,
                                                float* __restrict__ out, int H, int W, int B) {
@@ -695,7 +727,7 @@ __global__ __launch_bounds__(256) void k_conv6(const __bf16* __restrict__ a5,
     const size_t n = img + (size_t)wrap1(y + tp / 3 - 1, H) * W + wrap1(x + tp % 3 - 1, W);
 #pragma unroll
     for (int ch = 0; ch < kC; ch += 8) {
-      const bf16x8 v = *(const bf16x8*)(a5 + n * kC + ch);
+      const f16x8 v = *(const f16x8*)(a5 + n * kC + ch);
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc[ch / 16] = fmaf(w6[tp * kC + ch + u], (float)v[u], acc[ch / 16]);
     }
@@ -760,7 +792,7 @@ __global__ __launch_bounds__(256) void k_conv6_bwdd(const float* __restrict__ do
 // Backward-weights of layer 6: thread e < 288 owns W6[tap = e/32][ci = e%32], e = 288 owns
 // b6.  A block sums over a contiguous range of sites.
 __global__ __launch_bounds__(320) void k_conv6_bwdw(const float* __restrict__ dout,
-                                                    const __bf16* __restrict__ a5,
+                                                    const _Float16* __restrict__ a5,
                                                     float* __restrict__ part, int H, int W,
                                                     long nsites, long per_block) {
   const int e = threadIdx.x;
@@ -810,14 +842,118 @@ __global__ __launch_bounds__(320) void k_conv1_bwdw(const __bf16* __restrict__ d
   part[(size_t)blockIdx.x * 320 + e] = acc;
 }
 
-// out[i] = sum over p of part[p][i], p in order (deterministic).
+// out[i] = sum over p of part[p][i], p in order (deterministic); with accumulate, added to
+// out[i] instead of replacing it.
 __global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, int nparts,
-                                                      int width, float* __restrict__ out) {
+                                                      int width, float* __restrict__ out,
+                                                      int accumulate) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= width) return;
   float s = 0.0f;
   for (int p = 0; p < nparts; ++p) s += part[(size_t)p * width + i];
-  out[i] = s;
+  out[i] = accumulate ? out[i] + s : s;
+}
+
+// ---- unrolled-chain training (several residual applications, gradient through all) -----
+
+// Input gradient of layer 1, added to dx: dx[s] += sum_tap sum_co W1[co][tap] dz1[s - off(tap)][co]
+// (the forward pass reads x[s + off(tap)]).
+__global__ __launch_bounds__(256) void k_conv1_bwdd(const __bf16* __restrict__ dz1,
+                                                    const float* __restrict__ w1,
+                                                    float* __restrict__ dx, int H, int W, int B) {
+  const int HW = H * W;
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= (long)B * HW) return;
+  const int b = (int)(s / HW), p = (int)(s % HW), y = p / W, x = p % W;
+  const size_t img = (size_t)b * HW;
+  float acc = 0.0f;
+#pragma unroll
+  for (int tp = 0; tp < 9; ++tp) {
+    const size_t n = img + (size_t)wrap1(y - (tp / 3 - 1), H) * W + wrap1(x - (tp % 3 - 1), W);
+#pragma unroll
+    for (int ch = 0; ch < kC; ch += 8) {
+      const bf16x8 v = *(const bf16x8*)(dz1 + n * kC + ch);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = fmaf(w1[(ch + u) * 9 + tp], (float)v[u], acc);
+    }
+  }
+  dx[s] += acc;
+}
+
+// sums[b] = sum of image b's HW values in double (block b; fixed order).
+__global__ __launch_bounds__(1024) void k_img_sum(const float* __restrict__ v, long HW,
+                                                  double* __restrict__ sums) {
+  __shared__ double sh[16];
+  const float* img = v + (size_t)blockIdx.x * HW;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < HW; i += 1024) s += (double)img[i];
+  for (int st = 32; st >= 1; st >>= 1) s += __shfl_xor(s, st);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    sums[blockIdx.x] = t;
+  }
+}
+
+// c_new = c + scale (out - mean(out)) with image b's mean sums[b] / HW (FP32 state).
+__global__ __launch_bounds__(256) void k_chain_step(const float* __restrict__ c,
+                                                    const float* __restrict__ out,
+                                                    const double* __restrict__ sums,
+                                                    double scale, long HW, long n,
+                                                    float* __restrict__ cn) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const double mean = sums[s / HW] / (double)HW;
+  cn[s] = (float)((double)c[s] + scale * ((double)out[s] - mean));
+}
+
+// Output gradient of one application from the state gradient G: dout = scale (G - mean(G)).
+__global__ __launch_bounds__(256) void k_chain_dout(const float* __restrict__ G,
+                                                    const double* __restrict__ sums,
+                                                    double scale, long HW, long n,
+                                                    float* __restrict__ dout) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const double mean = sums[s / HW] / (double)HW;
+  dout[s] = (float)(scale * ((double)G[s] - mean));
+}
+
+// Loss term of the state after k applications: e = (c_k - t) / scale with t = snapshot
+// idx[b] + k of data; G = (accumulate ? G : 0) + coef e; part[block] = sum of w e^2.
+template <class TS>
+__global__ __launch_bounds__(256) void k_chain_err(const float* __restrict__ ck,
+                                                   const TS* __restrict__ data,
+                                                   const int* __restrict__ idx, int k,
+                                                   double inv_scale, double w, double coef,
+                                                   int accumulate, float* __restrict__ G,
+                                                   double* __restrict__ part, long HW, long n) {
+  __shared__ double sh[4];
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  double e2 = 0.0;
+  if (s < n) {
+    const long b = s / HW, p = s % HW;
+    const double t = (double)data[(size_t)(idx[b] + k) * HW + p];
+    const double e = ((double)ck[s] - t) * inv_scale;
+    const float ge = (float)(coef * e);
+    G[s] = accumulate ? G[s] + ge : ge;
+    e2 = w * e * e;
+  }
+  for (int st = 32; st >= 1; st >>= 1) e2 += __shfl_xor(e2, st);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// c0[b] = data[idx[b]] in FP32.
+template <class TS>
+__global__ __launch_bounds__(256) void k_gather(const TS* __restrict__ data,
+                                                const int* __restrict__ idx, long HW, long n,
+                                                float* __restrict__ c0) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  c0[s] = (float)data[(size_t)idx[s / HW] * HW + s % HW];
 }
 
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
@@ -834,25 +970,23 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float
   p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
-// FP32 master parameters -> BF16 working copies of the 32 -> 32 matrices [co][k] and
-// their flipped transposes [ci][tap'*32 + co] = W[co][(8 - tap')*32 + ci] (backward-data).
-// W1 and W6 are used in FP32 directly.
-// Weights of the four 32 -> 32 layers in BF16, as the matrix-core A operands: forward
-// (wmid, rows co, columns tap * 32 + ci) and backward-data (wmidT, rows ci, columns
-// tap' * 32 + co with tap' = 8 - tap).  Stored fragment-major: for K block kc (16 columns),
+// FP32 master parameters -> 16-bit working copies of the 32 -> 32 matrices, as the
+// matrix-core A operands: forward in FP16 (wmid, rows co, columns tap * 32 + ci) and
+// backward-data in BF16 (wmidT, rows ci, columns tap' * 32 + co with tap' = 8 - tap, i.e.
+// [ci][tap'*32 + co] = W[co][(8 - tap')*32 + ci]).  W1 and W6 are used in FP32 directly.  Stored fragment-major: for K block kc (16 columns),
 // lane (h, j) of a wave holds row j, columns kc * 16 + 8 h .. + 7, at
 // ((kc * 64 + 32 h + j) * 8), so each of a wave's 18 fragment loads reads 1 KiB contiguously.
 __device__ __forceinline__ int frag_index(int row, int col) {
   return ((col / 16) * 64 + ((col % 16) / 8) * 32 + row) * 8 + col % 8;
 }
-__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, __bf16* __restrict__ wmid,
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p, _Float16* __restrict__ wmid,
                                               __bf16* __restrict__ wmidT) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < 4 * kMidW) {
     const int l = i / kMidW, e = i % kMidW;
     const float* W = p + kOffMid + l * kMidStride;
     const int r = e / 288, k = e % 288;
-    wmid[l * kMidW + frag_index(r, k)] = (__bf16)W[e];
+    wmid[l * kMidW + frag_index(r, k)] = (_Float16)W[e];
     const int tpp = k / 32, co = k % 32;  // wmidT row r = ci, column k = tap' * 32 + co
     wmidT[l * kMidW + frag_index(r, k)] = (__bf16)W[co * 288 + (8 - tpp) * 32 + r];
   }
@@ -889,9 +1023,10 @@ struct gc_cnn {
   int H = 0, W = 0, Bmax = 0;
   double scale = 1.0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  __bf16 *wmid = nullptr, *wmidT = nullptr;
+  _Float16* wmid = nullptr;  // forward weights of the 32 -> 32 layers (FP16)
+  __bf16* wmidT = nullptr;   // backward-data weights (BF16)
   float* z[5] = {};
-  __bf16* a[5] = {};
+  _Float16* a[5] = {};       // activations (FP16)
   __bf16 *dzA = nullptr, *dzB = nullptr;
   float *out = nullptr, *dout = nullptr;
   float* part = nullptr;
@@ -905,6 +1040,11 @@ struct gc_cnn {
   double* hdiag = nullptr;  // learned integration: pinned copies of two chunks' diagnostics
   hipEvent_t ev[2] = {};    // learned integration: each chunk's diagnostics copied
   double* cpart = nullptr;  // learned integration: second tile sums, then reduction partials
+  float* chain_c = nullptr;    // chain training: states c_0 .. c_K of the batch (FP32)
+  float* chain_g = nullptr;    // chain training: gradient of the loss with respect to c_k
+  double* chain_sum = nullptr; // chain training: per-image sums
+  double* chain_part = nullptr;// chain training: loss partial sums, one row per step
+  int chain_k = 0;             // chain training: steps the buffers hold
   int adam_t = 0;
   bool packed = false;
 };
@@ -1004,9 +1144,12 @@ gc_status forward_chain(gc_cnn* n, const Chain<TS>& ch, float* out, double* part
   return GC_OK;
 }
 
-// Backward pass from dout; gradients into n->g.
+// Backward pass from dout; gradients into n->g (added to it with accumulate); with dx, the
+// gradient with respect to the network input is added to dx.
 template <class TS>
-gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
+gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B, bool accumulate = false,
+                   float* dx = nullptr) {
+  const int acc = accumulate ? 1 : 0;
   hipStream_t st = n->ctx->stream;
   const long ns = (long)B * n->H * n->W;
   const unsigned g1 = (unsigned)((ns + 255) / 256);
@@ -1016,7 +1159,7 @@ gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
                      n->H, n->W, ns, per);
   GC_LAUNCHED();
   hipLaunchKernelGGL(k_reduce_parts, dim3((289 + 255) / 256), dim3(256), 0, st, n->part,
-                     kPartBlocks, 289, n->g + kOffW6);
+                     kPartBlocks, 289, n->g + kOffW6, acc);
   GC_LAUNCHED();
   hipLaunchKernelGGL(k_conv6_bwdd, dim3(g1), dim3(256), 0, st, n->dout, n->p + kOffW6, n->z[4], n->dzA,
                      n->H, n->W, B);
@@ -1029,7 +1172,7 @@ gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
                        n->part, n->H, n->W, tx, ty, ntiles);
     GC_LAUNCHED();
     hipLaunchKernelGGL(k_reduce_parts, dim3((kMidStride + 255) / 256), dim3(256), 0, st, n->part,
-                       kPartBlocks, kMidStride, n->g + kOffMid + l * kMidStride);
+                       kPartBlocks, kMidStride, n->g + kOffMid + l * kMidStride, acc);
     GC_LAUNCHED();
     hipLaunchKernelGGL((k_conv_mid<1>), dim3((unsigned)ntiles), dim3(256), 0, st, dz,
                        n->wmidT + (size_t)l * kMidW, nullptr, n->z[l], nullptr, dzn, n->H, n->W,
@@ -1043,8 +1186,13 @@ gc_status backward(gc_cnn* n, const TS* c, const int* d_idx, int B) {
                      n->part, n->H, n->W, ns, per);
   GC_LAUNCHED();
   hipLaunchKernelGGL(k_reduce_parts, dim3((320 + 255) / 256), dim3(256), 0, st, n->part,
-                     kPartBlocks, 320, n->g + kOffW1);
+                     kPartBlocks, 320, n->g + kOffW1, acc);
   GC_LAUNCHED();
+  if (dx) {
+    hipLaunchKernelGGL(k_conv1_bwdd, dim3(g1), dim3(256), 0, st, dz, n->p + kOffW1, dx, n->H,
+                       n->W, B);
+    GC_LAUNCHED();
+  }
   return GC_OK;
 }
 
@@ -1064,6 +1212,80 @@ gc_status loss_and_dout(gc_cnn* n, const TS* c, const TS* target, const int* d_i
     double s = 0.0;
     for (double x : h) s += x;
     *loss = s / (double)ns;
+  }
+  return GC_OK;
+}
+
+constexpr int kChainMax = 8;  // residual applications one chain evaluation may unroll
+
+gc_status ensure_chain(gc_cnn* n, int K) {
+  if (n->chain_k >= K) return GC_OK;
+  GC_HIP(hipStreamSynchronize(n->ctx->stream));
+  void* old[] = {n->chain_c, n->chain_g, n->chain_sum, n->chain_part};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  n->chain_c = n->chain_g = nullptr;
+  n->chain_sum = n->chain_part = nullptr;
+  n->chain_k = 0;
+  const size_t ns = (size_t)n->Bmax * n->H * n->W, g1 = (ns + 255) / 256;
+  GC_TRY(cnn_alloc((void**)&n->chain_c, (size_t)(K + 1) * ns * sizeof(float)));
+  GC_TRY(cnn_alloc((void**)&n->chain_g, ns * sizeof(float)));
+  GC_TRY(cnn_alloc((void**)&n->chain_sum, (size_t)n->Bmax * sizeof(double)));
+  GC_TRY(cnn_alloc((void**)&n->chain_part, (size_t)K * g1 * sizeof(double)));
+  n->chain_k = K;
+  return GC_OK;
+}
+
+// The chain's states c_0..c_K by the per-layer kernels (so that the recomputed training
+// forward pass of each application sees the same bits), then the applications in reverse:
+// output gradient scale (G - mean G), activations recomputed, weight gradients accumulated,
+// G carried to the previous state through the residual path and the input gradient.
+template <class TS>
+gc_status chain_loss_grad(gc_cnn* n, const TS* data, const int* d_idx, int B, int K,
+                          const double* w, double* loss) {
+  hipStream_t st = n->ctx->stream;
+  const long HW = (long)n->H * n->W, ns = (long)B * HW;
+  const unsigned g1 = (unsigned)((ns + 255) / 256);
+  const size_t stride = (size_t)n->Bmax * HW;
+  auto state = [&](int k) { return n->chain_c + (size_t)k * stride; };
+  const double inv_s = 1.0 / n->scale, N = (double)ns;
+  hipLaunchKernelGGL((k_gather<TS>), dim3(g1), dim3(256), 0, st, data, d_idx, HW, ns, state(0));
+  GC_LAUNCHED();
+  for (int k = 1; k <= K; ++k) {
+    GC_TRY(forward<float>(n, state(k - 1), nullptr, B, false));
+    hipLaunchKernelGGL(k_img_sum, dim3(B), dim3(1024), 0, st, n->out, HW, n->chain_sum);
+    GC_LAUNCHED();
+    hipLaunchKernelGGL(k_chain_step, dim3(g1), dim3(256), 0, st, state(k - 1), n->out,
+                       n->chain_sum, n->scale, HW, ns, state(k));
+    GC_LAUNCHED();
+  }
+  hipLaunchKernelGGL((k_chain_err<TS>), dim3(g1), dim3(256), 0, st, state(K), data, d_idx, K,
+                     inv_s, w[K - 1], 2.0 * w[K - 1] * inv_s / N, 0, n->chain_g,
+                     n->chain_part + (size_t)(K - 1) * g1, HW, ns);
+  GC_LAUNCHED();
+  for (int k = K; k >= 1; --k) {
+    hipLaunchKernelGGL(k_img_sum, dim3(B), dim3(1024), 0, st, n->chain_g, HW, n->chain_sum);
+    GC_LAUNCHED();
+    hipLaunchKernelGGL(k_chain_dout, dim3(g1), dim3(256), 0, st, n->chain_g, n->chain_sum,
+                       n->scale, HW, ns, n->dout);
+    GC_LAUNCHED();
+    GC_TRY(forward<float>(n, state(k - 1), nullptr, B, true));
+    GC_TRY(backward<float>(n, state(k - 1), nullptr, B, k < K, k > 1 ? n->chain_g : nullptr));
+    if (k > 1) {
+      hipLaunchKernelGGL((k_chain_err<TS>), dim3(g1), dim3(256), 0, st, state(k - 1), data, d_idx,
+                         k - 1, inv_s, w[k - 2], 2.0 * w[k - 2] * inv_s / N, 1, n->chain_g,
+                         n->chain_part + (size_t)(k - 2) * g1, HW, ns);
+      GC_LAUNCHED();
+    }
+  }
+  if (loss) {
+    std::vector<double> h((size_t)K * g1);
+    GC_HIP(hipMemcpyAsync(h.data(), n->chain_part, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    GC_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double x : h) s += x;
+    *loss = s / N;
   }
   return GC_OK;
 }
@@ -1128,7 +1350,7 @@ gc_status gc_cnn_destroy(gc_cnn* n) {
   if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->wmid, n->wmidT, n->dzA, n->dzB,
                   n->out, n->dout, n->part, n->dpart, n->idx, n->done, n->hist, n->diag,
-                  n->out2, n->cpart};
+                  n->out2, n->cpart, n->chain_c, n->chain_g, n->chain_sum, n->chain_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (n->hdiag) (void)hipHostFree(n->hdiag);
@@ -1216,6 +1438,31 @@ gc_status gc_cnn_loss_grad(gc_cnn* n, gc_dtype dt, const void* c, const void* ta
   GC_TRY(forward<float>(n, (const float*)c, d_idx, B, true));
   GC_TRY(loss_and_dout<float>(n, (const float*)c, (const float*)target, d_idx, B, loss));
   return backward<float>(n, (const float*)c, d_idx, B);
+}
+
+gc_status gc_cnn_loss_grad_chain(gc_cnn* n, gc_dtype dt, const void* data, size_t n_images,
+                                 const int* idx, int B, int K, const double* weights,
+                                 double* loss) {
+  GC_TRY(check_cnn(n, B));
+  GC_TRY(check_dtype(dt));
+  if (!data || !idx) return set_error(GC_ERR_INVALID, "gridcalc: null data or chain indices");
+  if (K < 1 || K > kChainMax)
+    return set_error(GC_ERR_INVALID, "gridcalc: chain of %d applications (1 to %d)", K, kChainMax);
+  for (int b = 0; b < B; ++b)
+    if (idx[b] < 0 || (size_t)idx[b] + (size_t)K >= n_images)
+      return set_error(GC_ERR_INVALID,
+                       "gridcalc: chain %d starts at image %d; its %d targets pass image %zu", b,
+                       idx[b], K, n_images);
+  double w[kChainMax];
+  for (int k = 0; k < K; ++k) {
+    w[k] = weights ? weights[k] : 1.0 / K;
+    if (!(w[k] >= 0.0) || !std::isfinite(w[k]))
+      return set_error(GC_ERR_INVALID, "gridcalc: chain step weight %d is not finite and >= 0", k);
+  }
+  GC_TRY(ensure_chain(n, K));
+  GC_TRY(upload_idx(n, idx, B));
+  if (dt == GC_F64) return chain_loss_grad<double>(n, (const double*)data, n->idx, B, K, w, loss);
+  return chain_loss_grad<float>(n, (const float*)data, n->idx, B, K, w, loss);
 }
 
 gc_status gc_cnn_adam(gc_cnn* n, double lr, double beta1, double beta2, double eps) {

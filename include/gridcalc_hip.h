@@ -366,7 +366,8 @@ gc_status gc_cnn_destroy(gc_cnn* n);
 /* Host FP32 parameters and the output scale; resets the optimiser state. */
 gc_status gc_cnn_set_params(gc_cnn* n, const float* host, double scale);
 gc_status gc_cnn_get_params(gc_cnn* n, float* host);
-/* Gradient of the last gc_cnn_loss_grad call (host FP32, param-count values). */
+/* Gradient of the last gc_cnn_loss_grad or gc_cnn_loss_grad_chain call (host FP32,
+ * param-count values). */
 gc_status gc_cnn_get_grads(gc_cnn* n, float* host);
 /* Network output `out` (before scale and mean removal) for B device images of dtype dt;
  * out_dev (device FP32, B*H*W) may be NULL.  One kernel runs all six layers per 8 x 32
@@ -380,6 +381,19 @@ gc_status gc_cnn_forward_layers(gc_cnn* n, gc_dtype dt, const void* c, int B, fl
  * loss is not NULL it receives the loss (synchronous). */
 gc_status gc_cnn_loss_grad(gc_cnn* n, gc_dtype dt, const void* c, const void* target,
                            const int* idx, int B, double* loss);
+/* Unrolled-chain training evaluation (back-propagation through K residual applications).
+ * data is a device array of n_images images of dtype dt; idx (host, B entries) names the
+ * starting image of each chain, whose targets after k = 1..K applications are images
+ * idx[b] + k (so idx[b] + K < n_images).  With c_0 = data[idx[b]] and
+ * c_k = c_{k-1} + scale (out(c_{k-1}) - mean(out(c_{k-1}))) (per image, FP32 states), the loss
+ * is sum_k weights[k-1] mean(((c_k - data[idx + k]) / scale)^2) (weights NULL: 1/K each);
+ * its gradient with respect to the parameters, through every application, goes to the
+ * parameter gradient buffer.  1 <= K <= 8.  If loss is not NULL it receives the loss
+ * (synchronous).  The network's chain buffers are allocated on the first call (and again
+ * for a larger K). */
+gc_status gc_cnn_loss_grad_chain(gc_cnn* n, gc_dtype dt, const void* data, size_t n_images,
+                                 const int* idx, int B, int K, const double* weights,
+                                 double* loss);
 /* Adam update of the parameters from the current gradient. */
 gc_status gc_cnn_adam(gc_cnn* n, double lr, double beta1, double beta2, double eps);
 /* c_new = c + scale*(out - mean(out)) for B device images (c_new may not alias c). */

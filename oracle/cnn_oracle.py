@@ -72,8 +72,8 @@ def shift(x, dy, dx):
 def forward(p, x, keep=False, q=None, gelu_fn=None):
     """x: (B, H, W) network input.  Returns out (B, H, W) and, with keep, the cache.
     q, if given, is applied to the weights of the four 32 -> 32 layers and to every
-    activation (e.g. bf16_round, to model a device that stores those in bfloat16 and keeps
-    the input, W1 and W6 in float32); gelu_fn replaces gelu (e.g. gelu_fit)."""
+    activation (e.g. fp16_round, to model a device that stores those in half precision and
+    keeps the input, W1 and W6 in float32); gelu_fn replaces gelu (e.g. gelu_fit)."""
     gelu_ = gelu_fn if gelu_fn is not None else gelu
     W1, b1, mids, W6, b6 = split(p)
     if q is not None:
@@ -101,13 +101,17 @@ def forward(p, x, keep=False, q=None, gelu_fn=None):
     return out
 
 
-def backward(p, cache, dout, q=None):
-    """Gradient of sum(dout * out) with respect to the flat parameters (q as in forward;
-    applied here to the weights and to every pre-activation gradient)."""
+def backward(p, cache, dout, q=None, want_input=False, qa=None):
+    """Gradient of sum(dout * out) with respect to the flat parameters.  q, if given, is
+    applied to the weights of the four 32 -> 32 layers and to every pre-activation gradient,
+    qa to the cached activations that enter those four layers' weight gradients (e.g. q = qa
+    = bf16_round for a device whose backward pass runs in bfloat16).  With want_input also
+    the gradient with respect to the network input x, returned second."""
     W1, b1, mids, W6, b6 = split(p)
     if q is not None:
         mids = [(q(W), b) for W, b in mids]
     qg = q if q is not None else (lambda a: a)
+    qa_ = qa if qa is not None else (lambda a: a)
     x, zs, acts = cache
     g = []
     # last layer
@@ -122,7 +126,7 @@ def backward(p, cache, dout, q=None):
     for li in range(3, -1, -1):
         W, b = mids[li]
         dz = qg(da * gelu_grad(zs[li + 1]))
-        a_in = acts[li]
+        a_in = qa_(acts[li])
         dW = np.zeros_like(W)
         da = np.zeros_like(a_in)
         for t, (dy, dx) in enumerate(TAPS):
@@ -134,6 +138,10 @@ def backward(p, cache, dout, q=None):
     for t, (dy, dx) in enumerate(TAPS):
         dW1[:, t] = np.einsum("bhwc,bhw->c", dz1, shift(x, dy, dx))
     db1 = dz1.sum(axis=(0, 1, 2))
+    if want_input:
+        dx = np.zeros_like(x)
+        for t, (dy, dx_) in enumerate(TAPS):
+            dx += shift(dz1, -dy, -dx_) @ W1[:, t]
     g.append(dW1.ravel())
     g.append(db1)
     for dW, db in mid_grads:
@@ -141,7 +149,49 @@ def backward(p, cache, dout, q=None):
         g.append(db)
     g.append(dW6.ravel())
     g.append(db6)
+    if want_input:
+        return np.concatenate(g), dx
     return np.concatenate(g)
+
+
+def chain(p, x0, K, scale, q=None, gelu_fn=None):
+    """K applications of the residual update c_k = c_{k-1} + scale (out(c_{k-1}) - mean(out)),
+    the mean taken per image: returns [c_0, ..., c_K] and the K forward caches."""
+    cs, caches = [np.asarray(x0, dtype=np.float64)], []
+    for _ in range(K):
+        out, cache = forward(p, cs[-1], keep=True, q=q, gelu_fn=gelu_fn)
+        caches.append(cache)
+        cs.append(cs[-1] + scale * (out - out.mean(axis=(1, 2), keepdims=True)))
+    return cs, caches
+
+
+def chain_loss(p, x0, targets, scale, weights, q=None, gelu_fn=None):
+    """L = sum_k weights[k-1] mean(((c_k - targets[k-1]) / scale)^2), k = 1..K, the mean over
+    all B H W sites; targets (K, B, H, W)."""
+    cs, _ = chain(p, x0, len(targets), scale, q, gelu_fn)
+    return sum(w * np.mean(((c - t) / scale) ** 2) for w, c, t in zip(weights, cs[1:], targets))
+
+
+def chain_loss_grad(p, x0, targets, scale, weights, q=None, gelu_fn=None, qb=None, qa=None):
+    """chain_loss and its gradient with respect to the flat parameters, through every
+    application (back-propagation through the unrolled chain).  With G_k = dL/dc_k:
+    G_K = 2 w_K (c_K - t_K) / (scale^2 N); application k's output gradient is
+    scale (G_k - mean(G_k)) (the mean removal is a projection); G_{k-1} = G_k (the residual
+    path) + the network's input gradient + 2 w_{k-1} (c_{k-1} - t_{k-1}) / (scale^2 N).
+    q and gelu_fn as in forward; qb and qa as backward's q and qa."""
+    K = len(targets)
+    cs, caches = chain(p, x0, K, scale, q, gelu_fn)
+    N = cs[0].size
+    L = sum(w * np.mean(((c - t) / scale) ** 2) for w, c, t in zip(weights, cs[1:], targets))
+    G = 2.0 * weights[K - 1] * (cs[K] - targets[K - 1]) / (scale * scale * N)
+    grad = np.zeros(N_PARAMS)
+    for k in range(K, 0, -1):
+        dout = scale * (G - G.mean(axis=(1, 2), keepdims=True))
+        gk, dx = backward(p, caches[k - 1], dout, q=qb, want_input=True, qa=qa)
+        grad += gk
+        if k > 1:
+            G = G + dx + 2.0 * weights[k - 2] * (cs[k - 1] - targets[k - 2]) / (scale * scale * N)
+    return L, grad
 
 
 def init_params(seed=0):
@@ -156,6 +206,11 @@ def init_params(seed=0):
     parts.append(rng.uniform(-1, 1, 288) * math.sqrt(6.0 / 288.0) * 0.1)
     parts.append(np.zeros(1))
     return np.concatenate(parts)
+
+
+def fp16_round(a):
+    """Round-to-nearest-even to IEEE half precision, returned as float64."""
+    return np.asarray(a, dtype=np.float64).astype(np.float16).astype(np.float64)
 
 
 def bf16_round(a):

@@ -66,6 +66,49 @@ def test_gradients_match_finite_differences(cnn):
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_input_gradient_matches_finite_differences(cnn):
+    p = cnn.init_params(6)
+    p[-289:-1] *= 10.0
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, size=(2, 5, 6))
+    w = rng.normal(size=(2, 5, 6))
+    _, cache = cnn.forward(p, x, keep=True)
+    _, dx = cnn.backward(p, cache, w, want_input=True)
+    eps = 1e-6
+    for b, y, xx in [(0, 0, 0), (0, 4, 5), (1, 2, 3), (1, 0, 5), (0, 3, 1)]:
+        xp, xm = x.copy(), x.copy()
+        xp[b, y, xx] += eps
+        xm[b, y, xx] -= eps
+        fd = ((w * cnn.forward(p, xp)).sum() - (w * cnn.forward(p, xm)).sum()) / (2 * eps)
+        assert abs(fd - dx[b, y, xx]) <= 1e-6 * max(1.0, abs(fd)), (b, y, xx, fd, dx[b, y, xx])
+
+
+def test_chain_gradient_matches_finite_differences(cnn):
+    """Back-propagation through three residual applications (per-image mean removal, loss
+    on every step with unequal weights) against central differences of the chain loss."""
+    p = cnn.init_params(8)
+    p[-289:-1] *= 10.0
+    p[-1] = 0.3  # a mean the update must remove
+    rng = np.random.default_rng(4)
+    x0 = rng.uniform(-1, 1, size=(2, 5, 6))
+    targets = x0 + 0.05 * rng.normal(size=(3, 2, 5, 6)).cumsum(axis=0)
+    scale, weights = 0.3, (0.5, 0.2, 1.0)
+    L, g = cnn.chain_loss_grad(p, x0, targets, scale, weights)
+    assert abs(L - cnn.chain_loss(p, x0, targets, scale, weights)) <= 1e-14 * L
+    idx = list(range(0, 288, 41)) + [290, 320 + 11, 320 + 9216 + 3, 2 * 9248 + 320 + 500,
+                                    4 * 9248 + 320 + 9, cnn.N_PARAMS - 2]
+    eps = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (cnn.chain_loss(pp, x0, targets, scale, weights)
+              - cnn.chain_loss(pm, x0, targets, scale, weights)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+    # the output bias only shifts every image's mean, which the update removes
+    assert abs(g[-1]) <= 1e-12 * np.abs(g).max()
+
+
 def test_gelu_fit_error_bound(cnn):
     """The device's GELU form against z Phi(z): the stated maximum error, the limits z and 0
     far out, and z / 2 slope at the origin (a wrong coefficient or sign breaks one)."""
@@ -91,3 +134,32 @@ def test_bf16_round(cnn):
     assert cnn.bf16_round(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
     assert cnn.bf16_round(-3.140625) == -3.140625
     assert math.isclose(cnn.bf16_round(0.1), 0.10009765625)
+
+
+def test_fp16_round(cnn):
+    assert cnn.fp16_round(1.0 + 2 ** -11) == 1.0          # ties to even
+    assert cnn.fp16_round(1.0 + 3 * 2 ** -11) == 1.0 + 2 ** -9
+    assert cnn.fp16_round(65504.0) == 65504.0              # largest finite half
+    assert cnn.fp16_round(2.0 ** -24) == 2.0 ** -24        # smallest subnormal
+    assert cnn.fp16_round(2.0 ** -26) == 0.0
+    assert math.isclose(cnn.fp16_round(0.1), 0.0999755859375)
+
+
+def test_backward_activation_rounding_applies_to_mid_weight_gradients_only(cnn):
+    """qa rounds the activations entering the four 32 -> 32 weight gradients; the first and
+    last layers' gradients and the input gradient do not see it."""
+    p = cnn.init_params(9)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(1, 4, 5))
+    w = rng.normal(size=(1, 4, 5))
+    _, cache = cnn.forward(p, x, keep=True)
+    g0, dx0 = cnn.backward(p, cache, w, want_input=True)
+    g1, dx1 = cnn.backward(p, cache, w, want_input=True, qa=lambda a: 2.0 * a)
+    mid = slice(320, 320 + 4 * 9248)
+    assert np.array_equal(g0[:320], g1[:320]) and np.array_equal(g0[-289:], g1[-289:])
+    assert np.array_equal(dx0, dx1)
+    for l in range(4):  # weights doubled, biases unchanged
+        o = 320 + l * 9248
+        assert np.allclose(g1[o:o + 9216], 2.0 * g0[o:o + 9216], rtol=1e-12, atol=0)
+        assert np.array_equal(g1[o + 9216:o + 9248], g0[o + 9216:o + 9248])
+    assert not np.array_equal(g0[mid], g1[mid])

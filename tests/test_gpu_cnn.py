@@ -1,7 +1,9 @@
 """GPU kernels of the learned corrector against the FP64 reference network
-(oracle/cnn_oracle.py) evaluated with the same bfloat16 roundings (32 -> 32 weights,
-activations, pre-activation gradients) and the device's GELU form (gelu_fit); the remaining
-differences are FP32 accumulation order and single-precision GELU arithmetic."""
+(oracle/cnn_oracle.py) evaluated with the device's roundings and GELU form (gelu_fit): half
+precision for the 32 -> 32 layers' weights and activations in the forward pass; bfloat16 for
+those weights, the pre-activation gradients and the activations entering the weight
+gradients in the backward pass.  The remaining differences are FP32 accumulation order and
+single-precision GELU arithmetic."""
 import importlib.util
 import os
 
@@ -42,7 +44,7 @@ def test_forward_matches_reference(gc, cnn, ctx, dtype):
     net.set_params(p, 0.1)
     d_c = gc.DeviceArray.from_numpy(ctx, c.astype(dtype))
     got = net.forward(d_c, B).astype(np.float64)
-    want = cnn.forward(p, c.astype(dtype).astype(np.float64), q=cnn.bf16_round,
+    want = cnn.forward(p, c.astype(dtype).astype(np.float64), q=cnn.fp16_round,
                        gelu_fn=cnn.gelu_fit)
     assert _rel(got, want) < 3e-3, _rel(got, want)
     net.close()
@@ -61,10 +63,10 @@ def test_gradients_match_reference(gc, cnn, ctx):
     d_t = gc.DeviceArray.from_numpy(ctx, target)
     loss = net.loss_grad(d_c, d_t, B=B)
     g = net.grads().astype(np.float64)
-    out, cache = cnn.forward(p, c, keep=True, q=cnn.bf16_round, gelu_fn=cnn.gelu_fit)
+    out, cache = cnn.forward(p, c, keep=True, q=cnn.fp16_round, gelu_fn=cnn.gelu_fit)
     d = out - (target - c) / scale
     assert abs(loss - np.mean(d * d)) < 3e-3 * np.mean(d * d)
-    want = cnn.backward(p, cache, 2 * d / d.size, q=cnn.bf16_round)
+    want = cnn.backward(p, cache, 2 * d / d.size, q=cnn.bf16_round, qa=cnn.bf16_round)
     # per parameter block: W1 b1 (W b)x4 W6 b6
     bounds = [0, 288, 320]
     for l in range(4):
@@ -72,6 +74,55 @@ def test_gradients_match_reference(gc, cnn, ctx):
     bounds += [bounds[-1] + 288, bounds[-1] + 289]
     for lo, hi in zip(bounds, bounds[1:]):
         assert _rel(g[lo:hi], want[lo:hi]) < 2e-2, (lo, hi, _rel(g[lo:hi], want[lo:hi]))
+    net.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_chain_gradients_match_reference(gc, cnn, ctx, dtype):
+    """gc_cnn_loss_grad_chain through three residual applications against the oracle's
+    back-propagation through the same chain (same bfloat16 roundings and GELU form): chains
+    start at images 1 and 3 of a 7-image array, targets are the three images after each."""
+    B, H, W, K = 2, 16, 64, 3
+    p = _params(cnn, 17)
+    rng = np.random.default_rng(5)
+    data = (rng.uniform(-1, 1, (1, H, W)) + 0.05 * rng.normal(size=(7, H, W)).cumsum(0)).astype(dtype)
+    idx = np.array([1, 3])
+    scale, weights = 0.1, (0.25, 0.25, 0.5)
+    net = gc.Corrector(ctx, H, W, B)
+    net.set_params(p, scale)
+    d = gc.DeviceArray.from_numpy(ctx, data)
+    loss = net.chain_loss_grad(d, idx, K, weights)
+    g = net.grads().astype(np.float64)
+    x0 = data[idx].astype(np.float64)
+    targets = np.stack([data[idx + k] for k in range(1, K + 1)]).astype(np.float64)
+    L, want = cnn.chain_loss_grad(p, x0, targets, scale, weights, q=cnn.fp16_round,
+                                  gelu_fn=cnn.gelu_fit, qb=cnn.bf16_round, qa=cnn.bf16_round)
+    assert abs(loss - L) < 3e-3 * L, (loss, L)
+    bounds = [0, 288, 320]
+    for l in range(4):
+        bounds += [bounds[-1] + 9216, bounds[-1] + 9216 + 32]
+    bounds += [bounds[-1] + 288]
+    for lo, hi in zip(bounds, bounds[1:]):
+        assert _rel(g[lo:hi], want[lo:hi]) < 2e-2, (lo, hi, _rel(g[lo:hi], want[lo:hi]))
+    # the output bias only shifts each image's mean, which every application removes
+    assert abs(g[-1]) <= 1e-4 * np.abs(g).max()
+    d.free()
+    net.close()
+
+
+def test_chain_rejects_bad_arguments(gc, cnn, ctx):
+    H, W = 8, 32
+    net = gc.Corrector(ctx, H, W, 2)
+    net.set_params(_params(cnn, 3), 0.1)
+    d = gc.DeviceArray.from_numpy(ctx, np.zeros((5, H, W), np.float32))
+    for idx, K, w in [([1, 2], 3, None),   # chain from image 2 needs images 3..5
+                      ([0, 1], 0, None), ([0, 0], 9, None), ([-1, 0], 1, None),
+                      ([0, 1], 2, (1.0, -0.5)), ([0, 1], 2, (1.0, np.nan))]:
+        with pytest.raises(gc.GridcalcError):
+            net.chain_loss_grad(d, idx, K, w)
+    # a constant state gives a constant output, which the mean removal cancels exactly
+    assert net.chain_loss_grad(d, [0, 1], 3, None) == 0.0
+    d.free()
     net.close()
 
 
@@ -157,6 +208,18 @@ def test_finetune_with_pushforward_runs(gc, ctx):
     cfg = dict(cr.CONFIG, batch_traj=4, n_steps=64)
     p0 = cr.init_params(5)
     p, hist = cr.finetune(p0, 0.01, 0, 3, ((2, 4, 1e-4, 1e-5, 11, 2),), cfg, log=lambda *a: None)
+    assert p.shape == p0.shape and np.isfinite(p).all()
+    assert not np.array_equal(p, p0)
+    assert len(hist) == 2 and all(np.isfinite(h) and h > 0 for h in hist)
+
+
+def test_finetune_with_chains_runs(gc, ctx):
+    """corrector.finetune with a chain stage (three applications per training chain) on a
+    small configuration: finite parameters that moved, one positive loss per batch."""
+    from gridcalc_amd import corrector as cr
+    cfg = dict(cr.CONFIG, batch_traj=4, n_steps=80)
+    p0 = cr.init_params(6)
+    p, hist = cr.finetune(p0, 0.01, 0, 5, ((2, 3, 1e-4, 1e-5, 12, 0, 3),), cfg, log=lambda *a: None)
     assert p.shape == p0.shape and np.isfinite(p).all()
     assert not np.array_equal(p, p0)
     assert len(hist) == 2 and all(np.isfinite(h) and h > 0 for h in hist)

@@ -5,6 +5,8 @@ drifts by more than mass_tol; otherwise the macro step is recomputed by the fuse
 run in which every proposal is rejected is bit-identical to plain RK4, and a network that
 proposes no change leaves the state untouched.  The trained-network tests show what the
 network gains over the method without it (c(t + 16 dt) ~ c(t)) on held-out trajectories."""
+import time
+
 import numpy as np
 import pytest
 
@@ -159,8 +161,8 @@ def test_trained_network_beats_identity_one_step(gc, ctx, trained):
 
 
 def test_trained_learned_integration(gc, ctx, trained):
-    """T = 20 from a held-out initial state: mass is conserved to the tolerance, and the
-    learned path stays within 2 % (relative L2) of the FP64 RK4 solution."""
+    """T = 20 from a held-out initial state with the single network: mass is conserved to the
+    tolerance, and the learned path stays within 5e-3 (relative L2) of the FP64 RK4 solution."""
     cr, p, meta = trained
     net = gc.Corrector(ctx, H, W, 1)
     net.set_params(p, meta["scale"])
@@ -168,8 +170,71 @@ def test_trained_learned_integration(gc, ctx, trained):
     net.close()
     print("learned integration to T=20:", r)
     assert r["mass_drift"] <= 1e-6
-    assert r["rel_l2"] < 2e-2
+    assert r["rel_l2"] < 5e-3
     assert r["fallbacks"] < r["n_macro"]
+
+
+def test_phased_meets_accuracy_target(gc, ctx):
+    """The learned-corrector target on four held-out initial states (seeds never used in
+    training): the phase networks reach T = 20 (125 macro steps of 16 RK4 steps) within a
+    relative L2 error of 1e-3 of the FP64 RK4 solution on every state, with no fallback and
+    the mass conserved.  The speed ratio against the fastest plain GPU RK4 (the one-member
+    ensemble kernel replayed in HIP graphs, tile shape tuned) is measured beside it (median of
+    three runs each) and printed; it is asserted only loosely (>= 3x) so that the test does not
+    depend on the box's clocks; the bench records the measured ratio."""
+    from gridcalc_amd import corrector as cr
+    nets = cr.phase_nets(ctx, H, W)
+    if nets is None:
+        pytest.skip("no phase parameters in the package data")
+    g = gc.Grid.make(W, H, 1)
+    seeds = [991001 + i for i in range(4)]
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    ens = gc.Ensemble(ctx, [dict(nx=W, ny=H, kappa=1.0, M=1.0)], DT, gc.F64)
+    ens.tune()
+    errs, t_plain, t_learned = [], [], []
+    for k, seed in enumerate(seeds):
+        c0 = cr.initial_state(H, W, seed).astype(np.float64)
+        ens.set_state(0, c0)
+        if k == 0:
+            ens.step(32)
+            ens.set_state(0, c0)
+        t_plain.append(timed(lambda: ens.step(2000)))
+        r = ens.get_state(0).reshape(c0.shape)
+        d = gc.DeviceArray.from_numpy(ctx, c0)
+        if k == 0:
+            cr.integrate_phased(ctx, g, d, nets, 1.0, 1.0, DT, 16, 4)
+            d.upload(ctx, c0)
+        nf = [0]
+        t_learned.append(timed(lambda: nf.__setitem__(
+            0, cr.integrate_phased(ctx, g, d, nets, 1.0, 1.0, DT, 16, 125))))
+        a = d.numpy(ctx)
+        errs.append(float(np.linalg.norm(a - r) / np.linalg.norm(r)))
+        assert nf[0] == 0
+        assert abs(a.mean() - c0.mean()) <= 1e-6
+        d.free()
+    for k in range(2):  # two more timed runs of each path on the first state
+        c0 = cr.initial_state(H, W, seeds[0]).astype(np.float64)
+        ens.set_state(0, c0)
+        t_plain.append(timed(lambda: ens.step(2000)))
+        d = gc.DeviceArray.from_numpy(ctx, c0)
+        t_learned.append(timed(lambda: cr.integrate_phased(ctx, g, d, nets, 1.0, 1.0, DT, 16, 125)))
+        d.free()
+    ens.close()
+    for n, _, _ in nets:
+        n.close()
+    ratio = float(np.median(t_plain) / np.median(t_learned))
+    print(f"T=20 phased rel L2 per state {['%.3e' % e for e in errs]}; plain graph RK4 "
+          f"{np.median(t_plain) * 1e3:.2f} ms, learned {np.median(t_learned) * 1e3:.2f} ms, "
+          f"ratio {ratio:.2f}")
+    assert max(errs) <= 1e-3
+    assert ratio >= 3.0
 
 
 def test_phased_all_warm_is_plain_rk4(gc, ctx):
