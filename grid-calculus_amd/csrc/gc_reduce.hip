@@ -11,10 +11,15 @@
 // shuffles inside a wavefront, then LDS, then further passes -- whose every addition
 // pairs exactly the two halves the recursion pairs (IEEE addition is commutative).
 //
-// Loads are coalesced: 16 lanes fetch 16 consecutive values of one node into LDS,
-// then every lane consumes its own node's values in order from LDS.
+// One lane per leaf: the two lanes 2q, 2q+1 of a wave take the two leaves of a node (the
+// second one empty for a node of 64 values), so a wave covers 32 nodes.  Loads are coalesced:
+// 16 lanes fetch 16 consecutive values of one leaf into the wave's own LDS rows, then every
+// lane adds its leaf's values in order; the next 16 values of every leaf are already being
+// fetched into registers meanwhile.  No wave reads another wave's rows, so the loop needs no
+// workgroup barrier.
 #include "gc_common.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <mutex>
@@ -22,9 +27,11 @@
 namespace gc {
 namespace {
 
-constexpr int kChunk = 16;          // values per node per LDS round
+constexpr int kChunk = 16;          // values per leaf per LDS round
 constexpr int kPad = kChunk + 1;    // padded LDS row (conflict-free b64 reads)
-constexpr int kWaves = 4;           // waves per block in the leaf kernel
+constexpr int kWaves = 4;           // waves per block of the tree passes
+constexpr int kLeafWaves = 8;       // waves per block of the leaf kernel
+constexpr unsigned kNodesPerBlock = 32u * kLeafWaves;  // 256 nodes (512 leaves) per block
 
 // ---- value loaders ------------------------------------------------------------------
 
@@ -239,21 +246,49 @@ __device__ __forceinline__ double block_tree_sum(double s, unsigned cnt, double*
   return s;
 }
 
-// One lane per depth-`depth` node of the recursion over [0, n); every node holds at
-// most `maxsize` (= ceil(n / 2^depth) <= 128) values.
+// Orders the wave's LDS writes before its later reads (and reads before later writes) without
+// a workgroup barrier: the rows are private to the wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Perfect-tree sum over `cnt` (power of two, <= kNodesPerBlock) consecutive nodes of the
+// block whose sums are held by lane pairs (both lanes of a pair hold the node's sum); result
+// valid in thread 0.
+__device__ __forceinline__ double block_tree_sum_pairs(double s, unsigned cnt, double* sh) {
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const unsigned in_wave = cnt < 32u ? cnt : 32u;
+  for (unsigned st = 1; st < in_wave; st <<= 1) s = s + __shfl_xor(s, (int)(2 * st));
+  if (cnt <= 32u) return s;
+  if (lane == 0) sh[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nw = cnt / 32u;  // 2, 4 or 8 waves, combined as a perfect tree
+    if (nw == 2) s = sh[0] + sh[1];
+    else if (nw == 4) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    else s = ((sh[0] + sh[1]) + (sh[2] + sh[3])) + ((sh[4] + sh[5]) + (sh[6] + sh[7]));
+  }
+  return s;
+}
+
+// Depth-`depth` nodes of the recursion over [0, n), each of 64..128 values (or fewer for
+// n < 128), one leaf per lane; leafmax (<= 64) bounds every leaf's size.
 template <class L>
-__global__ __launch_bounds__(64 * kWaves) void k_pairwise_leaves(L ld, size_t n, int depth,
-                                                                 unsigned P, int maxsize,
-                                                                 double* __restrict__ partial) {
-  __shared__ double buf[kWaves][64 * kPad];
-  __shared__ size_t noff[kWaves][64];
-  __shared__ int nsize[kWaves][64];
-  __shared__ double sh[kWaves];
+__global__ __launch_bounds__(64 * kLeafWaves) void k_pairwise_leaves(L ld, size_t n, int depth,
+                                                                     unsigned P, int leafmax,
+                                                                     double* __restrict__ partial) {
+  __shared__ double buf[kLeafWaves][64 * kPad];
+  __shared__ size_t loff[kLeafWaves][64];
+  __shared__ int lsize[kLeafWaves][64];
+  __shared__ double sh[kLeafWaves];
   const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   // logical block: each XCD takes one contiguous range of nodes, so the neighbour rows and
   // planes a stencil-reading load touches stay in that XCD's L2
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
-  const unsigned q = blk * (64u * kWaves) + threadIdx.x;
+  const unsigned q = blk * kNodesPerBlock + wave * 32u + (lane >> 1);
+  const unsigned h = lane & 1u;
 
   size_t off = 0, size = 0;
   if (q < P) {
@@ -268,34 +303,47 @@ __global__ __launch_bounds__(64 * kWaves) void k_pairwise_leaves(L ld, size_t n,
       }
     }
   }
-  noff[wave][lane] = off;
-  nsize[wave][lane] = (int)size;
-  __syncthreads();
+  const bool split = size > 64;  // two leaves of floor(size/2) and ceil(size/2)
+  int mine = 0;
+  if (split) {
+    const size_t m = size >> 1;
+    mine = h ? (int)(size - m) : (int)m;
+    off += h ? m : 0;
+  } else if (!h) {
+    mine = (int)size;
+  }
+  loff[wave][lane] = off;
+  lsize[wave][lane] = mine;
+  wave_lds_sync();
 
-  const int split = size > 64 ? (int)(size >> 1) : INT_MAX;  // leaf boundary inside the node
-  double s0 = 0.0, s1 = 0.0;
   double* row = &buf[wave][0];
   const int e = (int)(lane & (kChunk - 1));
-  for (int base = 0; base < maxsize; base += kChunk) {
-    // cooperative load: 16 lanes fetch 16 consecutive values of one node, 4 nodes per pass
-#pragma unroll 4
+  double v[kChunk];
+  auto fetch = [&](int base) {
+#pragma unroll
     for (int pass = 0; pass < kChunk; ++pass) {
-      const int node = pass * (64 / kChunk) + (int)(lane / kChunk);
+      const int leaf = pass * (64 / kChunk) + (int)(lane / kChunk);
       const int idx = base + e;
-      if (idx < nsize[wave][node]) row[node * kPad + e] = ld(noff[wave][node] + (size_t)idx);
+      v[pass] = idx < lsize[wave][leaf] ? ld(loff[wave][leaf] + (size_t)idx) : 0.0;
     }
-    __syncthreads();
-    const int lim = min(kChunk, (int)size - base);
-    for (int t = 0; t < lim; ++t) {
-      const double v = row[lane * kPad + t];
-      if (base + t < split) s0 += v;
-      else s1 += v;
-    }
-    __syncthreads();
+  };
+  double s = 0.0;  // func/Integrate.hpp:66-69: s = 0.0, then += in order
+  fetch(0);
+  for (int base = 0; base < leafmax; base += kChunk) {
+#pragma unroll
+    for (int pass = 0; pass < kChunk; ++pass)
+      row[(pass * (64 / kChunk) + (int)(lane / kChunk)) * kPad + e] = v[pass];
+    wave_lds_sync();
+    if (base + kChunk < leafmax) fetch(base + kChunk);
+    const int lim = min(kChunk, mine - base);
+    for (int t = 0; t < lim; ++t) s += row[lane * kPad + t];
+    wave_lds_sync();
   }
-  double s = (split == INT_MAX) ? s0 : s0 + s1;
-  const unsigned cnt = P < 64u * kWaves ? P : 64u * kWaves;
-  s = block_tree_sum(s, cnt, sh);
+  // node sum: left leaf + right leaf (IEEE addition is commutative: both lanes get its bits)
+  const double o = __shfl_xor(s, 1);
+  s = split ? s + o : (h ? o : s);
+  const unsigned cnt = P < kNodesPerBlock ? P : kNodesPerBlock;
+  s = block_tree_sum_pairs(s, cnt, sh);
   if (threadIdx.x == 0) partial[blk] = s;
 }
 
@@ -316,8 +364,7 @@ inline size_t pairwise_scratch_bytes(size_t n, unsigned* nb_out, int* depth_out)
   while (((n + ((size_t)1 << E) - 1) >> E) > 64) ++E;
   const int depth = E > 0 ? E - 1 : 0;
   const unsigned P = 1u << depth;
-  const unsigned per_block = 64u * kWaves;
-  const unsigned nb = (P + per_block - 1) / per_block;
+  const unsigned nb = (P + kNodesPerBlock - 1) / kNodesPerBlock;
   if (nb_out) *nb_out = nb;
   if (depth_out) *depth_out = depth;
   return 2 * (size_t)nb * sizeof(double) + 256;
@@ -332,12 +379,16 @@ gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, double*
   int depth = 0;
   pairwise_scratch_bytes(n, &nb, &depth);
   const unsigned P = 1u << depth;
-  const unsigned per_block = 64u * kWaves;
   double* a = (double*)scr;
   double* b = a + nb;
+  // nodes hold floor(n / 2^depth) or ceil(n / 2^depth) values; a node of more than 64 has
+  // leaves of at most ceil(size / 2), one of at most 64 is a single leaf (65: 33, but 64: 64)
   const int maxsize = (int)((n + ((size_t)1 << depth) - 1) >> depth);
-  hipLaunchKernelGGL((k_pairwise_leaves<L>), dim3(nb), dim3(per_block), 0, stream, ld, n, depth,
-                     P, maxsize, nb == 1 ? final_out : a);
+  const int minsize = (int)(n >> depth);
+  auto leaf_of = [](int sz) { return sz > 64 ? (sz + 1) / 2 : sz; };
+  const int leafmax = std::max(leaf_of(maxsize), leaf_of(minsize));
+  hipLaunchKernelGGL((k_pairwise_leaves<L>), dim3(nb), dim3(64 * kLeafWaves), 0, stream, ld, n,
+                     depth, P, leafmax, nb == 1 ? final_out : a);
   GC_LAUNCHED();
   unsigned cnt = nb;
   while (cnt > 1) {
