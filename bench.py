@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -256,7 +257,9 @@ def _kernel_name(kind, T, tile):
 
 def sweep(a):
     """3-D operator sweep: GB/s per operator from algorithmic bytes, FP64 and FP32, and the
-    FP32 error against FP64."""
+    FP32 result's relative L2 and max error against the FP64 result of the same operator
+    (each precision on its own-precision copy of the inputs; the FP64 results are the
+    reference's bits, which the parity tests pin)."""
     gc = entry.load_package()
     n = a.n or 512
     h = 1.0 / n
@@ -275,18 +278,27 @@ def sweep(a):
         vv[..., c] = gc.inputs.sweep_field(n, h, seed=11 + c)
     D = gc.inputs.lognormal(5, n ** 3).reshape(n, n, n)
     with gc.Context() as ctx:
-        for dtype, w in ((np.float64, 8), (np.float32, 4)):
+        dev = {}
+        for dtype in (np.float64, np.float32):
             d_u = gc.DeviceArray.from_numpy(ctx, u.astype(dtype))
             d_v = gc.DeviceArray.from_numpy(ctx, vv.astype(dtype))
             d_D = gc.DeviceArray.from_numpy(ctx, D.astype(dtype))
             outs = {"laplacian": gc.DeviceArray((n, n, n), dtype),
                     "gradient": gc.DeviceArray((n, n, n, 3), dtype),
                     "mat": gc.DeviceArray((n, n, n, 9), dtype)}
-            # the tensor input of divergence_mat: the order-2 gradient of the vector field
+            # the tensor input of divergence_mat: the order-2 gradient of the vector field (in
+            # FP32 from the FP32 vector field, so each precision runs on its own inputs)
             gc.gradient_vec(ctx, g, d_v, 2, out=outs["mat"])
             d_m = gc.DeviceArray((n, n, n, 9), dtype)
             d_m.copy_from(ctx, outs["mat"])
-            for name, order in ops:
+            dev[dtype] = (d_u, d_v, d_D, d_m, outs)
+        for name, order in ops:
+            out64 = None
+            for dtype, w in ((np.float64, 8), (np.float32, 4)):
+                d_u, d_v, d_D, d_m, outs = dev[dtype]
+                key = {"gradient_vec": "mat", "divergence_mat": "gradient",
+                       "gradient": "gradient"}.get(name, "laplacian")
+
                 def run():
                     if name == "gradient_vec":
                         gc.gradient_vec(ctx, g, d_v, order, out=outs["mat"])
@@ -312,7 +324,25 @@ def sweep(a):
                 gbs = words[name] * w * n ** 3 / (ms / 1e3) / 1e9
                 rows.append({"op": name, "order": order, "dtype": "f64" if w == 8 else "f32",
                              "ms": ms, "GBs_alg": gbs, "hbm_frac": gbs / HBM_PEAK_GBS})
+                res = outs[key].numpy(ctx)
+                if dtype is np.float64:
+                    out64 = res
+                else:  # FP32 against FP64 (each on its own-precision input), in chunks
+                    sq = ref = 0.0
+                    mx = mref = 0.0
+                    a64, a32 = out64.reshape(-1), res.reshape(-1)
+                    for lo in range(0, a64.size, 1 << 26):
+                        r64 = a64[lo:lo + (1 << 26)]
+                        dlt = a32[lo:lo + (1 << 26)].astype(np.float64) - r64
+                        sq += float(np.dot(dlt, dlt))
+                        ref += float(np.dot(r64, r64))
+                        mx = max(mx, float(np.abs(dlt).max()))
+                        mref = max(mref, float(np.abs(r64).max()))
+                    rows[-1]["vs_f64"] = {"l2_rel": math.sqrt(sq / ref), "linf_rel": mx / mref}
+                    out64 = None
+                del res
                 print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        for d_u, d_v, d_D, d_m, outs in dev.values():
             for x in (d_u, d_v, d_D, d_m, *outs.values()):
                 x.free()
     cpu = None
