@@ -533,32 +533,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   }
 }
 
-// ---- vector- and tensor-field operators: z-queue engine -----------------------------------
+// ---- vector- and tensor-field operators: z-accumulating engine ----------------------------
 //
-// The inputs are AoS (CIN values per site).  Each derivative direction reads only some of the
-// components: those differentiated along x are staged per plane in LDS with an x halo, those
-// along y with a y halo, and those along z are kept by the thread that owns the column in a
-// register queue of the 2R+1 planes z-R..z+R.  A block owns a 64 x 8 tile (two columns per
-// thread) and marches a chunk of planes; the next plane's values are fetched while the
-// current plane is formed.  Accumulation order is the reference's, as in the kernels above.
+// The inputs are AoS (CIN values per site).  A block owns a 64 x 8 tile (one column per
+// thread) and marches a chunk of planes.  Each plane's own-column sites are loaded once, all
+// components together: a kernel that fetched a plane's z-differentiated components R planes
+// before its x- and y-differentiated ones found the lines gone from L2 the second time and
+// read the input 2-2.5 times from HBM.  Components differentiated along z are not queued:
+// plane p adds its term w[p-z'+R] v(p) to the running z sum of every output plane
+// z' = p-R..p+R on arrival, so each output's terms come in increasing offset, the reference's
+// order from 0.  Components differentiated along x and y are staged in LDS with an x or y halo
+// when the plane is itself an output plane, and the in-plane part of the result is kept in
+// registers until that output's z sum completes R planes later.  The z sum and in-plane part
+// of output plane z0+j sit in register j mod (2R+1); the plane loop is unrolled by 2R+1 so
+// every such index is a compile-time constant.
 
-template <int R, int NXc, int NYc, int NZc>
+template <int R, int TOY>
 struct VecTile {
-  static constexpr int OX = 64, OY = 8, NT = 256, COLS = OX * OY / NT;  // 2 columns per thread
-  static constexpr int PXW = OX + 2 * R;                                // x-staged plane width
-  static constexpr int PYH = OY + 2 * R;                                // y-staged plane height
-  static constexpr int NXH = 2 * R * OY, NYH = 2 * R * OX;              // halo sites
+  static constexpr int OX = 64, OY = TOY, NT = 64 * TOY;  // one column per thread
+  static constexpr int PXW = OX + 2 * R;            // x-staged plane width
+  static constexpr int PYH = OY + 2 * R;            // y-staged plane height
+  static constexpr int NXH = 2 * R * OY, NYH = 2 * R * OX;  // halo sites
   static constexpr int LXH = (NXH + NT - 1) / NT, LYH = (NYH + NT - 1) / NT;
 };
 
-template <class T, int O, class Op>
-__global__ __launch_bounds__(256) void k_vec_zq(Op op, const T* __restrict__ in,
+template <class T, int O, class Op, int TOY>
+__global__ __launch_bounds__(64 * TOY) void k_vec_za(Op op, const T* __restrict__ in,
                                                 T* __restrict__ out, Dims d, int ntx, int nty,
                                                 int zlen) {
-  constexpr int R = D1<O>::R, NXc = Op::NXc, NYc = Op::NYc, NZc = Op::NZc, CIN = Op::CIN;
-  using V = VecTile<R, NXc, NYc, NZc>;
-  __shared__ T px[NXc > 0 ? NXc : 1][V::OY * V::PXW];
-  __shared__ T py[NYc > 0 ? NYc : 1][V::PYH * V::OX];
+  using S = D1<O>;
+  constexpr int R = S::R, QP = 2 * R + 1;
+  constexpr int NXc = Op::NXc, NYc = Op::NYc, NZc = Op::NZc, CIN = Op::CIN, NTQ = Op::NTQ;
+  using V = VecTile<R, TOY>;
+  __shared__ T px[NXc][V::OY * V::PXW];
+  __shared__ T py[NYc][V::PYH * V::OX];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
@@ -567,16 +575,10 @@ __global__ __launch_bounds__(256) void k_vec_zq(Op op, const T* __restrict__ in,
   const int zc = (int)(rr / (unsigned)nty);
   const int x0 = tx * V::OX, y0 = ty * V::OY, z0 = zc * zlen;
   const int nout = min(zlen, d.nz - z0);
-  const int ox = t % V::OX, oy0 = t / V::OX;  // columns (ox, oy0) and (ox, oy0 + 4)
-
-  unsigned coff[V::COLS];  // own columns, wrapped (a column past the grid edge is a halo)
-  bool cin[V::COLS];
-#pragma unroll
-  for (int c = 0; c < V::COLS; ++c) {
-    const int gy = y0 + oy0 + c * (V::NT / V::OX);
-    cin[c] = x0 + ox < d.nx && gy < d.ny;
-    coff[c] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
-  }
+  const int ox = t % V::OX, oy = t / V::OX;
+  // own column, wrapped (a column past the grid edge is another tile's halo)
+  const bool cin = x0 + ox < d.nx && y0 + oy < d.ny;
+  const unsigned coff = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(y0 + oy, d.ny));
   // x halo: columns -R..-1 and OX..OX+R-1 of each row; y halo: rows -R..-1, OY..OY+R-1
   unsigned xhoff[V::LXH], yhoff[V::LYH];
   int xhslot[V::LXH], yhslot[V::LYH];
@@ -597,150 +599,149 @@ __global__ __launch_bounds__(256) void k_vec_zq(Op op, const T* __restrict__ in,
     yhoff[s] = h < V::NYH ? (unsigned)(wrap_any(x0 + col, d.nx) + d.nx * wrap_any(y0 + ly - R, d.ny)) : 0u;
   }
 
-  T own[V::COLS][NXc + NYc > 0 ? NXc + NYc : 1];  // own columns, x- and y-staged comps
-  T xh[V::LXH][NXc > 0 ? NXc : 1], yh[V::LYH][NYc > 0 ? NYc : 1];
-  T znext[V::COLS][NZc > 0 ? NZc : 1];             // z-staged comps of plane z+R
-  T q[V::COLS][NZc > 0 ? NZc : 1][2 * R + 1];       // z queue, planes z-R..z+R
-
-  auto load_xy = [&](int plane) {
-    const T* ip = in + (size_t)CIN * (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+  T own[CIN];                    // the own site of the plane being processed, every component
+  T xh[V::LXH][NXc], yh[V::LYH][NYc];
+  T acc[NZc][QP];                // z sums of output planes z0 + j at register j mod QP
+  T tq[NTQ][QP];                 // their in-plane parts
+  const int steps = nout + 2 * R;  // planes z0-R .. z0+nout-1+R
+  auto is_out = [&](int st) { return st >= R && st < R + nout; };
+  auto load = [&](int st) {
+    const size_t po = (size_t)d.plane * (size_t)wrap_any(z0 - R + st, d.nz);
+    const T* sp = in + (size_t)CIN * (po + coff);
 #pragma unroll
-    for (int c = 0; c < V::COLS; ++c) {
+    for (int i = 0; i < CIN; ++i) own[i] = sp[i];
+    if (is_out(st)) {
 #pragma unroll
-      for (int i = 0; i < NXc; ++i) own[c][i] = ip[(size_t)CIN * coff[c] + Op::xc(i)];
+      for (int s = 0; s < V::LXH; ++s)
+        if (xhslot[s] >= 0)
 #pragma unroll
-      for (int i = 0; i < NYc; ++i) own[c][NXc + i] = ip[(size_t)CIN * coff[c] + Op::yc(i)];
+          for (int i = 0; i < NXc; ++i) xh[s][i] = in[(size_t)CIN * (po + xhoff[s]) + Op::xc(i)];
+#pragma unroll
+      for (int s = 0; s < V::LYH; ++s)
+        if (yhslot[s] >= 0)
+#pragma unroll
+          for (int i = 0; i < NYc; ++i) yh[s][i] = in[(size_t)CIN * (po + yhoff[s]) + Op::yc(i)];
     }
-#pragma unroll
-    for (int s = 0; s < V::LXH; ++s)
-      if (xhslot[s] >= 0)
-#pragma unroll
-        for (int i = 0; i < NXc; ++i) xh[s][i] = ip[(size_t)CIN * xhoff[s] + Op::xc(i)];
-#pragma unroll
-    for (int s = 0; s < V::LYH; ++s)
-      if (yhslot[s] >= 0)
-#pragma unroll
-        for (int i = 0; i < NYc; ++i) yh[s][i] = ip[(size_t)CIN * yhoff[s] + Op::yc(i)];
   };
-  auto load_z = [&](int plane, T (&dst)[V::COLS][NZc > 0 ? NZc : 1]) {
-    const T* ip = in + (size_t)CIN * (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+  load(0);
+  const size_t obase = (size_t)d.plane * (size_t)z0 + coff;
+  for (int sb = 0; sb < steps; sb += QP) {
+    unroll_seq(std::make_integer_sequence<int, QP>{}, [&](auto Pc) {
+      constexpr int P = decltype(Pc)::value;  // step st = sb + P, st mod QP = P
+      const int st = sb + P;
+      if (st >= steps) return;
+      // z sums: this plane's term for output j = st - R - o (offset o)
 #pragma unroll
-    for (int c = 0; c < V::COLS; ++c)
+      for (int m = 0; m < S::N; ++m) {
+        const int o = m - S::R;
+        const int j = st - R - o;
+        const int slot = ((P - R - o) % QP + QP) % QP;
+        if (j >= 0 && j < nout) {
+          const T w = T(S::w(m));
 #pragma unroll
-      for (int i = 0; i < NZc; ++i) dst[c][i] = ip[(size_t)CIN * coff[c] + Op::zc(i)];
-  };
-
-  // queue holds planes z0-R .. z0+R-1 in slots 0..2R-1 before the first iteration
-  for (int j = 0; j < 2 * R; ++j) {
-    T tmp[V::COLS][NZc > 0 ? NZc : 1];
-    load_z(z0 - R + j, tmp);
-#pragma unroll
-    for (int c = 0; c < V::COLS; ++c)
-#pragma unroll
-      for (int i = 0; i < NZc; ++i) {
-#pragma unroll
-        for (int m = 0; m < 2 * R; ++m) q[c][i][m] = q[c][i][m + 1];
-        q[c][i][2 * R - 1] = tmp[c][i];
+          for (int i = 0; i < NZc; ++i) {
+            if (m == 0) acc[i][slot] = T(0);
+            acc[i][slot] += w * own[Op::zc(i)];
+          }
+        }
       }
-  }
-  load_xy(z0);
-  load_z(z0 + R, znext);
-  for (int k = 0; k < nout; ++k) {
+      if (is_out(st)) {  // in-plane part of output j = st - R
 #pragma unroll
-    for (int c = 0; c < V::COLS; ++c) {
-      const int oy = oy0 + c * (V::NT / V::OX);
+        for (int i = 0; i < NXc; ++i) px[i][oy * V::PXW + ox + R] = own[Op::xc(i)];
 #pragma unroll
-      for (int i = 0; i < NXc; ++i) px[i][oy * V::PXW + ox + R] = own[c][i];
+        for (int i = 0; i < NYc; ++i) py[i][(oy + R) * V::OX + ox] = own[Op::yc(i)];
 #pragma unroll
-      for (int i = 0; i < NYc; ++i) py[i][(oy + R) * V::OX + ox] = own[c][NXc + i];
+        for (int s = 0; s < V::LXH; ++s)
+          if (xhslot[s] >= 0)
 #pragma unroll
-      for (int i = 0; i < NZc; ++i) q[c][i][2 * R] = znext[c][i];
-    }
+            for (int i = 0; i < NXc; ++i) px[i][xhslot[s]] = xh[s][i];
 #pragma unroll
-    for (int s = 0; s < V::LXH; ++s)
-      if (xhslot[s] >= 0)
+        for (int s = 0; s < V::LYH; ++s)
+          if (yhslot[s] >= 0)
 #pragma unroll
-        for (int i = 0; i < NXc; ++i) px[i][xhslot[s]] = xh[s][i];
-#pragma unroll
-    for (int s = 0; s < V::LYH; ++s)
-      if (yhslot[s] >= 0)
-#pragma unroll
-        for (int i = 0; i < NYc; ++i) py[i][yhslot[s]] = yh[s][i];
-    __syncthreads();
-    if (k + 1 < nout) {
-      load_xy(z0 + k + 1);
-      load_z(z0 + k + 1 + R, znext);
-    }
-    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
-#pragma unroll
-    for (int c = 0; c < V::COLS; ++c) {
-      if (cin[c]) {
-        const int oy = oy0 + c * (V::NT / V::OX);
+            for (int i = 0; i < NYc; ++i) py[i][yhslot[s]] = yh[s][i];
+        __syncthreads();
+        if (st + 1 < steps) load(st + 1);
+        constexpr int sin = ((P - R) % QP + QP) % QP;
         auto X = [&](int i, int o) { return px[i][oy * V::PXW + ox + R + o]; };
         auto Y = [&](int i, int o) { return py[i][(oy + R + o) * V::OX + ox]; };
-        auto Z = [&](int i, int o) { return q[c][i][R + o]; };
-        op.template apply<O>(X, Y, Z, out, zo + coff[c]);
+        T tv[NTQ];
+        op.template inplane<O>(X, Y, tv);
+#pragma unroll
+        for (int q = 0; q < NTQ; ++q) tq[q][sin] = tv[q];
+        __syncthreads();
+      } else if (st + 1 < steps) {
+        load(st + 1);
       }
-    }
+      const int jd = st - 2 * R;  // output whose z sum this plane completed
+      if (jd >= 0 && jd < nout && cin) {
+        constexpr int sd = ((P - 2 * R) % QP + QP) % QP;
+        T tv[NTQ], dz[NZc];
 #pragma unroll
-    for (int c = 0; c < V::COLS; ++c)
+        for (int q = 0; q < NTQ; ++q) tv[q] = tq[q][sd];
 #pragma unroll
-      for (int i = 0; i < NZc; ++i)
-#pragma unroll
-        for (int m = 0; m < 2 * R; ++m) q[c][i][m] = q[c][i][m + 1];
-    __syncthreads();
+        for (int i = 0; i < NZc; ++i) dz[i] = acc[i][sd];
+        op.finish(tv, dz, out + (size_t)Op::NOUT * (obase + (size_t)d.plane * (size_t)jd));
+      }
+    });
   }
 }
 
-// diff/Divergence.hpp:56-69 (Vec3 -> scalar): V_x along x, V_y along y, V_z along z
+// diff/Divergence.hpp:56-69 (Vec3 -> scalar): V_x along x, V_y along y, V_z along z,
+// ((ax ihx + ay ihy) + az ihz)
 template <class T>
 struct OpDivVec {
-  static constexpr int CIN = 3, NXc = 1, NYc = 1, NZc = 1;
+  static constexpr int CIN = 3, NXc = 1, NYc = 1, NZc = 1, NTQ = 1, NOUT = 1;
   __host__ __device__ static constexpr int xc(int) { return 0; }
   __host__ __device__ static constexpr int yc(int) { return 1; }
   __host__ __device__ static constexpr int zc(int) { return 2; }
   T ihx, ihy, ihz;
-  template <int O, class FX, class FY, class FZ>
-  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+  template <int O, class FX, class FY>
+  __device__ __forceinline__ void inplane(FX X, FY Y, T (&tq)[NTQ]) const {
     using S = D1<O>;
-    T ax = 0, ay = 0, az = 0;
+    T ax = 0, ay = 0;
 #pragma unroll
     for (int m = 0; m < S::N; ++m) {
       const int o = m - S::R;
       const T w = T(S::w(m));
       ax += w * X(0, o);
       ay += w * Y(0, o);
-      az += w * Z(0, o);
     }
-    out[c] = ax * ihx + ay * ihy + az * ihz;
+    tq[0] = ax * ihx + ay * ihy;
+  }
+  __device__ __forceinline__ void finish(const T (&tq)[NTQ], const T (&dz)[NZc], T* o) const {
+    o[0] = tq[0] + dz[0] * ihz;
   }
 };
 
-// diff/Divergence.hpp:113-131 (Mat3 -> Vec3): column a of M along axis a
+// diff/Divergence.hpp:113-131 (Mat3 -> Vec3): column a of M along axis a,
+// out_i = (d_x M(i,0) ihx + d_y M(i,1) ihy) + d_z M(i,2) ihz
 template <class T>
 struct OpDivMat {
-  static constexpr int CIN = 9, NXc = 3, NYc = 3, NZc = 3;
+  static constexpr int CIN = 9, NXc = 3, NYc = 3, NZc = 3, NTQ = 3, NOUT = 3;
   __host__ __device__ static constexpr int xc(int i) { return i; }
   __host__ __device__ static constexpr int yc(int i) { return 3 + i; }
   __host__ __device__ static constexpr int zc(int i) { return 6 + i; }
   T ihx, ihy, ihz;
-  template <int O, class FX, class FY, class FZ>
-  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+  template <int O, class FX, class FY>
+  __device__ __forceinline__ void inplane(FX X, FY Y, T (&tq)[NTQ]) const {
     using S = D1<O>;
-    T* o3 = out + 3 * c;
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      T dx = 0, dy = 0, dz = 0;
+      T dx = 0, dy = 0;
 #pragma unroll
       for (int m = 0; m < S::N; ++m) {
         const int o = m - S::R;
         const T w = T(S::w(m));
         dx += w * X(comp, o);
         dy += w * Y(comp, o);
-        dz += w * Z(comp, o);
       }
-      o3[comp] = dx * ihx + dy * ihy + dz * ihz;
+      tq[comp] = dx * ihx + dy * ihy;
     }
+  }
+  __device__ __forceinline__ void finish(const T (&tq)[NTQ], const T (&dz)[NZc], T* o) const {
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) o[comp] = tq[comp] + dz[comp] * ihz;
   }
 };
 
@@ -748,40 +749,49 @@ struct OpDivMat {
 // along every axis
 template <class T>
 struct OpGradVec {
-  static constexpr int CIN = 3, NXc = 3, NYc = 3, NZc = 3;
+  // The in-plane entries wait for the z entries in registers: stored at once, each site's
+  // line would be written twice, R planes apart, and the partial writes do not merge in L2
+  // (5.5 ms against 3.96 ms at 512^3 in FP64).
+  static constexpr int CIN = 3, NXc = 3, NYc = 3, NZc = 3, NTQ = 6, NOUT = 9;
   __host__ __device__ static constexpr int xc(int i) { return i; }
   __host__ __device__ static constexpr int yc(int i) { return i; }
   __host__ __device__ static constexpr int zc(int i) { return i; }
   T ihx, ihy, ihz;
-  template <int O, class FX, class FY, class FZ>
-  __device__ __forceinline__ void apply(FX X, FY Y, FZ Z, T* __restrict__ out, size_t c) const {
+  template <int O, class FX, class FY>
+  __device__ __forceinline__ void inplane(FX X, FY Y, T (&tq)[NTQ]) const {
     using S = D1<O>;
-    T* o9 = out + 9 * c;
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      T dx = 0, dy = 0, dz = 0;
+      T dx = 0, dy = 0;
 #pragma unroll
       for (int m = 0; m < S::N; ++m) {
         const int o = m - S::R;
         const T w = T(S::w(m));
         dx += w * X(comp, o);
         dy += w * Y(comp, o);
-        dz += w * Z(comp, o);
       }
-      o9[comp + 0] = dx * ihx;
-      o9[comp + 3] = dy * ihy;
-      o9[comp + 6] = dz * ihz;
+      tq[comp] = dx * ihx;
+      tq[3 + comp] = dy * ihy;
     }
+  }
+  __device__ __forceinline__ void finish(const T (&tq)[NTQ], const T (&dz)[NZc], T* o) const {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) o[q] = tq[q];
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) o[6 + comp] = dz[comp] * ihz;
   }
 };
 
 constexpr int kVLen = 32;  // planes per block
 
+constexpr int kVOY = 8;  // tile height of the vector / tensor kernels (64 x kVOY sites)
+
 template <class T, int O, class Op>
 void vec_launch(gc_context* ctx, const Dims& d, const Op& op, const T* in, T* out) {
-  const int ntx = (d.nx + 63) / 64, nty = (d.ny + 7) / 8, nzc = (d.nz + kVLen - 1) / kVLen;
-  hipLaunchKernelGGL((k_vec_zq<T, O, Op>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
-                     ctx->stream, op, in, out, d, ntx, nty, kVLen);
+  const int ntx = (d.nx + 63) / 64, nty = (d.ny + kVOY - 1) / kVOY;
+  const int nzc = (d.nz + kVLen - 1) / kVLen;
+  hipLaunchKernelGGL((k_vec_za<T, O, Op, kVOY>), dim3((unsigned)(ntx * nty * nzc)),
+                     dim3(64 * kVOY), 0, ctx->stream, op, in, out, d, ntx, nty, kVLen);
 }
 
 template <class T>
