@@ -28,6 +28,7 @@ namespace {
 template <class T, int O>
 struct OpLap {
   static constexpr int R = D2<O>::R, NF = 1;
+  static constexpr int kTileRows = 16;  // 512^3 FP64: O2 0.444 -> 0.428 ms, O4 0.562 -> 0.524
   T ihx2, ihy2, ihz2;
   template <class A>
   __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
@@ -49,6 +50,7 @@ struct OpLap {
 template <class T, int O>
 struct OpGrad {
   static constexpr int R = D1<O>::R, NF = 1;
+  static constexpr int kTileRows = 16;  // 512^3 FP64: O2 1.082 -> 0.978 ms, O4 1.124 -> 1.007
   T ihx, ihy, ihz;
   template <class A>
   __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
@@ -367,9 +369,10 @@ __global__ __launch_bounds__(256) void k_axis_zq(Op op, const T* __restrict__ f,
 
 template <class T, class Op>
 void axis_launch(gc_context* ctx, const Dims& d, const Op& op, const T* in, T* out) {
-  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + kZOY - 1) / kZOY;
+  constexpr int OY = ZRows<Op>::OY;
+  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + OY - 1) / OY;
   const int nzc = (d.nz + kZLen - 1) / kZLen;
-  hipLaunchKernelGGL((k_axis_zq<T, kZOX, kZOY, Op>), dim3((unsigned)(ntx * nty * nzc)),
+  hipLaunchKernelGGL((k_axis_zq<T, kZOX, OY, Op>), dim3((unsigned)(ntx * nty * nzc)),
                      dim3(256), 0, ctx->stream, op, in, out, d, ntx, nty, kZLen);
 }
 
