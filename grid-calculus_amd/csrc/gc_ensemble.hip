@@ -86,8 +86,12 @@ __global__ __launch_bounds__(256) void k_ens_step(const EnsMember* __restrict__ 
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem wi = work[w];
     const EnsMember m = mem[wi.member];
-    const T* __restrict__ src = pool + m.off[parity];
-    T* __restrict__ dst = pool + m.off[parity ^ 1];
+    // selects, not m.off[parity]: a runtime index into the copied struct would place the
+    // whole struct in per-lane scratch memory (80 bytes a lane, written back through L2)
+    const size_t off_in = parity ? m.off[1] : m.off[0];
+    const size_t off_out = parity ? m.off[0] : m.off[1];
+    const T* __restrict__ src = pool + off_in;
+    T* __restrict__ dst = pool + off_out;
     const T M = T(m.M), kappa = T(m.kappa), ihx2 = T(m.ihx2), ihy2 = T(m.ihy2);
     __syncthreads();  // the previous item is done with LDS
     for (int idx = t; idx < C::N; idx += C::NT) {
