@@ -16,6 +16,7 @@
 // (every block exits when the list is exhausted).  gc_ensemble_step replays a captured
 // hipGraph of an even number of steps.
 #include "gc_common.hpp"
+#include "gc_arith.hpp"
 
 #include <algorithm>
 #include <map>
@@ -53,17 +54,39 @@ __device__ __forceinline__ int wrap_mod(int x, int n) {
 }
 
 // Order-2 2-D Laplacian from an LDS plane of width W (diff/Laplacian.hpp:60-70 order).
-template <class T, int W>
-__device__ __forceinline__ T lap2d(const T* __restrict__ f, int o, T ihx2, T ihy2) {
-  const T c = f[o];
-  T dxx = 0, dyy = 0;
-  dxx += T(1) * f[o - 1];
-  dyy += T(1) * f[o - W];
-  dxx += T(-2) * c;
-  dyy += T(-2) * c;
-  dxx += T(1) * f[o + 1];
-  dyy += T(1) * f[o + W];
+template <int W>
+__device__ __forceinline__ double lap2d(const double* __restrict__ f, int o, double ihx2,
+                                        double ihy2) {
+  const double c = f[o];
+  double dxx = 0, dyy = 0;
+  dxx += f[o - 1];
+  dyy += f[o - W];
+  dxx += -2.0 * c;
+  dyy += -2.0 * c;
+  dxx += f[o + 1];
+  dyy += f[o + W];
   return dxx * ihx2 + dyy * ihy2;
+}
+
+// mu and k = M lap(mu) at LDS offset o.  FP64: the reference's arithmetic without the zero
+// z terms; FP32: the library's FP32 form (gc_arith.hpp).
+template <int W>
+__device__ __forceinline__ double ens_mu(const CHCoef<double>& cf, const double* f, int o) {
+  const double l = lap2d<W>(f, o, cf.ihx2, cf.ihy2);
+  const double p = f[o];
+  return -p + p * p * p - cf.kappa * l;
+}
+template <int W>
+__device__ __forceinline__ double ens_k(const CHCoef<double>& cf, const double* f, int o) {
+  return lap2d<W>(f, o, cf.ihx2, cf.ihy2) * cf.M;
+}
+template <int W>
+__device__ __forceinline__ float ens_mu(const CHCoef<float>& cf, const float* f, int o) {
+  return ch_mu2(cf, f[o], f[o - 1], f[o + 1], f[o - W], f[o + W]);
+}
+template <int W>
+__device__ __forceinline__ float ens_k(const CHCoef<float>& cf, const float* f, int o) {
+  return ch_k2(cf, f[o], f[o - 1], f[o + 1], f[o - W], f[o + W]);
 }
 
 template <int OX, int OY>
@@ -92,7 +115,7 @@ __global__ __launch_bounds__(256) void k_ens_step(const EnsMember* __restrict__ 
     const size_t off_out = parity ? m.off[0] : m.off[1];
     const T* __restrict__ src = pool + off_in;
     T* __restrict__ dst = pool + off_out;
-    const T M = T(m.M), kappa = T(m.kappa), ihx2 = T(m.ihx2), ihy2 = T(m.ihy2);
+    const CHCoef<T> cf = make_ch_coef<T>(T(m.M), T(m.kappa), T(m.ihx2), T(m.ihy2), T(0));
     __syncthreads();  // the previous item is done with LDS
     for (int idx = t; idx < C::N; idx += C::NT) {
       const int px = idx % W, py = idx / W;
@@ -108,9 +131,7 @@ __global__ __launch_bounds__(256) void k_ens_step(const EnsMember* __restrict__ 
         for (int idx = t; idx < nm; idx += C::NT) {
           const int mx = idx % wm, my = idx / wm;
           const int o = (my + H - rm) * W + (mx + H - rm);
-          const T l = lap2d<T, W>(in, o, ihx2, ihy2);
-          const T p = in[o];
-          MU[o] = -p + p * p * p - kappa * l;
+          MU[o] = ens_mu<W>(cf, in, o);
         }
       }
       __syncthreads();
@@ -119,24 +140,24 @@ __global__ __launch_bounds__(256) void k_ens_step(const EnsMember* __restrict__ 
         for (int idx = t; idx < nk; idx += C::NT) {
           const int kx = idx % wk, ky = idx / wk;
           const int o = (ky + H - rk) * W + (kx + H - rk);
-          const T kv = lap2d<T, W>(MU, o, ihx2, ihy2) * M;
+          const T kv = ens_k<W>(cf, MU, o);
           const int ix = kx - rk, iy = ky - rk;
           const bool interior = ix >= 0 && ix < OX && iy >= 0 && iy < OY;
           const int ii = iy * OX + ix;
           if (s == 1) {
-            Y[o] = P[o] + co.half_dt * kv;
+            Y[o] = stage_axpy(P[o], co.half_dt, kv);
             if (interior) K1[ii] = kv;
           } else if (s == 2) {
-            Y[o] = P[o] + co.half_dt * kv;
+            Y[o] = stage_axpy(P[o], co.half_dt, kv);
             if (interior) K23[ii] = kv;
           } else if (s == 3) {
-            Y[o] = P[o] + co.dt * kv;
+            Y[o] = stage_axpy(P[o], co.dt, kv);
             if (interior) K23[ii] = K23[ii] + kv;
           } else {
             const int gx = wi.x0 + ix, gy = wi.y0 + iy;
             if (gx < m.nx && gy < m.ny)
               dst[(size_t)gx + (size_t)m.nx * (size_t)gy] =
-                  P[o] + (co.sixth_dt * (K1[ii] + kv) + co.third_dt * K23[ii]);
+                  rk4_final(P[o], K1[ii], K23[ii], kv, co.sixth_dt, co.third_dt);
           }
         }
       }

@@ -14,6 +14,7 @@
 // which performs exactly the reference's roundings (k2 + k3 is formed once, as the
 // reference forms p2 + p3).  HBM traffic per step: 3 + 4 + 5 + 5 = 17 words per site.
 #include "gc_common.hpp"
+#include "gc_arith.hpp"
 
 #include <map>
 #include <mutex>
@@ -76,37 +77,31 @@ __device__ __forceinline__ size_t lin(const Dims& d, int i, int j, int k) {
   return (size_t)i + (size_t)d.nx * (size_t)j + (size_t)d.plane * (size_t)k;
 }
 
-// Order-2 Laplacian of y at (i,j,k) (diff/Laplacian.hpp:60-70).
+// The seven values of the order-2 Laplacian stencil at (i,j,k): centre, x-, x+, y-, y+, z-, z+.
 template <class T>
-__device__ __forceinline__ T lap2_at(const T* __restrict__ y, const Dims& d, int i, int j, int k,
-                                     T ihx2, T ihy2, T ihz2) {
+struct Nb7 {
+  T c, xm, xp, ym, yp, zm, zp;
+};
+template <class T>
+__device__ __forceinline__ Nb7<T> nb7_at(const T* __restrict__ y, const Dims& d, int i, int j,
+                                         int k) {
   const int im = wrap(i - 1, d.nx), ip = wrap(i + 1, d.nx);
   const int jm = wrap(j - 1, d.ny), jp = wrap(j + 1, d.ny);
   const int km = wrap(k - 1, d.nz), kp = wrap(k + 1, d.nz);
-  const T c = y[lin(d, i, j, k)];
-  T dxx = 0, dyy = 0, dzz = 0;
-  dxx += T(1) * y[lin(d, im, j, k)];
-  dyy += T(1) * y[lin(d, i, jm, k)];
-  dzz += T(1) * y[lin(d, i, j, km)];
-  dxx += T(-2) * c;
-  dyy += T(-2) * c;
-  dzz += T(-2) * c;
-  dxx += T(1) * y[lin(d, ip, j, k)];
-  dyy += T(1) * y[lin(d, i, jp, k)];
-  dzz += T(1) * y[lin(d, i, j, kp)];
-  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
+  return Nb7<T>{y[lin(d, i, j, k)],  y[lin(d, im, j, k)], y[lin(d, ip, j, k)],
+                y[lin(d, i, jm, k)], y[lin(d, i, jp, k)], y[lin(d, i, j, km)],
+                y[lin(d, i, j, kp)]};
 }
 
 // Cahn-Hilliard: M * lap(mu), mu = -y + y^3 - kappa*lap(y)
-// (examples/common/cahn_hilliard.hpp:41-57).
+// (examples/common/cahn_hilliard.hpp:41-57; arithmetic in gc_arith.hpp).
 template <class T>
 struct CHRhs {
-  T M, kappa, ihx2, ihy2, ihz2;
+  CHCoef<T> cf;
   __device__ __forceinline__ T mu(const T* __restrict__ y, const Dims& d, int i, int j,
                                   int k) const {
-    const T l = lap2_at(y, d, i, j, k, ihx2, ihy2, ihz2);
-    const T p = y[lin(d, i, j, k)];
-    return -p + p * p * p - kappa * l;
+    const Nb7<T> n = nb7_at(y, d, i, j, k);
+    return ch_mu(cf, n.c, n.xm, n.xp, n.ym, n.yp, n.zm, n.zp);
   }
   __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
                                           int k) const {
@@ -114,28 +109,18 @@ struct CHRhs {
     const T mxm = mu(y, d, wrap(i - 1, d.nx), j, k), mxp = mu(y, d, wrap(i + 1, d.nx), j, k);
     const T mym = mu(y, d, i, wrap(j - 1, d.ny), k), myp = mu(y, d, i, wrap(j + 1, d.ny), k);
     const T mzm = mu(y, d, i, j, wrap(k - 1, d.nz)), mzp = mu(y, d, i, j, wrap(k + 1, d.nz));
-    T dxx = 0, dyy = 0, dzz = 0;
-    dxx += T(1) * mxm;
-    dyy += T(1) * mym;
-    dzz += T(1) * mzm;
-    dxx += T(-2) * mc;
-    dyy += T(-2) * mc;
-    dzz += T(-2) * mc;
-    dxx += T(1) * mxp;
-    dyy += T(1) * myp;
-    dzz += T(1) * mzp;
-    const T lap = dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
-    return lap * M;
+    return ch_k(cf, mc, mxm, mxp, mym, myp, mzm, mzp);
   }
 };
 
 // solve/Diffusion.hpp:108 -- laplacian<2>
 template <class T>
 struct LapRhs {
-  T ihx2, ihy2, ihz2;
+  LapCoef<T> lc;
   __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
                                           int k) const {
-    return lap2_at(y, d, i, j, k, ihx2, ihy2, ihz2);
+    const Nb7<T> n = nb7_at(y, d, i, j, k);
+    return lap7(lc, n.c, n.xm, n.xp, n.ym, n.yp, n.zm, n.zp);
   }
 };
 
@@ -143,22 +128,18 @@ struct LapRhs {
 template <class T>
 struct FvmRhs {
   const T* __restrict__ D;
-  T ihx2, ihy2, ihz2;
-  __device__ __forceinline__ static T hm(T a, T b) { return T(2) * a * b / (a + b); }
+  LapCoef<T> lc;
   __device__ __forceinline__ T operator()(const T* __restrict__ y, const Dims& d, int i, int j,
                                           int k) const {
     const size_t c = lin(d, i, j, k);
     const size_t xp = lin(d, wrap(i + 1, d.nx), j, k), xm = lin(d, wrap(i - 1, d.nx), j, k);
     const size_t yp = lin(d, i, wrap(j + 1, d.ny), k), ym = lin(d, i, wrap(j - 1, d.ny), k);
     const size_t zp = lin(d, i, j, wrap(k + 1, d.nz)), zm = lin(d, i, j, wrap(k - 1, d.nz));
-    const T pc = y[c], Dc = D[c];
-    const T dxp = hm(Dc, D[xp]), dxm = hm(D[xm], Dc);
-    const T dyp = hm(Dc, D[yp]), dym = hm(D[ym], Dc);
-    const T dzp = hm(Dc, D[zp]), dzm = hm(D[zm], Dc);
-    const T rx = (dxp * (y[xp] - pc) - dxm * (pc - y[xm])) * ihx2;
-    const T ry = (dyp * (y[yp] - pc) - dym * (pc - y[ym])) * ihy2;
-    const T rz = (dzp * (y[zp] - pc) - dzm * (pc - y[zm])) * ihz2;
-    return rx + ry + rz;
+    const T Dc = D[c];
+    return fvm_k(y[c], y[xm], y[xp], y[ym], y[yp], y[zm], y[zp], face_coef(D[xm], Dc, lc.ihx2),
+                 face_coef(Dc, D[xp], lc.ihx2), face_coef(D[ym], Dc, lc.ihy2),
+                 face_coef(Dc, D[yp], lc.ihy2), face_coef(D[zm], Dc, lc.ihz2),
+                 face_coef(Dc, D[zp], lc.ihz2), lc.ihx2, lc.ihy2, lc.ihz2);
   }
 };
 
@@ -182,21 +163,21 @@ __device__ __forceinline__ void stage_update_regs(int stage, size_t c, T kv, T p
       break;
     case kS1:
       a[c] = kv;
-      ynext[c] = pv + co.half_dt * kv;
+      ynext[c] = stage_axpy(pv, co.half_dt, kv);
       break;
     case kS2:
       b[c] = kv;
-      ynext[c] = pv + co.half_dt * kv;
+      ynext[c] = stage_axpy(pv, co.half_dt, kv);
       break;
     case kS3:
       b[c] = bv + kv;
-      ynext[c] = pv + co.dt * kv;
+      ynext[c] = stage_axpy(pv, co.dt, kv);
       break;
     case kS4:
-      psi[c] = pv + (co.sixth_dt * (av + kv) + co.third_dt * bv);
+      psi[c] = rk4_final(pv, av, bv, kv, co.sixth_dt, co.third_dt);
       break;
     default:  // explicit Euler: ynext = y + dt*f(y), pv = y[c]
-      ynext[c] = pv + co.dt * kv;
+      ynext[c] = stage_axpy(pv, co.dt, kv);
       break;
   }
 }
@@ -241,30 +222,10 @@ __global__ __launch_bounds__(256) void k_stage(Rhs rhs, int stage, const T* __re
 // 2-D (Nz = 1): the same with a single plane; the degenerate z terms of both Laplacians
 // are formed from the centre value, exactly as the reference's wrapped access does.
 
-template <class T>
-struct CHCoef {
-  T M, kappa, ihx2, ihy2, ihz2;
-};
-
-// Order-2 Laplacian in reference order from LDS: x/y neighbours in plane `c`, z
-// neighbours from planes `lo` (k-1) and `hi` (k+1) at the same offset.
-template <class T>
-__device__ __forceinline__ T lap_lds(const T* __restrict__ pc, const T* __restrict__ plo,
-                                     const T* __restrict__ phi, int o, int W, T ihx2, T ihy2,
-                                     T ihz2) {
-  const T cc = pc[o];
-  T dxx = 0, dyy = 0, dzz = 0;
-  dxx += T(1) * pc[o - 1];
-  dyy += T(1) * pc[o - W];
-  dzz += T(1) * plo[o];
-  dxx += T(-2) * cc;
-  dyy += T(-2) * cc;
-  dzz += T(-2) * cc;
-  dxx += T(1) * pc[o + 1];
-  dyy += T(1) * pc[o + W];
-  dzz += T(1) * phi[o];
-  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
-}
+// The order-2 stencil around LDS offset o: in-plane neighbours from plane `c` (row width W),
+// z neighbours from planes `lo` (k-1) and `hi` (k+1) at the same offset.
+#define GC_NB_LDS(pc, plo, phi, o, W) \
+  (pc)[o], (pc)[(o) - 1], (pc)[(o) + 1], (pc)[(o) - (W)], (pc)[(o) + (W)], (plo)[o], (phi)[o]
 
 template <class T, int OX, int OY, int ZC>
 struct Ch3Cfg {
@@ -296,17 +257,17 @@ __device__ __forceinline__ void stage_out(unsigned c, T kv, T pv, T av, T bv, T*
     __builtin_nontemporal_store(kv, ynext + c);
   } else if constexpr (STAGE == kS1) {
     __builtin_nontemporal_store(kv, a + c);
-    __builtin_nontemporal_store(pv + co.half_dt * kv, ynext + c);
+    __builtin_nontemporal_store(stage_axpy(pv, co.half_dt, kv), ynext + c);
   } else if constexpr (STAGE == kS2) {
     __builtin_nontemporal_store(kv, b + c);
-    __builtin_nontemporal_store(pv + co.half_dt * kv, ynext + c);
+    __builtin_nontemporal_store(stage_axpy(pv, co.half_dt, kv), ynext + c);
   } else if constexpr (STAGE == kS3) {
     __builtin_nontemporal_store(bv + kv, b + c);
-    __builtin_nontemporal_store(pv + co.dt * kv, ynext + c);
+    __builtin_nontemporal_store(stage_axpy(pv, co.dt, kv), ynext + c);
   } else if constexpr (STAGE == kS4) {
-    __builtin_nontemporal_store(pv + (co.sixth_dt * (av + kv) + co.third_dt * bv), psi + c);
+    __builtin_nontemporal_store(rk4_final(pv, av, bv, kv, co.sixth_dt, co.third_dt), psi + c);
   } else {  // explicit Euler: ynext = y + dt*f(y)
-    __builtin_nontemporal_store(pv + co.dt * kv, ynext + c);
+    __builtin_nontemporal_store(stage_axpy(pv, co.dt, kv), ynext + c);
   }
 }
 
@@ -408,9 +369,7 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __res
         if (s + 1 < C::ML || idx < C::MN) {
           const int mx = idx % C::MW, my = idx / C::MW;
           const int o = (my + 1) * C::YW + (mx + 1);
-          const T l = lap_lds(yc, yb, ya, o, C::YW, cf.ihx2, cf.ihy2, cf.ihz2);
-          const T pp = yc[o];
-          mw[idx] = -pp + pp * pp * pp - cf.kappa * l;
+          mw[idx] = ch_mu(cf, GC_NB_LDS(yc, yb, ya, o, C::YW));
         }
       }
     }
@@ -425,7 +384,7 @@ __global__ __launch_bounds__(256) void k_ch_stage3d(CHCoef<T> cf, const T* __res
         if (oin[q]) {
           const int oy = oy0 + q * ROWS;
           const int o = (oy + 1) * C::MW + (ox + 1);
-          const T kv = lap_lds(mc, mb, ma, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const T kv = ch_k(cf, GC_NB_LDS(mc, mb, ma, o, C::MW));
           stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi + kb, ynext + kb, a + kb,
                            b + kb, co);
         }
@@ -475,10 +434,8 @@ __global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, const T* __res
     if (s + 1 < C::ML || idx < C::MN) {
       const int mx = idx % C::MW, my = idx / C::MW;
       const int o = (my + 1) * C::YW + (mx + 1);
-      // Nz = 1: the z neighbours are the centre itself
-      const T l = lap_lds(ys, ys, ys, o, C::YW, cf.ihx2, cf.ihy2, cf.ihz2);
-      const T pp = ys[o];
-      ms[idx] = -pp + pp * pp * pp - cf.kappa * l;
+      // Nz = 1 (ch_mu2: the z neighbours are the centre itself)
+      ms[idx] = ch_mu2(cf, ys[o], ys[o - 1], ys[o + 1], ys[o - C::YW], ys[o + C::YW]);
     }
   }
   __syncthreads();
@@ -487,7 +444,7 @@ __global__ __launch_bounds__(256) void k_ch_stage2d(CHCoef<T> cf, const T* __res
     if (oin[q]) {
       const int oy = oy0 + q * ROWS;
       const int o = (oy + 1) * C::MW + (ox + 1);
-      const T kv = lap_lds(ms, ms, ms, o, C::MW, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+      const T kv = ch_k2(cf, ms[o], ms[o - 1], ms[o + 1], ms[o - C::MW], ms[o + C::MW]);
       if constexpr (base_from_y<STAGE>()) pv[q] = ys[(oy + 2) * C::YW + (ox + 2)];
       stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi, ynext, a, b, co);
     }
@@ -516,8 +473,8 @@ gc_status launch_stage_untiled(gc_context* ctx, const gc_grid* g, const Rhs& rhs
 // LapRhs / FvmRhs.
 
 template <class T, int OX, int OY, int ZC, int STAGE, bool FVM>
-__global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfield, T ihx2,
-                                                      T ihy2, T ihz2, const T* __restrict__ y,
+__global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfield,
+                                                      LapCoef<T> lc, const T* __restrict__ y,
                                                       T* psi, T* __restrict__ ynext,
                                                       T* __restrict__ a, T* __restrict__ b,
                                                       Dims d, int ntx, int nty,
@@ -598,16 +555,13 @@ __global__ __launch_bounds__(256) void k_diff_stage3d(const T* __restrict__ Dfie
           T kv;
           if constexpr (FVM) {
             const T* dc = ds[s1];
-            const T pc = yc[o], Dc = dc[o];
-            const T dxp = FvmRhs<T>::hm(Dc, dc[o + 1]), dxm = FvmRhs<T>::hm(dc[o - 1], Dc);
-            const T dyp = FvmRhs<T>::hm(Dc, dc[o + PW]), dym = FvmRhs<T>::hm(dc[o - PW], Dc);
-            const T dzp = FvmRhs<T>::hm(Dc, ds[s0][o]), dzm = FvmRhs<T>::hm(ds[s2][o], Dc);
-            const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
-            const T ry = (dyp * (yc[o + PW] - pc) - dym * (pc - yc[o - PW])) * ihy2;
-            const T rz = (dzp * (yhi[o] - pc) - dzm * (pc - ylo[o])) * ihz2;
-            kv = rx + ry + rz;
+            const T Dc = dc[o];
+            kv = fvm_k(GC_NB_LDS(yc, ylo, yhi, o, PW), face_coef(dc[o - 1], Dc, lc.ihx2),
+                       face_coef(Dc, dc[o + 1], lc.ihx2), face_coef(dc[o - PW], Dc, lc.ihy2),
+                       face_coef(Dc, dc[o + PW], lc.ihy2), face_coef(ds[s2][o], Dc, lc.ihz2),
+                       face_coef(Dc, ds[s0][o], lc.ihz2), lc.ihx2, lc.ihy2, lc.ihz2);
           } else {
-            kv = lap_lds(yc, ylo, yhi, o, PW, ihx2, ihy2, ihz2);
+            kv = lap7(lc, GC_NB_LDS(yc, ylo, yhi, o, PW));
           }
           if constexpr (base_from_y<STAGE>()) pv[q] = yc[o];
           stage_out<STAGE>(ooff[q], kv, pv[q], av[q], bv[q], psi + kb, ynext + kb, a + kb,
@@ -657,28 +611,9 @@ __device__ __forceinline__ void ring1_site(int r, int& mx, int& my) {
   }
 }
 
-// right-hand side at LDS offset o (row width W) from y planes (lo, c, hi) and D planes
-template <class T, bool FVM>
-__device__ __forceinline__ T diff_rhs(const T* yc, const T* ylo, const T* yhi, const T* dc,
-                                      const T* dlo, const T* dhi, int o, int W, T ihx2, T ihy2,
-                                      T ihz2) {
-  if constexpr (FVM) {
-    const T pc = yc[o], Dc = dc[o];
-    const T dxp = FvmRhs<T>::hm(Dc, dc[o + 1]), dxm = FvmRhs<T>::hm(dc[o - 1], Dc);
-    const T dyp = FvmRhs<T>::hm(Dc, dc[o + W]), dym = FvmRhs<T>::hm(dc[o - W], Dc);
-    const T dzp = FvmRhs<T>::hm(Dc, dhi[o]), dzm = FvmRhs<T>::hm(dlo[o], Dc);
-    const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
-    const T ry = (dyp * (yc[o + W] - pc) - dym * (pc - yc[o - W])) * ihy2;
-    const T rz = (dzp * (yhi[o] - pc) - dzm * (pc - ylo[o])) * ihz2;
-    return rx + ry + rz;
-  } else {
-    return lap_lds(yc, ylo, yhi, o, W, ihx2, ihy2, ihz2);
-  }
-}
-
 template <class T, int OX, int OY, int ZC, int NT, int PAIR, bool FVM>
-__global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield, T ihx2, T ihy2,
-                                                    T ihz2, const T* __restrict__ y,
+__global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield, LapCoef<T> lc,
+                                                    const T* __restrict__ y,
                                                     const T* __restrict__ psi,
                                                     T* __restrict__ out, T* __restrict__ a,
                                                     T* __restrict__ b, Dims d, int ntx, int nty,
@@ -792,21 +727,21 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
         const int idx = t + s * NT;
         if (idx < C::N2) {
           const int mx = idx % C::W2, my = idx / C::W2;
-          if (mx < C::W2 - 1) fx[s0][idx] = FvmRhs<T>::hm(dp[idx], dp[idx + 1]);
-          if (my < C::H2 - 1) fy[s0][idx] = FvmRhs<T>::hm(dp[idx], dp[idx + C::W2]);
+          if (mx < C::W2 - 1) fx[s0][idx] = face_coef(dp[idx], dp[idx + 1], lc.ihx2);
+          if (my < C::H2 - 1) fy[s0][idx] = face_coef(dp[idx], dp[idx + C::W2], lc.ihy2);
         }
       }
       if (p == 1) {  // z-faces of plane z0-2 (under the first k_A plane)
 #pragma unroll
         for (int q = 0; q < C::OR; ++q) {
           const int e2 = (oy0 + q * C::ROWS + 2) * C::W2 + (ox + 2);
-          fzt0[q] = FvmRhs<T>::hm(ds[d1][e2], ds[d0][e2]);
+          fzt0[q] = face_coef(ds[d1][e2], ds[d0][e2], lc.ihz2);
         }
 #pragma unroll
         for (int s = 0; s < C::LR; ++s)
           if (rmx[s] >= 0) {
             const int e2 = (rmy[s] + 1) * C::W2 + (rmx[s] + 1);
-            fzr[s] = FvmRhs<T>::hm(ds[d1][e2], ds[d0][e2]);
+            fzr[s] = face_coef(ds[d1][e2], ds[d0][e2], lc.ihz2);
           }
       }
       __syncthreads();
@@ -818,16 +753,10 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
       auto kA = [&](int mx, int my, T fzp, T fzm) {
         const int o = (my + 1) * C::W2 + (mx + 1);
         if constexpr (FVM) {
-          const T* yc = ys[s1];
-          const T pc = yc[o];
-          const T dxp = fx[s1][o], dxm = fx[s1][o - 1];
-          const T dyp = fy[s1][o], dym = fy[s1][o - C::W2];
-          const T rx = (dxp * (yc[o + 1] - pc) - dxm * (pc - yc[o - 1])) * ihx2;
-          const T ry = (dyp * (yc[o + C::W2] - pc) - dym * (pc - yc[o - C::W2])) * ihy2;
-          const T rz = (fzp * (ys[s0][o] - pc) - fzm * (pc - ys[s2][o])) * ihz2;
-          return rx + ry + rz;
+          return fvm_k(GC_NB_LDS(ys[s1], ys[s2], ys[s0], o, C::W2), fx[s1][o - 1], fx[s1][o],
+                       fy[s1][o - C::W2], fy[s1][o], fzm, fzp, lc.ihx2, lc.ihy2, lc.ihz2);
         } else {
-          return lap_lds(ys[s1], ys[s2], ys[s0], o, C::W2, ihx2, ihy2, ihz2);
+          return lap7(lc, GC_NB_LDS(ys[s1], ys[s2], ys[s0], o, C::W2));
         }
       };
 #pragma unroll
@@ -836,7 +765,7 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
         T fzp = T(0), fzm = T(0);
         if constexpr (FVM) {
           const int o = (my + 1) * C::W2 + (mx + 1);
-          fzp = FvmRhs<T>::hm(ds[d1][o], ds[d0][o]);  // planes zA, zA+1
+          fzp = face_coef(ds[d1][o], ds[d0][o], lc.ihz2);  // planes zA, zA+1
           fzm = fzt0[q];
           fzt2[q] = fzt1[q];
           fzt1[q] = fzt0[q];
@@ -845,11 +774,11 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
         const T kv = kA(mx, my, fzp, fzm);
         const int e = my * C::W1 + mx;
         if constexpr (PAIR == 0) {
-          y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
+          y2s[s0][e] = stage_axpy(ys[s1][(my + 1) * C::W2 + (mx + 1)], co.half_dt, kv);
           if (own && oin[q])
             __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + ooff[q]);
         } else {
-          y2s[s0][e] = pt[q] + co.dt * kv;
+          y2s[s0][e] = stage_axpy(pt[q], co.dt, kv);
           k1r[q] = k0r[q];
           k0r[q] = kv;
         }
@@ -861,14 +790,14 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           T fzp = T(0), fzm = T(0);
           if constexpr (FVM) {
             const int o = (my + 1) * C::W2 + (mx + 1);
-            fzp = FvmRhs<T>::hm(ds[d1][o], ds[d0][o]);
+            fzp = face_coef(ds[d1][o], ds[d0][o], lc.ihz2);
             fzm = fzr[s];
             fzr[s] = fzp;
           }
           const T kv = kA(mx, my, fzp, fzm);
           const int e = my * C::W1 + mx;
-          if constexpr (PAIR == 0) y2s[s0][e] = ys[s1][(my + 1) * C::W2 + (mx + 1)] + co.half_dt * kv;
-          else y2s[s0][e] = pe[s] + co.dt * kv;
+          if constexpr (PAIR == 0) y2s[s0][e] = stage_axpy(ys[s1][(my + 1) * C::W2 + (mx + 1)], co.half_dt, kv);
+          else y2s[s0][e] = stage_axpy(pe[s], co.dt, kv);
         }
       }
       if constexpr (PAIR == 0) {  // base psi of the output plane z0-4+p (input plane p-2)
@@ -890,23 +819,18 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           T kv;
           if constexpr (FVM) {
             const int o2 = (oy + 2) * C::W2 + (ox + 2);  // the same site in the face planes
-            const T* yc = y2s[s1];
-            const T pc = yc[o1];
-            const T dxp = fx[s2][o2], dxm = fx[s2][o2 - 1];
-            const T dyp = fy[s2][o2], dym = fy[s2][o2 - C::W2];
-            const T rx = (dxp * (yc[o1 + 1] - pc) - dxm * (pc - yc[o1 - 1])) * ihx2;
-            const T ry = (dyp * (yc[o1 + C::W1] - pc) - dym * (pc - yc[o1 - C::W1])) * ihy2;
-            const T rz = (fzt1[q] * (y2s[s0][o1] - pc) - fzt2[q] * (pc - y2s[s2][o1])) * ihz2;
-            kv = rx + ry + rz;
+            kv = fvm_k(GC_NB_LDS(y2s[s1], y2s[s2], y2s[s0], o1, C::W1), fx[s2][o2 - 1], fx[s2][o2],
+                       fy[s2][o2 - C::W2], fy[s2][o2], fzt2[q], fzt1[q], lc.ihx2, lc.ihy2,
+                       lc.ihz2);
           } else {
-            kv = lap_lds(y2s[s1], y2s[s2], y2s[s0], o1, C::W1, ihx2, ihy2, ihz2);
+            kv = lap7(lc, GC_NB_LDS(y2s[s1], y2s[s2], y2s[s0], o1, C::W1));
           }
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + ooff[q]);
-            __builtin_nontemporal_store(pv[q] + co.half_dt * kv, out + kb + ooff[q]);
+            __builtin_nontemporal_store(stage_axpy(pv[q], co.half_dt, kv), out + kb + ooff[q]);
           } else {
             const T bsum = bv[q] + k1r[q];  // k3 of this plane (k_A pass of iteration p-1)
-            __builtin_nontemporal_store(pv[q] + (co.sixth_dt * (av[q] + kv) + co.third_dt * bsum),
+            __builtin_nontemporal_store(rk4_final(pv[q], av[q], bsum, kv, co.sixth_dt, co.third_dt),
                                         out + kb + ooff[q]);
           }
         }
@@ -930,7 +854,7 @@ constexpr int kDiffOX = 64, kDiffOY = 8, kDiffZC = 32;
 
 // Grids with Nz >= 4 run the tiled kernel; flatter grids the untiled one.
 template <class T, bool FVM>
-gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, T ihx2, T ihy2, T ihz2,
+gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, const LapCoef<T>& lc,
                             int stage, const T* y, T* psi, T* ynext, T* a, T* b,
                             const StageCoef<T>& co) {
   const Dims d = dims_of(g);
@@ -939,7 +863,7 @@ gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, T ihx
   const dim3 grid((unsigned)(ntx * nty * nzc));
 #define GC_LD(S)                                                                               \
   hipLaunchKernelGGL((k_diff_stage3d<T, kDiffOX, kDiffOY, kDiffZC, S, FVM>), grid, dim3(256), 0, \
-                     ctx->stream, D, ihx2, ihy2, ihz2, y, psi, ynext, a, b, d, ntx, nty, co)
+                     ctx->stream, D, lc, y, psi, ynext, a, b, d, ntx, nty, co)
   GC_STAGE_SWITCH(GC_LD)
 #undef GC_LD
   GC_LAUNCHED();
@@ -951,17 +875,17 @@ gc_status launch_diff_stage(gc_context* ctx, const gc_grid* g, const T* D, T ihx
 constexpr int kDpOX = 64, kDpOY = 16, kDpZC = 32, kDpNT = 1024;
 
 template <class T, bool FVM>
-gc_status diff_pair_step(gc_context* ctx, const gc_grid* g, const T* D, T ihx2, T ihy2, T ihz2,
+gc_status diff_pair_step(gc_context* ctx, const gc_grid* g, const T* D, const LapCoef<T>& lc,
                          const T* psi_in, T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
   const Dims d = dims_of(g);
   const int ntx = (d.nx + kDpOX - 1) / kDpOX, nty = (d.ny + kDpOY - 1) / kDpOY;
   const int nzc = (d.nz + kDpZC - 1) / kDpZC;
   const dim3 grid((unsigned)(ntx * nty * nzc));
   hipLaunchKernelGGL((k_diff_pair3d<T, kDpOX, kDpOY, kDpZC, kDpNT, 0, FVM>), grid, dim3(kDpNT), 0,
-                     ctx->stream, D, ihx2, ihy2, ihz2, psi_in, psi_in, y3, a, b, d, ntx, nty, co);
+                     ctx->stream, D, lc, psi_in, psi_in, y3, a, b, d, ntx, nty, co);
   GC_LAUNCHED();
   hipLaunchKernelGGL((k_diff_pair3d<T, kDpOX, kDpOY, kDpZC, kDpNT, 1, FVM>), grid, dim3(kDpNT), 0,
-                     ctx->stream, D, ihx2, ihy2, ihz2, y3, psi_in, psi_out, a, b, d, ntx, nty, co);
+                     ctx->stream, D, lc, y3, psi_in, psi_out, a, b, d, ntx, nty, co);
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -969,31 +893,27 @@ gc_status diff_pair_step(gc_context* ctx, const gc_grid* g, const T* D, T ihx2, 
 template <class T>
 gc_status pair_step(gc_context* ctx, const gc_grid* g, const LapRhs<T>& rhs, const T* psi_in,
                     T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
-  return diff_pair_step<T, false>(ctx, g, nullptr, rhs.ihx2, rhs.ihy2, rhs.ihz2, psi_in, psi_out,
-                                  y3, a, b, co);
+  return diff_pair_step<T, false>(ctx, g, nullptr, rhs.lc, psi_in, psi_out, y3, a, b, co);
 }
 
 template <class T>
 gc_status pair_step(gc_context* ctx, const gc_grid* g, const FvmRhs<T>& rhs, const T* psi_in,
                     T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
-  return diff_pair_step<T, true>(ctx, g, rhs.D, rhs.ihx2, rhs.ihy2, rhs.ihz2, psi_in, psi_out, y3,
-                                 a, b, co);
+  return diff_pair_step<T, true>(ctx, g, rhs.D, rhs.lc, psi_in, psi_out, y3, a, b, co);
 }
 
 template <class T>
 gc_status launch_stage(gc_context* ctx, const gc_grid* g, const LapRhs<T>& rhs, int stage,
                        const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
   if (g->nz < 4) return launch_stage_untiled<T>(ctx, g, rhs, stage, y, psi, ynext, a, b, co);
-  return launch_diff_stage<T, false>(ctx, g, nullptr, rhs.ihx2, rhs.ihy2, rhs.ihz2, stage, y, psi,
-                                     ynext, a, b, co);
+  return launch_diff_stage<T, false>(ctx, g, nullptr, rhs.lc, stage, y, psi, ynext, a, b, co);
 }
 
 template <class T>
 gc_status launch_stage(gc_context* ctx, const gc_grid* g, const FvmRhs<T>& rhs, int stage,
                        const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co) {
   if (g->nz < 4) return launch_stage_untiled<T>(ctx, g, rhs, stage, y, psi, ynext, a, b, co);
-  return launch_diff_stage<T, true>(ctx, g, rhs.D, rhs.ihx2, rhs.ihy2, rhs.ihz2, stage, y, psi,
-                                    ynext, a, b, co);
+  return launch_diff_stage<T, true>(ctx, g, rhs.D, rhs.lc, stage, y, psi, ynext, a, b, co);
 }
 
 // ---- two RK4 stages per pass (3-D) ---------------------------------------------------------
@@ -1178,9 +1098,7 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       for (int s = 0; s < C::L3; ++s) {
         const int o = oA[s];
         if (o >= 0) {
-          const T l = lap_lds(ys[s1], ys[s2], ys[s0], o, C::W4, cf.ihx2, cf.ihy2, cf.ihz2);
-          const T yy = ys[s1][o];
-          mas[s0][t + s * NT] = -yy + yy * yy * yy - cf.kappa * l;
+          mas[s0][t + s * NT] = ch_mu(cf, GC_NB_LDS(ys[s1], ys[s2], ys[s0], o, C::W4));
         }
       }
     }
@@ -1190,7 +1108,7 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       const bool own = zA >= z0 && zA < z0 + nout;
       auto kA = [&](int mx, int my) {
         const int o = (my + 1) * C::W3 + (mx + 1);
-        return lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+        return ch_k(cf, GC_NB_LDS(mas[s1], mas[s2], mas[s0], o, C::W3));
       };
       // tile sites: the output owner
 #pragma unroll
@@ -1199,11 +1117,11 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
         const T kv = kA(mx, my);
         const int e = my * C::W2 + mx;
         if constexpr (PAIR == 0) {
-          y2s[s0][e] = ys[s2][(my + 2) * C::W4 + (mx + 2)] + co.half_dt * kv;
+          y2s[s0][e] = stage_axpy(ys[s2][(my + 2) * C::W4 + (mx + 2)], co.half_dt, kv);
           if (own && oin[q])
             __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + ooff[q]);
         } else {
-          y2s[s0][e] = pt[q] + co.dt * kv;
+          y2s[s0][e] = stage_axpy(pt[q], co.dt, kv);
           k2r[q] = k1r[q];
           k1r[q] = k0r[q];
           k0r[q] = kv;
@@ -1214,10 +1132,10 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       for (int s = 0; s < C::LR; ++s) {
         const int o = rk[s];
         if (o >= 0) {
-          const T kv = lap_lds(mas[s1], mas[s2], mas[s0], o, C::W3, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const T kv = ch_k(cf, GC_NB_LDS(mas[s1], mas[s2], mas[s0], o, C::W3));
           const int e = re[s];
-          if constexpr (PAIR == 0) y2s[s0][e] = ys[s2][rb[s]] + co.half_dt * kv;
-          else y2s[s0][e] = pe[s] + co.dt * kv;
+          if constexpr (PAIR == 0) y2s[s0][e] = stage_axpy(ys[s2][rb[s]], co.half_dt, kv);
+          else y2s[s0][e] = stage_axpy(pe[s], co.dt, kv);
         }
       }
     }
@@ -1227,9 +1145,7 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
       for (int s = 0; s < C::L1; ++s) {
         const int o = oB[s];
         if (o >= 0) {
-          const T l = lap_lds(y2s[s1], y2s[s2], y2s[s0], o, C::W2, cf.ihx2, cf.ihy2, cf.ihz2);
-          const T yy = y2s[s1][o];
-          mbs[s0][t + s * NT] = -yy + yy * yy * yy - cf.kappa * l;
+          mbs[s0][t + s * NT] = ch_mu(cf, GC_NB_LDS(y2s[s1], y2s[s2], y2s[s0], o, C::W2));
         }
       }
     }
@@ -1241,13 +1157,13 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
         if (oin[q]) {
           const int oy = oy0 + q * C::ROWS;
           const int o = (oy + 1) * C::W1 + (ox + 1);
-          const T kv = lap_lds(mbs[s1], mbs[s2], mbs[s0], o, C::W1, cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const T kv = ch_k(cf, GC_NB_LDS(mbs[s1], mbs[s2], mbs[s0], o, C::W1));
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + ooff[q]);
-            __builtin_nontemporal_store(pv[q] + co.half_dt * kv, out + kb + ooff[q]);
+            __builtin_nontemporal_store(stage_axpy(pv[q], co.half_dt, kv), out + kb + ooff[q]);
           } else {
             const T bsum = bv[q] + k2r[q];  // k3 of this plane (k_A pass of iteration p-2)
-            __builtin_nontemporal_store(pv[q] + (co.sixth_dt * (av[q] + kv) + co.third_dt * bsum),
+            __builtin_nontemporal_store(rk4_final(pv[q], av[q], bsum, kv, co.sixth_dt, co.third_dt),
                                         out + kb + ooff[q]);
           }
         }
@@ -1272,21 +1188,10 @@ __global__ __launch_bounds__(NT) void k_ch_pair3d(CHCoef<T> cf, const T* __restr
 // Each LDS plane is written in the phase after its last reader and read after the next
 // barrier, so one slot per quantity suffices.  The arithmetic (operands, order) is that of
 // k_ch_pair3d.
-template <class T>
-__device__ __forceinline__ T lap_q(const T* __restrict__ pl, int o, int W, T cc, T zlo, T zhi,
-                                   T ihx2, T ihy2, T ihz2) {
-  T dxx = 0, dyy = 0, dzz = 0;
-  dxx += T(1) * pl[o - 1];
-  dyy += T(1) * pl[o - W];
-  dzz += T(1) * zlo;
-  dxx += T(-2) * cc;
-  dyy += T(-2) * cc;
-  dzz += T(-2) * cc;
-  dxx += T(1) * pl[o + 1];
-  dyy += T(1) * pl[o + W];
-  dzz += T(1) * zhi;
-  return dxx * ihx2 + dyy * ihy2 + dzz * ihz2;
-}
+// The order-2 stencil with the centre and z neighbours from registers (cc, zlo, zhi) and the
+// in-plane neighbours from LDS plane `pl` (row width W) around offset o.
+#define GC_NB_Q(pl, o, W, cc, zlo, zhi) \
+  cc, (pl)[(o) - 1], (pl)[(o) + 1], (pl)[(o) - (W)], (pl)[(o) + (W)], zlo, zhi
 
 template <class T, int OX, int OY, int ZC, int NT, int PAIR>
 __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __restrict__ y,
@@ -1332,6 +1237,15 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   T yn[L] = {};  // the next plane of y, in flight
   T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
   T pv[L] = {}, av[L] = {}, bv[L] = {};
+  // Phase Y's point operands are requested one iteration ahead (after that iteration's y
+  // plane, so waiting for them never waits for the newer y plane): they have a whole
+  // iteration to arrive instead of phase X only.  The S3+S4 kernels of FP64 and of tiles
+  // with more than three columns per thread have no registers to spare for them (they
+  // would spill) and request them in the iteration that uses them.
+  constexpr bool kAhead = PAIR == 0 || (sizeof(T) == 4 && L <= 3);
+  T pen[PAIR == 1 && kAhead ? L : 1] = {};
+  T pvn[kAhead ? L : 1] = {}, avn[kAhead && PAIR == 1 ? L : 1] = {},
+    bvn[kAhead && PAIR == 1 ? L : 1] = {};
 
   auto load_y = [&](int plane) {
     const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
@@ -1345,7 +1259,18 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
 #pragma unroll
     for (int s = 0; s < L; ++s) yq[R0][s] = yn[s];
     if (p < last) load_y(z0 - 3 + p);
-    if constexpr (PAIR == 1) {
+    if constexpr (PAIR == 1 && kAhead) {
+      if (p >= 4) {
+#pragma unroll
+        for (int s = 0; s < L; ++s) pe[s] = pen[s];
+      }
+      if (p >= 3 && p < last) {  // psi on P_{p-1}, for iteration p+1
+        const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 5 + p, d.nz);
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+          if (dist[s] >= 2) pen[s] = pp[goff[s]];
+      }
+    } else if constexpr (PAIR == 1) {
       if (p >= 4) {
         const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 6 + p, d.nz);
 #pragma unroll
@@ -1353,16 +1278,38 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           if (dist[s] >= 2) pe[s] = pp[goff[s]];
       }
     }
-    if (p >= 9) {
+    if constexpr (kAhead) {
+      if (p >= 9) {
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+          pv[s] = pvn[s];
+          if constexpr (PAIR == 1) {
+            av[s] = avn[s];
+            bv[s] = bvn[s];
+          }
+        }
+      }
+      if (p >= 8 && p < last) {  // the outputs of iteration p+1: plane z0-8+p
+        const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+          if (oin[s]) {
+            pvn[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
+            if constexpr (PAIR == 1) {
+              avn[s] = __builtin_nontemporal_load(a + kb + goff[s]);
+              bvn[s] = __builtin_nontemporal_load(b + kb + goff[s]);
+            }
+          }
+        }
+      }
+    } else if (p >= 9) {
       const size_t kb = (size_t)d.plane * (size_t)(z0 - 9 + p);
 #pragma unroll
       for (int s = 0; s < L; ++s) {
         if (oin[s]) {
           pv[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
-          if constexpr (PAIR == 1) {
-            av[s] = __builtin_nontemporal_load(a + kb + goff[s]);
-            bv[s] = __builtin_nontemporal_load(b + kb + goff[s]);
-          }
+          av[s] = __builtin_nontemporal_load(a + kb + goff[s]);
+          bv[s] = __builtin_nontemporal_load(b + kb + goff[s]);
         }
       }
     }
@@ -1373,16 +1320,10 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
       const int o = t + s * NT;
       if (p >= 3 && dist[s] >= 1) ms[o] = mq[R1][s];
       if (p >= 8 && dist[s] >= 3) bs[o] = bq[R1][s];
-      if (p >= 2 && dist[s] >= 1) {
-        const T yy = yq[R1][s];
-        const T l = lap_q(ys, o, W, yy, yq[R2][s], yq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2);
-        mq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
-      }
-      if (p >= 7 && dist[s] >= 3) {  // y' on P_{p-3}, P_{p-4}, P_{p-5}: slots R1, R2, R0
-        const T yy = wq[R2][s];
-        const T l = lap_q(ws, o, W, yy, wq[R0][s], wq[R1][s], cf.ihx2, cf.ihy2, cf.ihz2);
-        bq[R0][s] = -yy + yy * yy * yy - cf.kappa * l;
-      }
+      if (p >= 2 && dist[s] >= 1)
+        mq[R0][s] = ch_mu(cf, GC_NB_Q(ys, o, W, yq[R1][s], yq[R2][s], yq[R0][s]));
+      if (p >= 7 && dist[s] >= 3)  // y' on P_{p-3}, P_{p-4}, P_{p-5}: slots R1, R2, R0
+        bq[R0][s] = ch_mu(cf, GC_NB_Q(ws, o, W, wq[R2][s], wq[R0][s], wq[R1][s]));
     }
     __syncthreads();
     // Y: the outputs on P_{p-5}, k_A and y' on P_{p-2}; store y(P_p) and y'(P_{p-3}) (formed
@@ -1397,28 +1338,26 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
         if (dist[s] >= 0) ys[o] = yq[R0][s];
         if (p >= 5 && dist[s] >= 2) ws[o] = wq[R1][s];
         if (p >= 9 && oin[s]) {  // mu_B on P_{p-4}, P_{p-5}, P_{p-6}: slots R0, R1, R2
-          const T kv =
-              lap_q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const T kv = ch_k(cf, GC_NB_Q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s]));
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + goff[s]);
-            __builtin_nontemporal_store(pv[s] + co.half_dt * kv, out + kb + goff[s]);
+            __builtin_nontemporal_store(stage_axpy(pv[s], co.half_dt, kv), out + kb + goff[s]);
           } else {
             // k3 of P_{p-5}: phase Y of iteration p-3, slot R0 (read before it is refilled
             // below)
             const T bsum = bv[s] + kq[R0][s];
-            __builtin_nontemporal_store(pv[s] + (co.sixth_dt * (av[s] + kv) + co.third_dt * bsum),
+            __builtin_nontemporal_store(rk4_final(pv[s], av[s], bsum, kv, co.sixth_dt, co.third_dt),
                                         out + kb + goff[s]);
           }
         }
         if (p >= 4 && dist[s] >= 2) {
-          const T kv =
-              lap_q(ms, o, W, mq[R1][s], mq[R2][s], mq[R0][s], cf.ihx2, cf.ihy2, cf.ihz2) * cf.M;
+          const T kv = ch_k(cf, GC_NB_Q(ms, o, W, mq[R1][s], mq[R2][s], mq[R0][s]));
           if constexpr (PAIR == 0) {
-            wq[R0][s] = yq[R2][s] + co.half_dt * kv;
+            wq[R0][s] = stage_axpy(yq[R2][s], co.half_dt, kv);
             if (own && oin[s])
               __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + goff[s]);
           } else {
-            wq[R0][s] = pe[s] + co.dt * kv;
+            wq[R0][s] = stage_axpy(pe[s], co.dt, kv);
             kq[R0][s] = kv;
           }
         }
@@ -1649,7 +1588,7 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
                        const T* y, T* psi, T* ynext, T* a, T* b, const StageCoef<T>& co,
                        int cfg = -1) {
   const Dims d = dims_of(g);
-  const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
+  const CHCoef<T>& cf = rhs.cf;
   if (ch_tiled(d)) {
     int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
     if (is_pair(d, id)) id = sizeof(T) == 8 ? kStage3F64 : kStage3F32;  // one stage only
@@ -1669,7 +1608,7 @@ template <class T>
 gc_status pair_step(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, int id,
                     const T* psi_in, T* psi_out, T* y3, T* a, T* b, const StageCoef<T>& co) {
   const Dims d = dims_of(g);
-  const CHCoef<T> cf{rhs.M, rhs.kappa, rhs.ihx2, rhs.ihy2, rhs.ihz2};
+  const CHCoef<T>& cf = rhs.cf;
   launch_pair_cfg<T>(ctx, id, d, cf, 0, psi_in, psi_in, y3, a, b, co);
   GC_LAUNCHED();
   launch_pair_cfg<T>(ctx, id, d, cf, 1, y3, psi_in, psi_out, a, b, co);
@@ -1770,17 +1709,21 @@ gc_status base_checks(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void
 
 template <class T>
 CHRhs<T> make_ch(const gc_grid* g, const gc_ch_params* p) {
-  return CHRhs<T>{T(p->M), T(p->kappa), T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)),
-                  T(1.0 / (g->hz * g->hz))};
+  return CHRhs<T>{make_ch_coef<T>(T(p->M), T(p->kappa), T(1.0 / (g->hx * g->hx)),
+                                  T(1.0 / (g->hy * g->hy)), T(1.0 / (g->hz * g->hz)))};
+}
+template <class T>
+LapCoef<T> lap_coef_of(const gc_grid* g) {
+  return make_lap_coef<T>(T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)),
+                          T(1.0 / (g->hz * g->hz)));
 }
 template <class T>
 LapRhs<T> make_lap(const gc_grid* g) {
-  return LapRhs<T>{T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)), T(1.0 / (g->hz * g->hz))};
+  return LapRhs<T>{lap_coef_of<T>(g)};
 }
 template <class T>
 FvmRhs<T> make_fvm(const gc_grid* g, const void* D) {
-  return FvmRhs<T>{(const T*)D, T(1.0 / (g->hx * g->hx)), T(1.0 / (g->hy * g->hy)),
-                   T(1.0 / (g->hz * g->hz))};
+  return FvmRhs<T>{(const T*)D, lap_coef_of<T>(g)};
 }
 
 // solve/Diffusion.hpp:45-59
