@@ -14,6 +14,7 @@
 // and are bounded by HBM.
 #include "gc_common.hpp"
 #include "gc_zmarch.hpp"
+#include "gc_arith.hpp"
 
 #include <type_traits>
 #include <utility>
@@ -110,6 +111,46 @@ struct OpBih {
   }
 };
 
+// The FP32 form (this library's own arithmetic, compared with the FP64 oracle within stated
+// bounds): the same sums with fused multiply-adds, the weight products and the final scale
+// factors folded; about half the operations of the reference's order.
+template <int O>
+struct OpBih<float, O> {
+  static constexpr int R = D4<O>::R, NF = 1;
+  float ihx2, ihy2, ihz2, ihx4, ihy4, ihz4;
+  template <class A>
+  __device__ __forceinline__ void operator()(const A& F, float* __restrict__ out,
+                                             size_t c) const {
+    using S2 = D2<O>;
+    using S4 = D4<O>;
+    float dx4 = 0.f, dy4 = 0.f, dz4 = 0.f;
+#pragma unroll
+    for (int m = 0; m < S4::N; ++m) {
+      const int o = m - S4::R;
+      const float w = float(S4::w(m));
+      dx4 = fmaf(w, F(0, o, 0, 0), dx4);
+      dy4 = fmaf(w, F(0, 0, o, 0), dy4);
+      dz4 = fmaf(w, F(0, 0, 0, o), dz4);
+    }
+    float dxy = 0.f, dxz = 0.f, dyz = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < S2::N; ++mb) {
+      const int ob = mb - S2::R;
+#pragma unroll
+      for (int ma = 0; ma < S2::N; ++ma) {
+        const int oa = ma - S2::R;
+        const float wab = float(S2::w(ma) * S2::w(mb));
+        dxy = fmaf(wab, F(0, oa, ob, 0), dxy);
+        dxz = fmaf(wab, F(0, oa, 0, ob), dxz);
+        dyz = fmaf(wab, F(0, 0, oa, ob), dyz);
+      }
+    }
+    const float cxy = 2.f * ihx2 * ihy2, cxz = 2.f * ihx2 * ihz2, cyz = 2.f * ihy2 * ihz2;
+    out[c] = fmaf(dx4, ihx4, fmaf(dy4, ihy4, fmaf(dz4, ihz4, fmaf(dxy, cxy, fmaf(dxz, cxz,
+                                                                          dyz * cyz)))));
+  }
+};
+
 // fvm/CellLaplacian.hpp:48-50
 template <class T>
 __device__ __forceinline__ T hmean(T a, T b) {
@@ -143,9 +184,11 @@ struct OpCell {
 // The harmonic mean of a face depends on D only, so each plane's x and y faces are formed
 // once (a faces pass into LDS after the plane is staged) and used by both cells they separate;
 // the z face above a site becomes the face below it on the next plane (a register per
-// column).  About 3 divisions per site instead of 6; every face mean is hmean(D_lower,
-// D_upper) and every flux difference is formed in the reference's order, so the result is
-// the per-cell evaluation's bit for bit.  Every D value the block stages is tested
+// column).  About 3 face means per site instead of 6, each formed by face_coef(D_lower,
+// D_upper) and the fluxes by fvm_k (gc_arith.hpp): in FP64 the reference's harmonic mean and
+// order, so the result is the per-cell evaluation's bit for bit; in FP32 (the precision this
+// kernel serves) the hardware reciprocal and fused multiply-adds.  Every D value the block
+// stages is tested
 // (!(D > 0)) and the smallest failing flat
 // index goes to *bad by atomicMin, so the caller needs no separate scan of D; on a failure
 // the output values are meaningless and the caller reports the error instead.
@@ -217,12 +260,12 @@ __global__ __launch_bounds__(256) void k_cell_zf(T ihx2, T ihy2, T ihz2,
     for (int i = t; i < OY * (OX + 1); i += Z::NT) {
       const int r = i / (OX + 1), c = i % (OX + 1);
       const int o = (r + 1) * PW + c;
-      fx[i] = hmean(Dc[o], Dc[o + 1]);
+      fx[i] = face_coef(Dc[o], Dc[o + 1], ihx2);
     }
     for (int i = t; i < (OY + 1) * OX; i += Z::NT) {
       const int r = i / OX, c = i % OX;
       const int o = r * PW + c + 1;
-      fy[i] = hmean(Dc[o], Dc[o + PW]);
+      fy[i] = face_coef(Dc[o], Dc[o + PW], ihy2);
     }
     __syncthreads();
     const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
@@ -236,16 +279,15 @@ __global__ __launch_bounds__(256) void k_cell_zf(T ihx2, T ihy2, T ihz2,
       const int oy = oy0 + q * Z::ROWS;
       const int o = (oy + 1) * PW + (ox + 1);
       const T pc = Pc[o], dcen = Dc[o];
-      if (k == 0) fzm[q] = hmean(Dm[o], dcen);
+      if (k == 0) fzm[q] = face_coef(Dm[o], dcen, ihz2);
       const T dxp = fx[oy * (OX + 1) + ox + 1], dxm = fx[oy * (OX + 1) + ox];
       const T dyp = fy[(oy + 1) * OX + ox], dym = fy[oy * OX + ox];
-      const T dzp = hmean(dcen, Dp[o]), dzm = fzm[q];
+      const T dzp = face_coef(dcen, Dp[o], ihz2), dzm = fzm[q];
       fzm[q] = dzp;
-      const T rx = (dxp * (Pc[o + 1] - pc) - dxm * (pc - Pc[o - 1])) * ihx2;
-      const T ry = (dyp * (Pc[o + PW] - pc) - dym * (pc - Pc[o - PW])) * ihy2;
-      const T rz = (dzp * (Pp[o] - pc) - dzm * (pc - Pm[o])) * ihz2;
+      const T v = fvm_k(pc, Pc[o - 1], Pc[o + 1], Pc[o - PW], Pc[o + PW], Pm[o], Pp[o], dxm, dxp,
+                        dym, dyp, dzm, dzp, ihx2, ihy2, ihz2);
       if (x0 + ox < d.nx && y0 + oy < d.ny)
-        out[zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] = rx + ry + rz;
+        out[zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] = v;
     }
     __syncthreads();
   }
