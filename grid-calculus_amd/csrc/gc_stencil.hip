@@ -1,10 +1,13 @@
 // gc_stencil.hip -- the stencil engine: Laplacian, gradient, divergence, biharmonic
 // and the finite-volume cell Laplacian on periodic grids.
 //
-// Every operator keeps the reference's arithmetic: per-axis partial sums accumulated
+// Every FP64 operator keeps the reference's arithmetic: per-axis partial sums accumulated
 // in stencil-offset order starting from 0, then the scaled axis terms summed left to
 // right; this file is compiled with -ffp-contract=off, so double-precision results
-// equal the CPU path bit for bit.
+// equal the CPU path bit for bit.  FP32 is this library's own form, compared with the FP64
+// oracle within stated bounds: the same order for the first-derivative and Laplacian
+// operators, fused multiply-adds for the cell Laplacian, and the separable form for the
+// biharmonic (k_bih_sep).
 //
 // Two engines, both marching through chunks of z-planes per block with block ids remapped so
 // each XCD owns a contiguous range of tiles (neighbouring tiles meet in the same L2):
@@ -108,46 +111,6 @@ struct OpBih {
     }
     out[c] = dx4 * ihx4 + dy4 * ihy4 + dz4 * ihz4 +
              T(2) * (dxy * ihx2 * ihy2 + dxz * ihx2 * ihz2 + dyz * ihy2 * ihz2);
-  }
-};
-
-// The FP32 form (this library's own arithmetic, compared with the FP64 oracle within stated
-// bounds): the same sums with fused multiply-adds, the weight products and the final scale
-// factors folded; about half the operations of the reference's order.
-template <int O>
-struct OpBih<float, O> {
-  static constexpr int R = D4<O>::R, NF = 1;
-  float ihx2, ihy2, ihz2, ihx4, ihy4, ihz4;
-  template <class A>
-  __device__ __forceinline__ void operator()(const A& F, float* __restrict__ out,
-                                             size_t c) const {
-    using S2 = D2<O>;
-    using S4 = D4<O>;
-    float dx4 = 0.f, dy4 = 0.f, dz4 = 0.f;
-#pragma unroll
-    for (int m = 0; m < S4::N; ++m) {
-      const int o = m - S4::R;
-      const float w = float(S4::w(m));
-      dx4 = fmaf(w, F(0, o, 0, 0), dx4);
-      dy4 = fmaf(w, F(0, 0, o, 0), dy4);
-      dz4 = fmaf(w, F(0, 0, 0, o), dz4);
-    }
-    float dxy = 0.f, dxz = 0.f, dyz = 0.f;
-#pragma unroll
-    for (int mb = 0; mb < S2::N; ++mb) {
-      const int ob = mb - S2::R;
-#pragma unroll
-      for (int ma = 0; ma < S2::N; ++ma) {
-        const int oa = ma - S2::R;
-        const float wab = float(S2::w(ma) * S2::w(mb));
-        dxy = fmaf(wab, F(0, oa, ob, 0), dxy);
-        dxz = fmaf(wab, F(0, oa, 0, ob), dxz);
-        dyz = fmaf(wab, F(0, 0, oa, ob), dyz);
-      }
-    }
-    const float cxy = 2.f * ihx2 * ihy2, cxz = 2.f * ihx2 * ihz2, cyz = 2.f * ihy2 * ihz2;
-    out[c] = fmaf(dx4, ihx4, fmaf(dy4, ihy4, fmaf(dz4, ihz4, fmaf(dxy, cxy, fmaf(dxz, cxz,
-                                                                          dyz * cyz)))));
   }
 };
 
@@ -578,6 +541,178 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   }
 }
 
+// ---- FP32 biharmonic: separable form on a z-march ------------------------------------------
+//
+// The FP32 form (this library's own arithmetic, compared with the FP64 oracle within stated
+// bounds) uses that the mixed fourth derivatives of the tensor-product stencil factor:
+//   Σ_a Σ_b w2(a) w2(b) f(x+a, y+b) = D2y(D2x f),  likewise D2z(D2x f) and D2z(D2y f),
+// so per plane a = D2x f and b = D2y f are formed once per site (a also on the R2 rows above
+// and below the tile, staged in LDS for D2y), and every z sum comes from per-column register
+// queues.  Per site about 21 LDS reads and 60 fused multiply-adds, against 96 LDS reads and
+// 100 multiply-adds for the per-site mixed sums.  Iteration k stages plane p = z0 - R4 + k
+// (tile sites from the owners' queues, the halo from a prefetch), forms a, b, D4x f, D4y f and
+// D2y a on it, and finishes output plane p - R4:
+//   out = D4x ihx4 + D4y ihy4 + D4z ihz4 + 2 (D2y a ihx2 ihy2 + D2z a ihx2 ihz2 + D2z b ihy2 ihz2).
+struct BihSepCoef {
+  float ihx4, ihy4, ihz4, cxy, cxz, cyz;  // cxy = 2 ihx2 ihy2, ...
+};
+
+template <int O, int OY>
+__global__ __launch_bounds__(256) void k_bih_sep(BihSepCoef c, const float* __restrict__ f,
+                                                 float* __restrict__ out, Dims d, int ntx,
+                                                 int nty, int zlen) {
+  using S2 = D2<O>;
+  using S4 = D4<O>;
+  constexpr int R4 = S4::R, R2 = S2::R, OX = 64, NT = 256;
+  constexpr int PW = OX + 2 * R4, PH = OY + 2 * R4, PN = PW * PH;  // f plane, halo R4
+  constexpr int AH = OY + 2 * R2, AN = OX * AH;                     // a plane, y halo R2
+  constexpr int ROWS = NT / OX, OR = OX * OY / NT, PL = (PN + NT - 1) / NT;
+  constexpr int NAH = 2 * R2 * OX, LA = (NAH + NT - 1) / NT;       // a halo sites
+  constexpr int QF = 2 * R4 + 1, QA = R4 + R2 + 1, QH = R4 + 1;     // queue lengths
+  static_assert((OX * OY) % NT == 0 && R4 >= R2, "tile must match the block");
+  __shared__ float fp[PN];
+  __shared__ float ap[AN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+  const int ox = t % OX, oy0 = t / OX;
+
+  // halo sites of the f plane this thread stages (tile sites come from the owners' queues)
+  unsigned hoff[PL];
+  int hslot[PL];
+#pragma unroll
+  for (int s = 0; s < PL; ++s) {
+    const int idx = t + s * NT;
+    const int px = idx % PW, py = idx / PW;
+    const bool tile = px >= R4 && px < R4 + OX && py >= R4 && py < R4 + OY;
+    hslot[s] = (idx < PN && !tile) ? idx : -1;
+    hoff[s] = hslot[s] >= 0
+                  ? (unsigned)(wrap_any(x0 + px - R4, d.nx) + d.nx * wrap_any(y0 + py - R4, d.ny))
+                  : 0u;
+  }
+  // a-plane halo rows this thread forms: a-plane row ar, column ac
+  int ahr[LA], ahc[LA];
+#pragma unroll
+  for (int s = 0; s < LA; ++s) {
+    const int idx = t + s * NT;
+    if (idx < NAH) {
+      const int hr = idx / OX;  // 0 .. 2 R2 - 1
+      ahr[s] = hr < R2 ? hr : OY + hr;
+      ahc[s] = idx % OX;
+    } else {
+      ahr[s] = -1;
+      ahc[s] = 0;
+    }
+  }
+  unsigned coff[OR];
+  bool cin[OR];
+#pragma unroll
+  for (int q = 0; q < OR; ++q) {
+    const int gy = y0 + oy0 + q * ROWS;
+    cin[q] = x0 + ox < d.nx && gy < d.ny;
+    coff[q] = (unsigned)(wrap_any(x0 + ox, d.nx) + d.nx * wrap_any(gy, d.ny));
+  }
+  auto plane_ptr = [&](int z) { return f + (size_t)d.plane * (size_t)wrap_any(z, d.nz); };
+  // register rings of period QP = 2 R4 + 1: plane p_k = z0 - R4 + k sits in slot k mod QP of
+  // each ring (f over the last QP planes, a and b over the last R4 + R2 + 1, the in-plane part
+  // over the last R4 + 1); the plane loop is unrolled by QP so every slot is a compile-time
+  // register and nothing is moved between planes
+  constexpr int QP = QF;
+  static_assert(QA <= QP && QH <= QP, "ring period");
+  float fq[OR][QP] = {}, aq[OR][QP] = {}, bq[OR][QP] = {}, hq[OR][QP] = {};
+  float hv[PL], fn[OR];
+  auto prefetch = [&](int z) {
+    const float* src = plane_ptr(z);
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+      if (hslot[s] >= 0) hv[s] = src[hoff[s]];
+#pragma unroll
+    for (int q = 0; q < OR; ++q) fn[q] = src[coff[q]];
+  };
+  prefetch(z0 - R4);
+  const int iters = nout + 2 * R4;
+  for (int kb = 0; kb < iters; kb += QP) {
+    unroll_seq(std::make_integer_sequence<int, QP>{}, [&](auto Pc) {
+      constexpr int P = decltype(Pc)::value;  // k mod QP
+      const int k = kb + P;
+      if (k >= iters) return;  // uniform over the block
+      auto slot = [](int back) { return (P - back + 2 * QP) % QP; };  // plane p_k - back
+#pragma unroll
+      for (int q = 0; q < OR; ++q) fq[q][P] = fn[q];
+#pragma unroll
+      for (int s = 0; s < PL; ++s)
+        if (hslot[s] >= 0) fp[hslot[s]] = hv[s];
+#pragma unroll
+      for (int q = 0; q < OR; ++q) fp[(oy0 + q * ROWS + R4) * PW + ox + R4] = fq[q][P];
+      __syncthreads();
+      if (k + 1 < iters) prefetch(z0 - R4 + k + 1);
+      // a = D2x f on the a-plane halo rows
+#pragma unroll
+      for (int s = 0; s < LA; ++s) {
+        if (ahr[s] >= 0) {
+          const float* row = fp + (ahr[s] - R2 + R4) * PW + ahc[s] + R4;
+          float a = 0.f;
+#pragma unroll
+          for (int m = 0; m < S2::N; ++m) a = fmaf(float(S2::w(m)), row[m - R2], a);
+          ap[ahr[s] * OX + ahc[s]] = a;
+        }
+      }
+      float d4x[OR], d4y[OR];
+#pragma unroll
+      for (int q = 0; q < OR; ++q) {
+        const int oy = oy0 + q * ROWS;
+        const float* cen = fp + (oy + R4) * PW + ox + R4;
+        float a = 0.f, b = 0.f, x4 = 0.f, y4 = 0.f;
+#pragma unroll
+        for (int m = 0; m < S2::N; ++m) {
+          const int o = m - R2;
+          a = fmaf(float(S2::w(m)), cen[o], a);
+          b = fmaf(float(S2::w(m)), cen[o * PW], b);
+        }
+#pragma unroll
+        for (int m = 0; m < S4::N; ++m) {
+          const int o = m - R4;
+          x4 = fmaf(float(S4::w(m)), cen[o], x4);
+          y4 = fmaf(float(S4::w(m)), cen[o * PW], y4);
+        }
+        ap[(oy + R2) * OX + ox] = a;
+        aq[q][P] = a;
+        bq[q][P] = b;
+        d4x[q] = x4;
+        d4y[q] = y4;
+      }
+      __syncthreads();
+      const int z = z0 - 2 * R4 + k;  // the output plane this iteration completes
+#pragma unroll
+      for (int q = 0; q < OR; ++q) {
+        const int oy = oy0 + q * ROWS;
+        float axy = 0.f;
+#pragma unroll
+        for (int m = 0; m < S2::N; ++m) axy = fmaf(float(S2::w(m)), ap[(oy + m) * OX + ox], axy);
+        hq[q][P] = fmaf(d4x[q], c.ihx4, fmaf(d4y[q], c.ihy4, axy * c.cxy));
+        if (k >= 2 * R4 && cin[q]) {
+          // plane z + o is p_k - (R4 - o)
+          float z4 = 0.f, az = 0.f, bz = 0.f;
+#pragma unroll
+          for (int m = 0; m < S4::N; ++m) z4 = fmaf(float(S4::w(m)), fq[q][slot(2 * R4 - m)], z4);
+#pragma unroll
+          for (int m = 0; m < S2::N; ++m) {
+            az = fmaf(float(S2::w(m)), aq[q][slot(R4 + R2 - m)], az);
+            bz = fmaf(float(S2::w(m)), bq[q][slot(R4 + R2 - m)], bz);
+          }
+          out[(size_t)d.plane * (size_t)z + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] =
+              fmaf(z4, c.ihz4, fmaf(az, c.cxz, fmaf(bz, c.cyz, hq[q][slot(R4)])));
+        }
+      }
+    });
+  }
+}
+
 // ---- vector- and tensor-field operators: z-accumulating engine ----------------------------
 //
 // The inputs are AoS (CIN values per site).  A block owns a 64 x 8 tile (one column per
@@ -933,7 +1068,15 @@ void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
   const Inv<double> i2 = inv_h2(g);
   // inv_h4 = inv_h2 * inv_h2 in double (Biharmonic.hpp:62-64), then cast
   const OpBih<T, O> op{T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x), T(i2.y * i2.y), T(i2.z * i2.z)};
-  if constexpr (std::is_same<T, double>::value && O == 4) {
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: the separable form (k_bih_sep), 64 x 8 tiles
+    const BihSepCoef c{float(i2.x * i2.x), float(i2.y * i2.y), float(i2.z * i2.z),
+                       float(2.0 * i2.x * i2.y), float(2.0 * i2.x * i2.z),
+                       float(2.0 * i2.y * i2.z)};
+    const int ntx = (d.nx + 63) / 64, nty = (d.ny + 7) / 8, nzc = (d.nz + kZLen - 1) / kZLen;
+    hipLaunchKernelGGL((k_bih_sep<O, 8>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
+                       ctx->stream, c, (const float*)in, (float*)out, d, ntx, nty, kZLen);
+  } else if constexpr (O == 4) {
     // the register-queue kernel pays where the plane ring is largest (9 planes of 8-byte
     // values): 1.42 ms against 1.64 ms on 512^3; for the other three cases the plain z-march
     // is as fast or faster
