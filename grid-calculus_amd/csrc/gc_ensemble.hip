@@ -105,23 +105,41 @@ __global__ __launch_bounds__(256) void k_ens_step(const EnsMember* __restrict__ 
   __shared__ T MU[C::N];  // mu of the current stage
   __shared__ T K1[OX * OY], K23[OX * OY];
   const int t = threadIdx.x;
+  // The next item's tile + halo is fetched into registers while the current item runs its
+  // stages, so an item starts from a tile already on chip.
+  constexpr int NL = (C::N + C::NT - 1) / C::NT;
+  T pre[NL];
+  auto fetch = [&](int w) {
+    const WorkItem wi = work[w];
+    const EnsMember m = mem[wi.member];
+    const T* __restrict__ src = pool + (parity ? m.off[1] : m.off[0]);
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int idx = t + s * C::NT;
+      if (idx < C::N) {
+        const int px = idx % W, py = idx / W;
+        pre[s] = src[(size_t)wrap_mod(wi.x0 + px - H, m.nx) +
+                     (size_t)m.nx * (size_t)wrap_mod(wi.y0 + py - H, m.ny)];
+      }
+    }
+  };
+  if ((int)blockIdx.x < nwork) fetch(blockIdx.x);
 
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem wi = work[w];
     const EnsMember m = mem[wi.member];
     // selects, not m.off[parity]: a runtime index into the copied struct would place the
     // whole struct in per-lane scratch memory (80 bytes a lane, written back through L2)
-    const size_t off_in = parity ? m.off[1] : m.off[0];
     const size_t off_out = parity ? m.off[0] : m.off[1];
-    const T* __restrict__ src = pool + off_in;
     T* __restrict__ dst = pool + off_out;
     const CHCoef<T> cf = make_ch_coef<T>(T(m.M), T(m.kappa), T(m.ihx2), T(m.ihy2), T(0));
     __syncthreads();  // the previous item is done with LDS
-    for (int idx = t; idx < C::N; idx += C::NT) {
-      const int px = idx % W, py = idx / W;
-      P[idx] = src[(size_t)wrap_mod(wi.x0 + px - H, m.nx) +
-                   (size_t)m.nx * (size_t)wrap_mod(wi.y0 + py - H, m.ny)];
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+      const int idx = t + s * C::NT;
+      if (idx < C::N) P[idx] = pre[s];
     }
+    if (w + (int)gridDim.x < nwork) fetch(w + (int)gridDim.x);
     __syncthreads();
 #pragma unroll
     for (int s = 1; s <= 4; ++s) {
