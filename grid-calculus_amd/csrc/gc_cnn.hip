@@ -371,17 +371,16 @@ struct Chain {
 // HO, input (in, LDS, site stride kSP) one site wider.  The output region's sites, row-major,
 // are cut into chunks of 32, one per wave (16 waves; every region here has at most 14);
 // store(site, g, v) receives channels 8 g + cq .. 8 g + cq + 3 of an output site as FP16.
-// The weight fragments (fragment-major, one contiguous 1 KiB per load) are read as the MFMAs
-// need them: four waves per SIMD hide the load latency.
+// wt: the layer's weight fragments (fragment-major, one contiguous 1 KiB per read), staged
+// in LDS by the kernel before its first 32 -> 32 layer.
 template <int HO, class Store>
 __device__ __forceinline__ void mid_layer(const _Float16* __restrict__ in, int l,
-                                          const _Float16* __restrict__ wmid,
+                                          const _Float16* __restrict__ wt,
                                           const float* __restrict__ p, int wave, int lane,
                                           Store store) {
   constexpr int Wo = kFC + 2 * HO, Ho = kFR + 2 * HO, Wi = Wo + 2, NS = Wo * Ho;
   constexpr int nch = (NS + 31) / 32;
   const int j = lane & 31, kh = (lane >> 5) * 8, cq = (lane >> 5) * 4;
-  const _Float16* wt = wmid + (size_t)l * kMidW;
   float bj[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) bj[i] = p[kOffMid + l * kMidStride + kMidW + (i / 4) * 8 + cq + i % 4];
@@ -408,6 +407,24 @@ __device__ __forceinline__ void mid_layer(const _Float16* __restrict__ in, int l
   }
 }
 
+// The fragments of two consecutive 32 -> 32 layers (36 KiB of FP16), fetched into registers
+// at kernel start and stored to LDS once the registers are free of the loads' latency.
+constexpr int kWQ = (2 * kMidW / 8 + kFNT - 1) / kFNT;  // 16-byte pieces per thread
+__device__ __forceinline__ void fetch_w2(const _Float16* __restrict__ w, int t, f16x8* r) {
+#pragma unroll
+  for (int i = 0; i < kWQ; ++i) {
+    const int q = t + i * kFNT;
+    if (q < 2 * kMidW / 8) r[i] = ((const f16x8*)w)[q];
+  }
+}
+__device__ __forceinline__ void store_w2(_Float16* ws, int t, const f16x8* r) {
+#pragma unroll
+  for (int i = 0; i < kWQ; ++i) {
+    const int q = t + i * kFNT;
+    if (q < 2 * kMidW / 8) ((f16x8*)ws)[q] = r[i];
+  }
+}
+
 // Kernel A: layers 1..3 of the network on the block's tile; layer 3's activations to a3.
 template <class TS, bool APPLY>
 __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
@@ -422,12 +439,15 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   __shared__ float w1s[288 + 32];
   __shared__ __attribute__((aligned(16))) _Float16 bufA[kAN1 * kSP];
   __shared__ __attribute__((aligned(16))) _Float16 bufB[kAN2 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 wls[2 * kMidW];  // layers 2 and 3
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   constexpr int NR = kAH0 * kAW0;
+  f16x8 wreg[kWQ];
+  fetch_w2(wmid, t, wreg);
   if constexpr (APPLY) {
     // Every load is issued before the first use: the state region, the tile's five-point
     // neighbourhoods and the previous tile sums.
@@ -525,13 +545,14 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
     };
     for (int cc = wave; cc < nch; cc += kFNW) emit1(chunk(cc), cc * 32 + j);
   }
+  store_w2(wls, t, wreg);
   __syncthreads();
-  mid_layer<1>(bufA, 0, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+  mid_layer<1>(bufA, 0, wls, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
     *(f16x4*)(bufB + s * kSP + ch0) = v;
   });
   __syncthreads();
   const size_t img = (size_t)b * H * W;
-  mid_layer<0>(bufB, 1, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+  mid_layer<0>(bufB, 1, wls + kMidW, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
     *(f16x4*)(a3 + (img + (size_t)(y0 + s / kFC) * W + x0 + s % kFC) * kC + ch0) = v;
   });
 }
@@ -550,12 +571,15 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const _Float16* __restrict__ a3,
   __shared__ float w6s[288];
   __shared__ __attribute__((aligned(16))) _Float16 bufA[kBN3 * kSP];
   __shared__ __attribute__((aligned(16))) _Float16 bufB[kBN4 * kSP];
+  __shared__ __attribute__((aligned(16))) _Float16 wls[2 * kMidW];  // layers 4 and 5
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tx = blockIdx.x % tiles_x;
   const int r1 = blockIdx.x / tiles_x;
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   const size_t img = (size_t)b * H * W;
+  f16x8 wreg[kWQ];
+  fetch_w2(wmid + 2 * kMidW, t, wreg);
   // layer 3's activations on the halo-3 region: every load issued before the first store
   constexpr int NQ = (kBN3 * 4 + kFNT - 1) / kFNT;
   f16x8 rv[NQ];
@@ -574,12 +598,13 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const _Float16* __restrict__ a3,
     if (q < kBN3 * 4) *(f16x8*)(bufA + (q >> 2) * kSP + (q & 3) * 8) = rv[r];
   }
   for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
+  store_w2(wls, t, wreg);
   __syncthreads();
-  mid_layer<2>(bufA, 2, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+  mid_layer<2>(bufA, 2, wls, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
     *(f16x4*)(bufB + s * kSP + ch0) = v;
   });
   __syncthreads();
-  mid_layer<1>(bufB, 3, wmid, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
+  mid_layer<1>(bufB, 3, wls + kMidW, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
     *(f16x4*)(bufA + s * kSP + ch0) = v;
   });
   __syncthreads();
