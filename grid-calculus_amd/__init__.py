@@ -504,7 +504,7 @@ def ch_config_shape(g, i):
 
 def ch_config_kind(g, i):
     """0: one kernel per RK4 stage; 1: two-stage kernels with LDS plane rings; 2: two-stage
-    kernels with register queues along z."""
+    kernels with register queues along z; 3: one kernel per RK4 step (2-D)."""
     r = lib().gc_ch_config_fused(C.byref(g), i)
     if r < 0:
         raise GridcalcError(1, f"gridcalc: config id {i} out of range")
@@ -512,7 +512,8 @@ def ch_config_kind(g, i):
 
 
 def ch_config_fused(g, i):
-    """True if 3-D candidate i runs RK4 as two two-stage kernels."""
+    """True if candidate i fuses RK4 stages: two two-stage kernels (3-D) or one kernel per
+    step (2-D)."""
     return ch_config_kind(g, i) > 0
 
 

@@ -1373,6 +1373,80 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   }
 }
 
+// ---- one kernel per RK4 step (2-D, temporal blocking) -------------------------------------
+//
+// A block owns an OX x OY tile of a 2-D grid, loads it with an 8-site periodic halo into LDS and
+// runs all four stages there: stage s forms mu on the tile + (9 - 2s) halo and k_s on the tile
+// + (8 - 2s) halo, so the halo shrinks by two per stage and k4 lands exactly on the tile; the
+// new state goes to the other state buffer.  A step reads the state once (plus halos) and
+// writes it once, in one launch instead of four.  Every site value is formed with the stage
+// kernels' arithmetic (ch_mu2, ch_k2, stage_axpy, rk4_final), so the results are theirs bit
+// for bit -- in FP64 the reference's (solve/RK4.hpp:127-153 with the right-hand side of
+// examples/common/cahn_hilliard.hpp:41-57).
+template <int OX, int OY>
+struct Tb2Cfg {
+  static constexpr int H = 8, W = OX + 2 * H, HH = OY + 2 * H, N = W * HH, NT = 256;
+};
+
+template <class T, int OX, int OY>
+__global__ __launch_bounds__(256) void k_ch_tb2d(CHCoef<T> cf, const T* __restrict__ src,
+                                                 T* __restrict__ dst, Dims d, int ntx,
+                                                 StageCoef<T> co) {
+  using C = Tb2Cfg<OX, OY>;
+  constexpr int W = C::W, H = C::H;
+  __shared__ T P[C::N];   // psi on the tile + 8 halo
+  __shared__ T Y[C::N];   // stage input y_s (s >= 2)
+  __shared__ T MU[C::N];  // mu of the current stage
+  __shared__ T K1[OX * OY], K23[OX * OY];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (int)(bid % (unsigned)ntx) * OX, y0 = (int)(bid / (unsigned)ntx) * OY;
+  for (int idx = t; idx < C::N; idx += C::NT) {
+    const int px = idx % W, py = idx / W;
+    P[idx] = src[(size_t)wrap_any(x0 + px - H, d.nx) + (size_t)d.nx * (size_t)wrap_any(y0 + py - H, d.ny)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 1; s <= 4; ++s) {
+    const T* __restrict__ in = (s == 1) ? P : Y;
+    {  // mu on the tile + rm halo
+      const int rm = 9 - 2 * s, wm = OX + 2 * rm, nm = wm * (OY + 2 * rm);
+      for (int idx = t; idx < nm; idx += C::NT) {
+        const int o = (idx / wm + H - rm) * W + (idx % wm + H - rm);
+        MU[o] = ch_mu2(cf, in[o], in[o - 1], in[o + 1], in[o - W], in[o + W]);
+      }
+    }
+    __syncthreads();
+    {  // k_s on the tile + rk halo, then the stage update
+      const int rk = 8 - 2 * s, wk = OX + 2 * rk, nk = wk * (OY + 2 * rk);
+      for (int idx = t; idx < nk; idx += C::NT) {
+        const int kx = idx % wk, ky = idx / wk;
+        const int o = (ky + H - rk) * W + (kx + H - rk);
+        const T kv = ch_k2(cf, MU[o], MU[o - 1], MU[o + 1], MU[o - W], MU[o + W]);
+        const int ix = kx - rk, iy = ky - rk;
+        const bool interior = ix >= 0 && ix < OX && iy >= 0 && iy < OY;
+        const int ii = iy * OX + ix;
+        if (s == 1) {
+          Y[o] = stage_axpy(P[o], co.half_dt, kv);
+          if (interior) K1[ii] = kv;
+        } else if (s == 2) {
+          Y[o] = stage_axpy(P[o], co.half_dt, kv);
+          if (interior) K23[ii] = kv;
+        } else if (s == 3) {
+          Y[o] = stage_axpy(P[o], co.dt, kv);
+          if (interior) K23[ii] = K23[ii] + kv;
+        } else {
+          const int gx = x0 + ix, gy = y0 + iy;
+          if (gx < d.nx && gy < d.ny)
+            dst[(size_t)gx + (size_t)d.nx * (size_t)gy] =
+                rk4_final(P[o], K1[ii], K23[ii], kv, co.sixth_dt, co.third_dt);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- tile shapes of the Cahn-Hilliard stage kernels and their selection ------------------
 //
 // Candidate shapes (tile width OX, tile height OY, z-chunk ZC).  The default is used until
@@ -1410,6 +1484,10 @@ constexpr PairTile kChPairCfgs[] = {
     {64, 32, 32, 1024, 1},  {64, 32, 64, 1024, 1},  {128, 16, 32, 1024, 1}};
 constexpr int kNP = sizeof(kChPairCfgs) / sizeof(kChPairCfgs[0]);
 inline bool is_pair(const Dims& d, int id) { return d.nz > 1 && id >= kN3; }
+// 2-D one-launch-per-step (temporal blocking) shapes: ids kN2 .. kN2 + kNT2 - 1
+constexpr TileCfg kChTb2Cfgs[] = {{32, 16, 1}, {32, 32, 1}, {64, 16, 1}, {64, 32, 1}};
+constexpr int kNT2 = sizeof(kChTb2Cfgs) / sizeof(kChTb2Cfgs[0]);
+inline bool is_tb2(const Dims& d, int id) { return d.nz == 1 && id >= kN2; }
 // 3-D RK4 defaults: the measured best on 256^3 (register-queue two-stage kernels; FP32
 // 64x32x32, FP64 64x16x64)
 constexpr int kDefault3F32 = kN3 + 12, kDefault3F64 = kN3 + 11;
@@ -1561,7 +1639,7 @@ inline long tile_blocks(const Dims& d, int ox, int oy, int zc) {
 // workgroups and, among those, the shortest z-chunk (measured: 64x8x8 is the fastest
 // candidate on 64^3 and 128^3 in both precisions, 3-5x the two-stage default there).
 int default_config(const Dims& d, int dt) {
-  if (d.nz == 1) return kDefault2;
+  if (d.nz == 1) return kN2;  // 2-D RK4: one launch per step, 32 x 16 tiles
   const int pair = dt == GC_F64 ? kDefault3F64 : kDefault3F32;
   const PairTile& pt = kChPairCfgs[pair - kN3];
   const int poy = (dt == GC_F64 && pt.q) ? 16 : pt.oy;  // FP64 queue kernels run 64 x 16
@@ -1592,6 +1670,7 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
   if (ch_tiled(d)) {
     int id = cfg >= 0 ? cfg : ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32);
     if (is_pair(d, id)) id = sizeof(T) == 8 ? kStage3F64 : kStage3F32;  // one stage only
+    if (is_tb2(d, id)) id = kDefault2;
     launch_ch_cfg<T>(ctx, id, d, cf, stage, y, psi, ynext, a, b, co);
   } else {
     const int nbx = (d.nx + TX - 1) / TX, nby = (d.ny + TY - 1) / TY;
@@ -1599,6 +1678,27 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, i
                        dim3(TX * TY), 0, ctx->stream, rhs, stage, y, psi, ynext, a, b, d, nbx,
                        nby, co);
   }
+  GC_LAUNCHED();
+  return GC_OK;
+}
+
+// One 2-D RK4 step in one launch (k_ch_tb2d): psi_in -> psi_out (distinct buffers).
+template <class T>
+gc_status tb2_step(gc_context* ctx, const gc_grid* g, const CHRhs<T>& rhs, int id,
+                   const T* psi_in, T* psi_out, const StageCoef<T>& co) {
+  const Dims d = dims_of(g);
+  const TileCfg c = kChTb2Cfgs[id - kN2];
+#define GC_TB(OX, OY)                                                                       \
+  do {                                                                                      \
+    const int ntx = (d.nx + OX - 1) / OX, nty = (d.ny + OY - 1) / OY;                       \
+    hipLaunchKernelGGL((k_ch_tb2d<T, OX, OY>), dim3((unsigned)(ntx * nty)), dim3(256), 0,  \
+                       ctx->stream, rhs.cf, psi_in, psi_out, d, ntx, co);                   \
+  } while (0)
+  if (c.ox == 32 && c.oy == 16) GC_TB(32, 16);
+  else if (c.ox == 32) GC_TB(32, 32);
+  else if (c.oy == 16) GC_TB(64, 16);
+  else GC_TB(64, 32);
+#undef GC_TB
   GC_LAUNCHED();
   return GC_OK;
 }
@@ -1638,6 +1738,19 @@ gc_status run_steps(gc_context* ctx, const gc_grid* g, const Rhs& rhs, T* psi, d
     if constexpr (std::is_same<Rhs, CHRhs<T>>::value) {
       const Dims d = dims_of(g);
       const int id = ch_tiled(d) ? ch_config(d, sizeof(T) == 8 ? GC_F64 : GC_F32) : -1;
+      if (id >= 0 && is_tb2(d, id)) {  // 2-D: one launch per step; yA holds the other state
+        T* cur = psi;
+        T* other = yA;
+        for (int s = 0; s < n_steps; ++s) {
+          GC_TRY(tb2_step<T>(ctx, g, rhs, id, cur, other, co));
+          T* tmp = cur;
+          cur = other;
+          other = tmp;
+        }
+        if (cur != psi)
+          GC_HIP(hipMemcpyAsync(psi, cur, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+        return GC_OK;
+      }
       if (id >= 0 && is_pair(d, id)) {  // two stages per pass; yA holds the other state
         T* cur = psi;
         T* other = yA;
@@ -1885,7 +1998,7 @@ namespace {
 template <class T>
 gc_status tune_ch(gc_context* ctx, const gc_grid* g, int* best, float* ms_out) {
   const Dims d = dims_of(g);
-  const int ncfg = d.nz == 1 ? kN2 : kN3 + kNP;
+  const int ncfg = d.nz == 1 ? kN2 + kNT2 : kN3 + kNP;
   const size_t n = sites(g);
   void* scr = nullptr;
   GC_TRY(scratch(ctx, 5 * n * sizeof(T), &scr));
@@ -1906,6 +2019,11 @@ gc_status tune_ch(gc_context* ctx, const gc_grid* g, int* best, float* ms_out) {
   int cur = 0;
   for (int id = 0; id < ncfg; ++id) {
     auto step = [&]() -> gc_status {
+      if (is_tb2(d, id)) {
+        GC_TRY(tb2_step<T>(ctx, g, rhs, id, bufs[cur], bufs[cur ^ 1], co));
+        cur ^= 1;
+        return GC_OK;
+      }
       if (is_pair(d, id)) {
         GC_TRY(pair_step<T>(ctx, g, rhs, id, bufs[cur], bufs[cur ^ 1], yB, a, b, co));
         cur ^= 1;
@@ -1952,11 +2070,12 @@ extern "C" {
 
 int gc_ch_config_count(const gc_grid* g) {
   if (!g || check_grid(g) != GC_OK || !ch_tiled(dims_of(g))) return 0;
-  return g->nz == 1 ? kN2 : kN3 + kNP;
+  return g->nz == 1 ? kN2 + kNT2 : kN3 + kNP;
 }
 
 int gc_ch_config_fused(const gc_grid* g, int id) {
   if (!g || id < 0 || id >= gc_ch_config_count(g)) return -1;
+  if (is_tb2(dims_of(g), id)) return 3;
   if (!is_pair(dims_of(g), id)) return 0;
   return kChPairCfgs[id - kN3].q ? 2 : 1;
 }
@@ -1973,7 +2092,7 @@ gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc
     *zc = c.zc;
     return GC_OK;
   }
-  const TileCfg c = g->nz == 1 ? kCh2Cfgs[id] : kCh3Cfgs[id];
+  const TileCfg c = g->nz == 1 ? (id >= kN2 ? kChTb2Cfgs[id - kN2] : kCh2Cfgs[id]) : kCh3Cfgs[id];
   *ox = c.ox;
   *oy = c.oy;
   *zc = c.zc;

@@ -291,11 +291,13 @@ int gc_ch_config_count(const gc_grid* g);
 /* Tile shape of candidate `id` (as run in FP32; two-stage candidates whose FP32 tile needs
  * more LDS than a workgroup has in FP64 run 64 x 16 x zc in FP64). */
 gc_status gc_ch_config_shape(const gc_grid* g, int id, int* ox, int* oy, int* zc);
-/* Nonzero if 3-D candidate `id` runs RK4 as two kernels of two stages each (S1+S2, S3+S4:
- * 9 words of HBM traffic per site and step instead of 17; same results bit for bit): 1 for
- * the kernel that keeps three planes of every intermediate in LDS, 2 for the kernel that
- * keeps the z direction in registers.  0 for one kernel per stage, -1 for an invalid id.
- * Euler and single right-hand-side evaluations always use one-stage kernels. */
+/* Nonzero if candidate `id` fuses RK4 stages (same results bit for bit): 3-D candidates run
+ * RK4 as two kernels of two stages each (S1+S2, S3+S4: 9 words of HBM traffic per site and
+ * step instead of 17), 1 for the kernel that keeps three planes of every intermediate in
+ * LDS, 2 for the kernel that keeps the z direction in registers; 3 for a 2-D candidate that
+ * runs all four stages of a step in one kernel (the tile and an 8-site halo in LDS, 2 words
+ * per site and step).  0 for one kernel per stage, -1 for an invalid id.  Euler and single
+ * right-hand-side evaluations always use one-stage kernels. */
 int gc_ch_config_fused(const gc_grid* g, int id);
 /* Forces candidate `id` for (extents, dtype); -1 restores the built-in default. */
 gc_status gc_ch_set_config(const gc_grid* g, gc_dtype dt, int id);

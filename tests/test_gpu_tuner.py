@@ -36,6 +36,25 @@ def test_every_3d_config_bit_exact_fp64(gc, orc, ctx, shape):
         gc.ch_set_config(g, gc.F64, -1)
 
 
+# 2-D: ragged against every tile, and smaller than the 8-site halo of the one-kernel-per-step
+# candidates (the halo wraps around the grid more than once)
+@pytest.mark.parametrize("shape", [(70, 37, 1), (16, 9, 1)], ids=lambda s: "x".join(map(str, s)))
+def test_every_2d_config_bit_exact_fp64(gc, orc, ctx, shape):
+    g = gc.Grid.make(*shape, (1.0, 0.9, 1.1))
+    og = orc.Grid.make(*shape, (1.0, 0.9, 1.1))
+    c0 = gc.inputs.ch_initial(*shape, 0.3, seed=47)
+    steps = 5  # odd: the one-kernel-per-step path ends in its second buffer and copies back
+    want = orc.ch_integrate(og, c0, 1.0, 0.7, 0.01, steps, orc.RK4)
+    n = gc.ch_config_count(g)
+    try:
+        for i in range(n):
+            gc.ch_set_config(g, gc.F64, i)
+            got = _run(gc, ctx, g, c0, steps, np.float64)
+            assert np.array_equal(got, want), (i, gc.ch_config_shape(g, i), gc.ch_config_kind(g, i))
+    finally:
+        gc.ch_set_config(g, gc.F64, -1)
+
+
 @pytest.mark.parametrize("shape", [(36, 20, 70), (64, 40, 1)], ids=lambda s: "x".join(map(str, s)))
 def test_every_config_same_bits_fp32(gc, ctx, shape):
     g = gc.Grid.make(*shape)
@@ -76,4 +95,5 @@ def test_config_queries(gc):
     with pytest.raises(gc.GridcalcError):
         gc.ch_set_config(g3, gc.F64, n)
     g2 = gc.Grid.make(64, 64, 1)
-    assert not any(gc.ch_config_fused(g2, i) for i in range(gc.ch_config_count(g2)))
+    kinds = [gc.ch_config_kind(g2, i) for i in range(gc.ch_config_count(g2))]
+    assert 0 in kinds and 3 in kinds and set(kinds) <= {0, 3}  # one stage / one step per kernel
