@@ -435,6 +435,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
                                                _Float16* __restrict__ a3, int H, int W,
                                                int tiles_x, int tiles_y, Chain<TS> ch) {
   __shared__ float cs[kAH0 * kAW0];
+  __shared__ TS csx[APPLY ? kAH0 * kAW0 : 1];  // chained: the new state on the region
   __shared__ Comp2 msh[kFR * kFC / 64];
   __shared__ float w1s[288 + 32];
   __shared__ __attribute__((aligned(16))) _Float16 bufA[kAN1 * kSP];
@@ -446,11 +447,23 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   constexpr int NR = kAH0 * kAW0;
-  f16x8 wreg[kWQ];
-  fetch_w2(wmid, t, wreg);
+  f16x8 wreg[kWQ];  // layers 2 and 3's weights, requested after the state's loads
+  // layer 1's weights and biases, requested first (staged in LDS with the state)
+  const float w1v = t < 288 ? w1[t] : (t < 288 + 32 ? b1[t - 288] : 0.0f);
   if constexpr (APPLY) {
-    // Every load is issued before the first use: the state region, the tile's five-point
-    // neighbourhoods and the previous tile sums.
+    // Every load is issued before the first use: the previous tile sums first (the mean
+    // waits only for them), then the state region.
+    // The sums are fetched eight per lane at a time and added in index order, as a plain
+    // lane-strided loop would (which waits for each load before issuing the next).
+    double sv = 0.0;
+    for (int k0 = lane; k0 < ch.nparts; k0 += 8 * 64) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = k0 + 64 * u < ch.nparts ? ch.pprev[k0 + 64 * u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + 64 * u < ch.nparts) sv += v[u];
+    }
     constexpr int kNH = (NR + kFNT - 1) / kFNT;
     double hc[kNH];
     float ho[kNH];
@@ -463,47 +476,23 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
         ho[r] = ch.oprev[g];
       }
     }
-    const bool own = t < kFR * kFC;
-    const int i = x0 + t % kFC, j = y0 + t / kFC;
-    size_t g5[5];
-    double c5[5];
-    float o5[5];
-    if (own) {
-      g5[0] = (size_t)j * W + i;
-      g5[1] = (size_t)j * W + wrap1(i - 1, W);
-      g5[2] = (size_t)wrap1(j - 1, H) * W + i;
-      g5[3] = (size_t)j * W + wrap1(i + 1, W);
-      g5[4] = (size_t)wrap1(j + 1, H) * W + i;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        c5[k] = (double)ch.cprev[g5[k]];
-        o5[k] = ch.oprev[g5[k]];
-      }
-    }
+    fetch_w2(wmid, t, wreg);
     // mean of the previous output, in every wave: lane 0's value of the lane-strided sums
     // and xor tree of block_sum_parts, so the value k_apply uses
-    double sv = 0.0;
-    for (int k = lane; k < ch.nparts; k += 64) sv += ch.pprev[k];
     for (int st = 32; st >= 1; st >>= 1) sv += __shfl_xor(sv, st);
     const double mean = __shfl(sv, 0) / (double)((size_t)H * W);
     auto nv = [&](double cp, float op) { return (TS)(cp + ch.scale * ((double)op - mean)); };
 #pragma unroll
     for (int r = 0; r < kNH; ++r) {
       const int q = t + r * kFNT;
-      if (q < NR) cs[q] = (float)nv(hc[r], ho[r]);
-    }
-    if (own) {  // the tile: write the new state, reduce its mass and density
-      TS x[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) x[k] = nv(c5[k], o5[k]);
-      ch.cdst[g5[0]] = x[0];
-      double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
-      site_mass_fe5((double)x[0], (double)x[1], (double)x[2], (double)x[3], (double)x[4],
-                    ch.ihx, ch.ihy, ch.kappa, ms, mc, fs, fc);
-      const Comp2 v = wave_merge(Comp2{ms, mc, fs, fc});
-      if (lane == 0) msh[wave] = v;
+      if (q < NR) {
+        const TS x = nv(hc[r], ho[r]);
+        csx[q] = x;
+        cs[q] = (float)x;
+      }
     }
   } else {
+    fetch_w2(wmid, t, wreg);
     const TS* img = c + (size_t)b * H * W;
     constexpr int kNH = (NR + kFNT - 1) / kFNT;
     TS hv[kNH];
@@ -518,11 +507,31 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
       if (q < NR) cs[q] = (float)hv[r];
     }
   }
-  for (int q = t; q < 288 + 32; q += kFNT) w1s[q] = q < 288 ? w1[q] : b1[q - 288];
+  static_assert(kFNT >= 288 + 32, "one layer-1 weight or bias per thread");
+  if (t < 288 + 32) w1s[t] = w1v;
   __syncthreads();
-  if constexpr (APPLY)
-    if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
-  // layer 1 on the halo-2 region (kFR + 4) x (kFC + 4), in 32-site chunks like layers 2..5
+  // layer 1 on the halo-2 region (kFR + 4) x (kFC + 4), in 32-site chunks like layers 2..5;
+  // chained: the two waves without a layer-1 chunk write the tile's new state and reduce its
+  // mass and density meanwhile, tile site v * 64 + lane in pass v of the four (one wave's
+  // worth of sites per pass, so the sums merge exactly as with one site per thread)
+  if constexpr (APPLY) {
+    static_assert((kAN1 + 31) / 32 <= kFNW - 2 && kFR * kFC == 4 * 64, "idle waves in layer 1");
+    if (wave >= kFNW - 2) {
+#pragma unroll
+      for (int pv = 0; pv < 2; ++pv) {
+        const int v = (wave - (kFNW - 2)) * 2 + pv, st = v * 64 + lane;
+        const int q0 = (st / kFC + 3) * kAW0 + st % kFC + 3;
+        const TS x0v = csx[q0], xm = csx[q0 - 1], ym = csx[q0 - kAW0], xp = csx[q0 + 1],
+                 yp = csx[q0 + kAW0];
+        ch.cdst[(size_t)(y0 + st / kFC) * W + x0 + st % kFC] = x0v;
+        double ms = 0.0, mc = 0.0, fs = 0.0, fc = 0.0;
+        site_mass_fe5((double)x0v, (double)xm, (double)ym, (double)xp, (double)yp, ch.ihx,
+                      ch.ihy, ch.kappa, ms, mc, fs, fc);
+        const Comp2 m2 = wave_merge(Comp2{ms, mc, fs, fc});
+        if (lane == 0) msh[v] = m2;
+      }
+    }
+  }
   {
     constexpr int RW = kFC + 4, RN = kAN1, nch = (RN + 31) / 32;
     const int j = lane & 31, cq = (lane >> 5) * 4;
@@ -547,6 +556,8 @@ __global__ __launch_bounds__(kFNT) void k_fwd_a(const TS* __restrict__ c,
   }
   store_w2(wls, t, wreg);
   __syncthreads();
+  if constexpr (APPLY)
+    if (t == 0) ch.mpart[blockIdx.x] = waves_merge(msh, kFR * kFC / 64);
   mid_layer<1>(bufA, 0, wls, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
     *(f16x4*)(bufB + s * kSP + ch0) = v;
   });
@@ -578,6 +589,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const _Float16* __restrict__ a3,
   const int ty = r1 % tiles_y, b = r1 / tiles_y;
   const int x0 = tx * kFC, y0 = ty * kFR;
   const size_t img = (size_t)b * H * W;
+  const float w6v = t < 288 ? w6[t] : 0.0f;  // layer 6's weights, staged in LDS below
   f16x8 wreg[kWQ];
   fetch_w2(wmid + 2 * kMidW, t, wreg);
   // layer 3's activations on the halo-3 region: every load issued before the first store
@@ -597,7 +609,7 @@ __global__ __launch_bounds__(kFNT) void k_fwd_b(const _Float16* __restrict__ a3,
     const int q = t + r * kFNT;
     if (q < kBN3 * 4) *(f16x8*)(bufA + (q >> 2) * kSP + (q & 3) * 8) = rv[r];
   }
-  for (int q = t; q < 288; q += kFNT) w6s[q] = w6[q];
+  if (t < 288) w6s[t] = w6v;
   store_w2(wls, t, wreg);
   __syncthreads();
   mid_layer<2>(bufA, 2, wls, p, wave, lane, [&](int s, int ch0, const f16x4& v) {
