@@ -387,13 +387,30 @@ __device__ __forceinline__ void mid_layer(const _Float16* __restrict__ in, int l
   for (int c = wave; c < nch; c += kFNW) {
     const int s0 = min(c * 32 + j, NS - 1);
     const int i0 = (s0 / Wo + 1) * Wi + s0 % Wo + 1;
+    // operand fragments of step kc; requested kAhead steps before their matrix instruction
+    auto frag_b = [&](int kc) {
+      const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
+      return *(const f16x8*)(in + (i0 + off) * kSP + (kc & 1) * 16 + kh);
+    };
+    auto frag_a = [&](int kc) { return *(const f16x8*)(wt + (size_t)(kc * 64 + lane) * 8); };
+    constexpr int kAhead = 4;
+    f16x8 fa[kAhead], fb[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      fa[u] = frag_a(u);
+      fb[u] = frag_b(u);
+    }
     f32x16 acc = {};
 #pragma unroll
     for (int kc = 0; kc < 18; ++kc) {
-      const int tap = kc >> 1, off = (tap / 3 - 1) * Wi + tap % 3 - 1;
-      const int ko = (kc & 1) * 16 + kh;
-      const f16x8 a = *(const f16x8*)(in + (i0 + off) * kSP + ko);
-      acc = mfma16(*(const f16x8*)(wt + (size_t)(kc * 64 + lane) * 8), a, acc);
+      acc = mfma16(fa[kc % kAhead], fb[kc % kAhead], acc);
+      if (kc + kAhead < 18) {
+        fa[kc % kAhead] = frag_a(kc + kAhead);
+        fb[kc % kAhead] = frag_b(kc + kAhead);
+      }
+      // keep the reads where they are: the scheduler would otherwise sink them to just
+      // before their matrix instruction and expose the LDS latency at every step
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (c * 32 + j < NS) {
 #pragma unroll
