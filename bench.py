@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=None)
+    p.add_argument("--nz", type=int, default=None, help="tune workload: Nz (default n; 1: 2-D)")
     p.add_argument("--workload", default="ch3d", choices=["ch3d", "sweep", "copy", "ensemble",
                                                           "learned", "tune", "diffuse", "reduce",
                                                           "rhs"])
@@ -659,21 +660,24 @@ def diffuse(a):
 
 
 def tune(a):
-    """The tile autotuner's measurements for the 3-D workload's grid (default 256^3): ms per
-    RK4 step of every candidate shape, one-stage and two-stage kernels, FP32 and FP64."""
+    """The tile autotuner's measurements for an n x n x nz grid (default 256^3; --nz 1 for
+    2-D): ms per RK4 step of every candidate shape and kernel family (kind 0: one kernel per
+    stage, 1 and 2: two-stage kernels, 3: one kernel per step), FP32 and FP64."""
     gc = entry.load_package()
     n = a.n or 256
-    g = gc.Grid.make(n, n, n, 1.0)
+    nz = a.nz if a.nz is not None else n
+    g = gc.Grid.make(n, n, nz, 1.0)
     rows = []
     with gc.Context() as ctx:
         for dtype, name, w in ((gc.F32, "f32", 4), (gc.F64, "f64", 8)):
+            gc.ch_tune(ctx, g, dtype)  # first launches of every candidate
             best, ms = gc.ch_tune(ctx, g, dtype)
             for i, m in enumerate(ms):
                 rows.append({"dtype": name, "id": i, "tile_ox_oy_zc": gc.ch_config_shape(g, i),
-                             "two_stage": gc.ch_config_fused(g, i), "ms_per_step": m,
-                             "hbm_frac_16W": WORDS_PER_SITE_STEP * w * n ** 3 / (m / 1e3) / 1e9
-                             / HBM_PEAK_GBS, "best": i == best})
-    return {"metric": "RK4 ms per step by tile shape", "grid": [n, n, n], "rows": rows}
+                             "kind": gc.ch_config_kind(g, i), "ms_per_step": m,
+                             "hbm_frac_16W": WORDS_PER_SITE_STEP * w * n * n * nz / (m / 1e3)
+                             / 1e9 / HBM_PEAK_GBS, "best": i == best})
+    return {"metric": "RK4 ms per step by tile shape", "grid": [n, n, nz], "rows": rows}
 
 
 def reduce(a):
