@@ -713,6 +713,149 @@ __global__ __launch_bounds__(256) void k_bih_sep(BihSepCoef c, const float* __re
   }
 }
 
+// ---- FP32 order-2 biharmonic: the Laplacian of the Laplacian -----------------------------
+//
+// With the order-2 stencils the fourth-derivative weights (1, -4, 6, -4, 1) are the second-
+// derivative weights (1, -2, 1) applied twice and every mixed term is a product of two second
+// differences, so the order-2 biharmonic equals L(L f) for the 7-point Laplacian L with
+// weights 1/h^2 (ihx4 = ihx2^2).  The FP32 form evaluates it that way: per plane l = L f on
+// the tile + 1 ring (LDS plane of f with a 2-site halo), then L l on the tile (LDS plane of l
+// with a 1-site halo); z neighbours of both come from per-column register queues.  Each f
+// and l plane is double-buffered in LDS, so one barrier per plane suffices: iteration k
+// stages f of plane q = z0 - 3 + k (tile and ring-1 columns from the owners' queues, the
+// outer ring from a prefetch), forms l(q), and forms the output plane q - 1 from l(q - 1)
+// (staged in the previous iteration) and the l queue.
+template <int OY>
+__global__ __launch_bounds__(256) void k_bih_ll(LapCoef<float> lc, const float* __restrict__ f,
+                                                float* __restrict__ out, Dims d, int ntx,
+                                                int nty, int zlen) {
+  constexpr int OX = 64, NT = 256, ROWS = NT / OX, OR = OX * OY / NT;
+  constexpr int FW = OX + 4, FH = OY + 4, FN = FW * FH;  // f plane, halo 2
+  constexpr int LW = OX + 2, LH = OY + 2, LN = LW * LH;  // l plane, halo 1
+  constexpr int NR1 = LN - OX * OY, NR2 = FN - LN;       // ring-1 columns, outer-ring sites
+  static_assert(NR1 <= NT && NR2 <= NT && (OX * OY) % NT == 0, "one ring site per thread");
+  __shared__ float fb[2][FN];
+  __shared__ float lb[2][LN];
+  const int t = threadIdx.x;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)ntx);
+  const unsigned rr = bid / (unsigned)ntx;
+  const int ty = (int)(rr % (unsigned)nty);
+  const int zc = (int)(rr / (unsigned)nty);
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
+  const int nout = min(zlen, d.nz - z0);
+  const int ox = t % OX, oy0 = t / OX;
+  auto goff = [&](int px, int py) {  // f-plane coordinates (halo 2) -> wrapped plane offset
+    return (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny));
+  };
+  // tile columns
+  unsigned coff[OR];
+  bool cin[OR];
+#pragma unroll
+  for (int q = 0; q < OR; ++q) {
+    const int oy = oy0 + q * ROWS;
+    coff[q] = goff(ox + 2, oy + 2);
+    cin[q] = x0 + ox < d.nx && y0 + oy < d.ny;
+  }
+  // the ring-1 column this thread owns (l on it), in l-plane coordinates
+  int r1x = -1, r1y = 0;
+  if (t < NR1) {
+    if (t < LW) {
+      r1x = t;
+      r1y = 0;
+    } else if (t < 2 * LW) {
+      r1x = t - LW;
+      r1y = LH - 1;
+    } else {
+      const int r = t - 2 * LW;
+      r1y = 1 + (r >> 1);
+      r1x = (r & 1) ? LW - 1 : 0;
+    }
+  }
+  const unsigned r1off = r1x >= 0 ? goff(r1x + 1, r1y + 1) : 0u;
+  // the outer-ring site this thread stages, in f-plane coordinates
+  int r2s = -1;
+  unsigned r2off = 0u;
+  if (t < NR2) {
+    int px, py;
+    if (t < FW) {
+      px = t;
+      py = 0;
+    } else if (t < 2 * FW) {
+      px = t - FW;
+      py = FH - 1;
+    } else {
+      const int r = t - 2 * FW;
+      py = 1 + (r >> 1);
+      px = (r & 1) ? FW - 1 : 0;
+    }
+    r2s = py * FW + px;
+    r2off = goff(px, py);
+  }
+  auto plane_ptr = [&](int z) { return f + (size_t)d.plane * (size_t)wrap_any(z, d.nz); };
+  float fq[OR][3] = {}, f1q[3] = {}, lq[OR][3] = {};
+  float fn[OR], f1n = 0.f, r2v = 0.f;
+  auto fetch = [&](int z_cols, int z_ring2) {  // columns of plane z_cols, outer ring of z_ring2
+    const float* pc = plane_ptr(z_cols);
+#pragma unroll
+    for (int q = 0; q < OR; ++q) fn[q] = pc[coff[q]];
+    if (r1x >= 0) f1n = pc[r1off];
+    if (r2s >= 0) r2v = plane_ptr(z_ring2)[r2off];
+  };
+  fetch(z0 - 2, z0 - 3);
+  const int iters = nout + 4;
+  for (int k = 0; k < iters; ++k) {
+    const int b = k & 1;
+    // plane p = z0 - 2 + k arrives; the queues then hold p - 2, p - 1, p
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      fq[q][0] = fq[q][1];
+      fq[q][1] = fq[q][2];
+      fq[q][2] = fn[q];
+    }
+    f1q[0] = f1q[1];
+    f1q[1] = f1q[2];
+    f1q[2] = f1n;
+    // stage f of plane q = p - 1
+    float* F = fb[b];
+#pragma unroll
+    for (int q = 0; q < OR; ++q) F[(oy0 + q * ROWS + 2) * FW + ox + 2] = fq[q][1];
+    if (r1x >= 0) F[(r1y + 1) * FW + r1x + 1] = f1q[1];
+    if (r2s >= 0) F[r2s] = r2v;
+    __syncthreads();
+    if (k + 1 < iters) fetch(z0 - 1 + k, z0 - 2 + k);
+    // l(q) on the tile and the ring-1 column
+    float* Lp = lb[b];
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      const int oy = oy0 + q * ROWS;
+      const int o = (oy + 2) * FW + ox + 2;
+      const float l = lap7(lc, fq[q][1], F[o - 1], F[o + 1], F[o - FW], F[o + FW], fq[q][0], fq[q][2]);
+      Lp[(oy + 1) * LW + ox + 1] = l;
+      lq[q][0] = lq[q][1];
+      lq[q][1] = lq[q][2];
+      lq[q][2] = l;
+    }
+    if (r1x >= 0) {
+      const int o = (r1y + 1) * FW + r1x + 1;
+      Lp[r1y * LW + r1x] = lap7(lc, f1q[1], F[o - 1], F[o + 1], F[o - FW], F[o + FW], f1q[0], f1q[2]);
+    }
+    // output plane q - 1 = z0 - 4 + k from l(q - 1) (staged in the previous iteration)
+    if (k >= 4) {
+      const float* Lm = lb[b ^ 1];
+      const size_t zo = (size_t)d.plane * (size_t)(z0 - 4 + k);
+#pragma unroll
+      for (int q = 0; q < OR; ++q) {
+        const int oy = oy0 + q * ROWS;
+        const int o = (oy + 1) * LW + ox + 1;
+        if (cin[q])
+          out[zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] =
+              lap7(lc, lq[q][1], Lm[o - 1], Lm[o + 1], Lm[o - LW], Lm[o + LW], lq[q][0], lq[q][2]);
+      }
+    }
+  }
+}
+
 // ---- vector- and tensor-field operators: z-accumulating engine ----------------------------
 //
 // The inputs are AoS (CIN values per site).  A block owns a 64 x 8 tile (one column per
@@ -1068,8 +1211,14 @@ void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
   const Inv<double> i2 = inv_h2(g);
   // inv_h4 = inv_h2 * inv_h2 in double (Biharmonic.hpp:62-64), then cast
   const OpBih<T, O> op{T(i2.x), T(i2.y), T(i2.z), T(i2.x * i2.x), T(i2.y * i2.y), T(i2.z * i2.z)};
-  if constexpr (std::is_same<T, float>::value) {
-    // FP32: the separable form (k_bih_sep), 64 x 8 tiles
+  if constexpr (std::is_same<T, float>::value && O == 2) {
+    // FP32 order 2: the Laplacian of the Laplacian (k_bih_ll), 64 x 8 tiles
+    const LapCoef<float> lc = make_lap_coef<float>(float(i2.x), float(i2.y), float(i2.z));
+    const int ntx = (d.nx + 63) / 64, nty = (d.ny + 7) / 8, nzc = (d.nz + kZLen - 1) / kZLen;
+    hipLaunchKernelGGL((k_bih_ll<8>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
+                       ctx->stream, lc, (const float*)in, (float*)out, d, ntx, nty, kZLen);
+  } else if constexpr (std::is_same<T, float>::value) {
+    // FP32 order 4: the separable form (k_bih_sep), 64 x 8 tiles
     const BihSepCoef c{float(i2.x * i2.x), float(i2.y * i2.y), float(i2.z * i2.z),
                        float(2.0 * i2.x * i2.y), float(2.0 * i2.x * i2.z),
                        float(2.0 * i2.y * i2.z)};
