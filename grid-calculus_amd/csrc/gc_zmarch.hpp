@@ -22,6 +22,10 @@ template <int R, int NF, int OX, int OY>
 struct ZTile {
   static constexpr int NT = 256;
   static constexpr int PW = OX + 2 * R, PH = OY + 2 * R, PN = PW * PH, NP = 2 * R + 1;
+  // Ring slots.  One field: one slot more than a site reads, so one barrier per plane
+  // suffices (k_zmarch).  Two fields: the extra slots would cost a block per CU of occupancy
+  // (measured slower), so the ring keeps 2R + 1 slots and two barriers per plane.
+  static constexpr int NS = NF == 1 ? NP + 1 : NP;
   static constexpr int PL = (PN + NT - 1) / NT;  // plane slots filled per thread
   static constexpr int ROWS = NT / OX;
   static constexpr int OR = OX * OY / NT;        // output sites per thread per plane
@@ -31,7 +35,7 @@ struct ZTile {
 // Neighbour access into the ring: F(field, dx, dy, dz) at the current output site.
 template <class T, int R, int PN, int NP, int PW>
 struct RingAt {
-  const T* ring;     // [NF][NP][PN]
+  const T* ring;     // [NF][NP][PN] (NP: slots of the ring)
   const int* slot;   // slot of plane z + dz at slot[dz + R]
   int base;          // (oy + R) * PW + (ox + R)
   __device__ __forceinline__ T operator()(int f, int dx, int dy, int dz) const {
@@ -49,7 +53,7 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
                                                 unsigned long long* __restrict__ bad) {
   constexpr int R = Op::R, NF = Op::NF;
   using Z = ZTile<R, NF, OX, OY>;
-  __shared__ T ring[NF * Z::NP * Z::PN];
+  __shared__ T ring[NF * Z::NS * Z::PN];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
@@ -90,10 +94,14 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int s = 0; s < Z::PL; ++s)
-        if (pin[s]) ring[(f * Z::NP + slot) * Z::PN + t + s * Z::NT] = reg[f][s];
+        if (pin[s]) ring[(f * Z::NS + slot) * Z::PN + t + s * Z::NT] = reg[f][s];
   };
 
-  // plane z0 - R + i lives in slot i % NP
+  // Plane z0 - R + i lives in slot i % NS.  With NS = 2R + 2 the plane stored in iteration k
+  // goes to the slot that iteration k - 1 did not read, and one barrier per plane suffices: a
+  // thread stores in iteration k only after every thread has passed the barrier of iteration
+  // k - 1, hence finished iteration k - 2, the last reader of that slot.  With NS = 2R + 1 the
+  // slot was read in iteration k - 1, so a second barrier closes each iteration.
   for (int i = 0; i < 2 * R; ++i) {
     load(z0 - R + i);
     store(i);
@@ -101,22 +109,22 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
   load(z0 + R);
   const int ox = t % OX, oy0 = t / OX;
   for (int k = 0; k < nout; ++k) {
-    store((k + 2 * R) % Z::NP);
+    store((k + 2 * R) % Z::NS);
     __syncthreads();
     if (k + 1 < nout) load(z0 + k + 1 + R);
     int slot[Z::NP];
 #pragma unroll
-    for (int j = 0; j < Z::NP; ++j) slot[j] = (k + j) % Z::NP;
+    for (int j = 0; j < Z::NP; ++j) slot[j] = (k + j) % Z::NS;
     const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
 #pragma unroll
     for (int q = 0; q < Z::OR; ++q) {
       const int oy = oy0 + q * Z::ROWS;
       if (x0 + ox < d.nx && y0 + oy < d.ny) {
-        const RingAt<T, R, Z::PN, Z::NP, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
+        const RingAt<T, R, Z::PN, Z::NS, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
         op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
       }
     }
-    __syncthreads();
+    if constexpr (Z::NS == Z::NP) __syncthreads();
   }
   if constexpr (CHECK) {
     for (int st = 32; st >= 1; st >>= 1) {
