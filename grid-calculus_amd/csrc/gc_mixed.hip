@@ -39,7 +39,7 @@ struct OpMixed {
   using SX = AxisSt<NX, O>;
   using SY = AxisSt<NY, O>;
   using SZ = AxisSt<NZ, O>;
-  static constexpr int R = cmax(cmax(SX::R, SY::R), cmax(SZ::R, 1)), NF = 1;
+  static constexpr int R = cmax(cmax(SX::R, SY::R), cmax(SZ::R, 1));
   T scale;
   template <class A>
   __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
@@ -72,7 +72,7 @@ gc_status run_mixed(gc_context* ctx, const gc_grid* g, const void* in, void* out
   for (int n = 0; n < NX; ++n) scale /= g->hx;
   for (int n = 0; n < NY; ++n) scale /= g->hy;
   for (int n = 0; n < NZ; ++n) scale /= g->hz;
-  zmarch_launch<T>(ctx, dims_of(g), Op{T(scale)}, (const T*)in, nullptr, (T*)out);
+  zmarch_launch<T>(ctx, dims_of(g), Op{T(scale)}, (const T*)in, (T*)out);
   GC_LAUNCHED();
   return GC_OK;
 }

@@ -348,10 +348,11 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_pairwise_leaves(L ld, size_
 }
 
 // Perfect-tree sum of consecutive groups of min(cnt, 256) values.
-__global__ __launch_bounds__(256) void k_tree_pass(const double* __restrict__ in, unsigned cnt,
-                                                   double* __restrict__ out) {
-  __shared__ double sh[4];
-  const unsigned g = cnt < 256u ? cnt : 256u;
+__global__ __launch_bounds__(64 * kWaves) void k_tree_pass(const double* __restrict__ in,
+                                                            unsigned cnt,
+                                                            double* __restrict__ out) {
+  __shared__ double sh[kWaves];
+  const unsigned g = cnt < 64u * kWaves ? cnt : 64u * kWaves;
   const unsigned idx = blockIdx.x * g + threadIdx.x;
   double s = (threadIdx.x < g) ? in[idx] : 0.0;
   s = block_tree_sum(s, g, sh);
@@ -392,9 +393,9 @@ gc_status pairwise_launch(hipStream_t stream, L ld, size_t n, void* scr, double*
   GC_LAUNCHED();
   unsigned cnt = nb;
   while (cnt > 1) {
-    const unsigned g = cnt < 256u ? cnt : 256u;
+    const unsigned g = cnt < 64u * kWaves ? cnt : 64u * kWaves;
     const unsigned blocks = cnt / g;
-    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(256), 0, stream, a, cnt,
+    hipLaunchKernelGGL(k_tree_pass, dim3(blocks), dim3(64 * kWaves), 0, stream, a, cnt,
                        blocks == 1 ? final_out : b);
     GC_LAUNCHED();
     cnt = blocks;

@@ -31,7 +31,7 @@ namespace {
 // diff/Laplacian.hpp:56-71
 template <class T, int O>
 struct OpLap {
-  static constexpr int R = D2<O>::R, NF = 1;
+  static constexpr int R = D2<O>::R;
   static constexpr int kTileRows = 16;  // 512^3 FP64: O2 0.444 -> 0.428 ms, O4 0.562 -> 0.524
   T ihx2, ihy2, ihz2;
   template <class A>
@@ -53,7 +53,7 @@ struct OpLap {
 // diff/Gradient.hpp:56-70 (scalar -> Vec3, AoS)
 template <class T, int O>
 struct OpGrad {
-  static constexpr int R = D1<O>::R, NF = 1;
+  static constexpr int R = D1<O>::R;
   static constexpr int kTileRows = 16;  // 512^3 FP64: O2 1.082 -> 0.978 ms, O4 1.124 -> 1.007
   T ihx, ihy, ihz;
   template <class A>
@@ -79,7 +79,7 @@ struct OpGrad {
 // the reference's order.
 template <class T, int O>
 struct OpBih {
-  static constexpr int R = D4<O>::R, NF = 1;
+  static constexpr int R = D4<O>::R;
   static_assert(D4<O>::R >= D2<O>::R, "ring radius");
   T ihx2, ihy2, ihz2, ihx4, ihy4, ihz4;
   template <class A>
@@ -114,58 +114,32 @@ struct OpBih {
   }
 };
 
-// fvm/CellLaplacian.hpp:48-50
-template <class T>
-__device__ __forceinline__ T hmean(T a, T b) {
-  return T(2) * a * b / (a + b);
-}
-
-// fvm/CellLaplacian.hpp:115-141; field 0 = psi, field 1 = D
-template <class T>
-struct OpCell {
-  static constexpr int R = 1, NF = 2;
-  T ihx2, ihy2, ihz2;
-  template <class A>
-  __device__ __forceinline__ void operator()(const A& F, T* __restrict__ out, size_t c) const {
-    const T pc = F(0, 0, 0, 0);
-    const T Dc = F(1, 0, 0, 0);
-    const T dxp = hmean(Dc, F(1, 1, 0, 0));
-    const T dxm = hmean(F(1, -1, 0, 0), Dc);
-    const T dyp = hmean(Dc, F(1, 0, 1, 0));
-    const T dym = hmean(F(1, 0, -1, 0), Dc);
-    const T dzp = hmean(Dc, F(1, 0, 0, 1));
-    const T dzm = hmean(F(1, 0, 0, -1), Dc);
-    const T rx = (dxp * (F(0, 1, 0, 0) - pc) - dxm * (pc - F(0, -1, 0, 0))) * ihx2;
-    const T ry = (dyp * (F(0, 0, 1, 0) - pc) - dym * (pc - F(0, 0, -1, 0))) * ihy2;
-    const T rz = (dzp * (F(0, 0, 0, 1) - pc) - dzm * (pc - F(0, 0, 0, -1))) * ihz2;
-    out[c] = rx + ry + rz;
-  }
-};
-
-
-// fvm::cellLaplacian (fvm/CellLaplacian.hpp:115-141) on the z-march with shared face means and the D > 0 check fused in.
-// The harmonic mean of a face depends on D only, so each plane's x and y faces are formed
-// once (a faces pass into LDS after the plane is staged) and used by both cells they separate;
-// the z face above a site becomes the face below it on the next plane (a register per
-// column).  About 3 face means per site instead of 6, each formed by face_coef(D_lower,
-// D_upper) and the fluxes by fvm_k (gc_arith.hpp): in FP64 the reference's harmonic mean and
-// order, so the result is the per-cell evaluation's bit for bit; in FP32 (the precision this
-// kernel serves) the hardware reciprocal and fused multiply-adds.  Every D value the block
-// stages is tested
-// (!(D > 0)) and the smallest failing flat
-// index goes to *bad by atomicMin, so the caller needs no separate scan of D; on a failure
-// the output values are meaningless and the caller reports the error instead.
-template <class T, int OX, int OY>
-__global__ __launch_bounds__(256) void k_cell_zf(T ihx2, T ihy2, T ihz2,
-                                                 const T* __restrict__ psi,
-                                                 const T* __restrict__ D, T* __restrict__ out,
-                                                 unsigned long long* __restrict__ bad, Dims d,
-                                                 int ntx, int nty, int zlen) {
-  using Z = ZTile<1, 2, OX, OY>;
-  constexpr int PW = Z::PW, PN = Z::PN;
-  __shared__ T ring[2 * 3 * PN];           // [field][slot][site], field 0 psi, 1 D
-  __shared__ T fx[OY * (OX + 1)];          // face between local x = i - 1 and i, row y
-  __shared__ T fy[(OY + 1) * OX];          // face between local y = r - 1 and r, column x
+// fvm::cellLaplacian (fvm/CellLaplacian.hpp:115-141) with register queues along z.  A block
+// of 64 OY / OR threads owns a 64 x OY tile of columns and a chunk of planes; each thread
+// owns OR consecutive rows of one column and keeps psi at z - 1, z, z + 1, D at z, z + 1 and the face
+// below each of its sites in registers.  LDS holds the psi and D planes z (tile + 1-site
+// halo), double-buffered, so one barrier per plane suffices: iteration k stages plane z0 + k
+// (tile sites from the owners' registers, halo sites requested PF planes ahead) into
+// buffer k & 1, whose previous readers (iteration k - 2) have all passed the barrier of
+// iteration k - 1.  Face coefficients: x faces of both sides, the y face above each row (the
+// face below it is the row beneath's, held by the same thread), the z face above (the face
+// below is the previous plane's), each by face_coef(D_lower, D_upper) and the fluxes by fvm_k
+// (gc_arith.hpp): FP64 is the reference's harmonic mean and order (the per-cell
+// evaluation's bits), FP32 the hardware reciprocal and fused multiply-adds.  Every D value
+// the block loads is tested (!(D > 0)) and the smallest failing flat index goes to *bad by
+// atomicMin; on a failure the output values are meaningless and the caller reports the error.
+template <class T, int OY, int OR, int PF>
+__global__ __launch_bounds__(64 * OY / OR) void k_cell_zq(T ihx2, T ihy2, T ihz2,
+                                                          const T* __restrict__ psi,
+                                                          const T* __restrict__ D,
+                                                          T* __restrict__ out,
+                                                          unsigned long long* __restrict__ bad,
+                                                          Dims d, int ntx, int nty, int zlen) {
+  constexpr int OX = 64, NT = OX * OY / OR;
+  constexpr int PW = OX + 2, PH = OY + 2, PN = PW * PH, NR = PN - OX * OY;
+  static_assert(OY % OR == 0 && NR <= NT, "one halo site per thread");
+  __shared__ T pb[2][PN];
+  __shared__ T db[2][PN];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
@@ -174,85 +148,129 @@ __global__ __launch_bounds__(256) void k_cell_zf(T ihx2, T ihy2, T ihz2,
   const int zc = (int)(rr / (unsigned)nty);
   const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
   const int nout = min(zlen, d.nz - z0);
-
-  unsigned poff[Z::PL];
-  bool pin[Z::PL];
+  const int ox = t % OX, oyb = (t / OX) * OR;  // this thread's rows oyb .. oyb + OR - 1
+  auto goff = [&](int px, int py) {  // plane coordinates (halo 1) -> wrapped plane offset
+    return (unsigned)(wrap_any(x0 + px - 1, d.nx) + d.nx * wrap_any(y0 + py - 1, d.ny));
+  };
+  unsigned coff[OR];
+  bool cin[OR];
 #pragma unroll
-  for (int s = 0; s < Z::PL; ++s) {
-    const int idx = t + s * Z::NT;
-    pin[s] = idx < PN;
-    const int px = idx % PW, py = idx / PW;
-    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - 1, d.nx) + d.nx * wrap_any(y0 + py - 1, d.ny))
-                     : 0u;
+  for (int q = 0; q < OR; ++q) {
+    coff[q] = goff(ox + 1, oyb + q + 1);
+    cin[q] = x0 + ox < d.nx && y0 + oyb + q < d.ny;
   }
-  T rp[Z::PL], rd[Z::PL];
+  // the halo site this thread stages: top row, bottom row, then left and right ends of rows
+  int hs = -1;
+  unsigned hoff = 0u;
+  if (t < NR) {
+    int px, py;
+    if (t < PW) {
+      px = t;
+      py = 0;
+    } else if (t < 2 * PW) {
+      px = t - PW;
+      py = PH - 1;
+    } else {
+      const int r = t - 2 * PW;
+      py = 1 + (r >> 1);
+      px = (r & 1) ? PW - 1 : 0;
+    }
+    hs = py * PW + px;
+    hoff = goff(px, py);
+  }
+  auto zoff = [&](int z) { return (size_t)d.plane * (size_t)wrap_any(z, d.nz); };
   unsigned long long first = ~0ull;
-  auto load = [&](int plane) {
-    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+  auto check = [&](T v, size_t idx) {
+    if (!(v > T(0))) first = min(first, (unsigned long long)idx);
+  };
+  // Register queues, index j of a column queue = plane z - 1 + j (psi) or z + j (D); the last
+  // entry receives the plane requested PF iterations before its use.  Halo queue entry j =
+  // plane z + j.
+  T ps[OR][3 + PF] = {}, ds[OR][2 + PF] = {}, fzm[OR];
+  T hps[PF + 1] = {}, hds[PF + 1] = {};
+  auto fetch_cols = [&](int z, int j, int jd) {
+    const size_t zo = zoff(z);
 #pragma unroll
-    for (int s = 0; s < Z::PL; ++s) {
-      if (pin[s]) {
-        rp[s] = psi[zo + poff[s]];
-        rd[s] = D[zo + poff[s]];
-        if (!(rd[s] > T(0))) first = min(first, (unsigned long long)(zo + poff[s]));
+    for (int q = 0; q < OR; ++q) {
+      ps[q][j] = psi[zo + coff[q]];
+      if (jd >= 0) {
+        ds[q][jd] = D[zo + coff[q]];
+        check(ds[q][jd], zo + coff[q]);
       }
     }
   };
-  auto store = [&](int slot) {
-#pragma unroll
-    for (int s = 0; s < Z::PL; ++s) {
-      if (pin[s]) {
-        ring[slot * PN + t + s * Z::NT] = rp[s];
-        ring[(3 + slot) * PN + t + s * Z::NT] = rd[s];
-      }
+  auto fetch_halo = [&](int z, int j) {
+    if (hs >= 0) {
+      const size_t zo = zoff(z);
+      hps[j] = psi[zo + hoff];
+      hds[j] = D[zo + hoff];
+      check(hds[j], zo + hoff);
     }
   };
-  load(z0 - 1);
-  store(0);
-  load(z0);
-  store(1);
-  load(z0 + 1);
-  const int ox = t % OX, oy0 = t / OX;
-  T fzm[Z::OR];  // the face below each of this thread's sites
+  {
+    T dm[OR];
+    const size_t zo = zoff(z0 - 1);
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      dm[q] = D[zo + coff[q]];
+      check(dm[q], zo + coff[q]);
+    }
+    fetch_cols(z0 - 1, 0, -1);
+#pragma unroll
+    for (int j = 0; j <= PF; ++j) fetch_cols(z0 + j, j + 1, j);
+#pragma unroll
+    for (int j = 0; j < PF; ++j) fetch_halo(z0 + j, j);
+#pragma unroll
+    for (int q = 0; q < OR; ++q) fzm[q] = face_coef(dm[q], ds[q][0], ihz2);
+  }
   for (int k = 0; k < nout; ++k) {
-    const int sm = k % 3, sc = (k + 1) % 3, sp = (k + 2) % 3;  // planes z-1, z, z+1
-    store(sp);
-    __syncthreads();
-    if (k + 1 < nout) load(z0 + k + 2);
-    const T* Dc = ring + (3 + sc) * PN;
-    for (int i = t; i < OY * (OX + 1); i += Z::NT) {
-      const int r = i / (OX + 1), c = i % (OX + 1);
-      const int o = (r + 1) * PW + c;
-      fx[i] = face_coef(Dc[o], Dc[o + 1], ihx2);
-    }
-    for (int i = t; i < (OY + 1) * OX; i += Z::NT) {
-      const int r = i / OX, c = i % OX;
-      const int o = r * PW + c + 1;
-      fy[i] = face_coef(Dc[o], Dc[o + PW], ihy2);
-    }
-    __syncthreads();
-    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
-    const T* Pm = ring + sm * PN;
-    const T* Pc = ring + sc * PN;
-    const T* Pp = ring + sp * PN;
-    const T* Dm = ring + (3 + sm) * PN;
-    const T* Dp = ring + (3 + sp) * PN;
+    const int b = k & 1;
+    T* P = pb[b];
+    T* Q = db[b];
 #pragma unroll
-    for (int q = 0; q < Z::OR; ++q) {
-      const int oy = oy0 + q * Z::ROWS;
-      const int o = (oy + 1) * PW + (ox + 1);
-      const T pc = Pc[o], dcen = Dc[o];
-      if (k == 0) fzm[q] = face_coef(Dm[o], dcen, ihz2);
-      const T dxp = fx[oy * (OX + 1) + ox + 1], dxm = fx[oy * (OX + 1) + ox];
-      const T dyp = fy[(oy + 1) * OX + ox], dym = fy[oy * OX + ox];
-      const T dzp = face_coef(dcen, Dp[o], ihz2), dzm = fzm[q];
-      fzm[q] = dzp;
-      const T v = fvm_k(pc, Pc[o - 1], Pc[o + 1], Pc[o - PW], Pc[o + PW], Pm[o], Pp[o], dxm, dxp,
-                        dym, dyp, dzm, dzp, ihx2, ihy2, ihz2);
-      if (x0 + ox < d.nx && y0 + oy < d.ny)
-        out[zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy)] = v;
+    for (int q = 0; q < OR; ++q) {
+      const int o = (oyb + q + 1) * PW + ox + 1;
+      P[o] = ps[q][1];
+      Q[o] = ds[q][0];
+    }
+    if (hs >= 0) {
+      P[hs] = hps[0];
+      Q[hs] = hds[0];
     }
     __syncthreads();
+    if (k + PF < nout) {
+      fetch_halo(z0 + k + PF, PF);
+      fetch_cols(z0 + k + 1 + PF, 2 + PF, 1 + PF);
+    }
+    const size_t zo = (size_t)d.plane * (size_t)(z0 + k);
+    T fyb = face_coef(Q[oyb * PW + ox + 1], ds[0][0], ihy2);  // face below row oyb
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      const int o = (oyb + q + 1) * PW + ox + 1;
+      const T dcq = ds[q][0];
+      const T fxm = face_coef(Q[o - 1], dcq, ihx2);
+      const T fxp = face_coef(dcq, Q[o + 1], ihx2);
+      const T dup = q + 1 < OR ? ds[q + 1][0] : Q[o + PW];
+      const T fyp = face_coef(dcq, dup, ihy2);
+      const T fzp = face_coef(dcq, ds[q][1], ihz2);
+      const T v = fvm_k(ps[q][1], P[o - 1], P[o + 1], P[o - PW], P[o + PW], ps[q][0], ps[q][2],
+                        fxm, fxp, fyb, fyp, fzm[q], fzp, ihx2, ihy2, ihz2);
+      if (cin[q]) out[zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oyb + q)] = v;
+      fyb = fyp;
+      fzm[q] = fzp;
+    }
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+#pragma unroll
+      for (int j = 0; j < 2 + PF; ++j) ps[q][j] = ps[q][j + 1];
+#pragma unroll
+      for (int j = 0; j < 1 + PF; ++j) ds[q][j] = ds[q][j + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      hps[j] = hps[j + 1];
+      hds[j] = hds[j + 1];
+    }
   }
   for (int st = 32; st >= 1; st >>= 1) {
     const unsigned long long o = __shfl_xor(first, st);
@@ -413,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   constexpr int R = S4::R, R2 = S2::R;  // 3, 2
   constexpr int QP = 2 * R2 + 1;        // 5: queue length, ring slots, unroll period
   static_assert(R == R2 + 1 && QP >= R + 2, "centre column = queue plus one plane each side");
-  using Z = ZTile<R, 1, 64, 4>;
+  using Z = ZTile<R, 64, 4>;
   static_assert(Z::OR == 1, "one column per thread");
   __shared__ T ring[QP * Z::PN];
   const int t = threadIdx.x;
@@ -1137,7 +1155,7 @@ static gc_status run_laplacian(gc_context* ctx, const gc_grid* g, int order, con
   // order 4 reads 4 z neighbours: the register queue pays (0.49 against 0.55 ms FP64 on
   // 512^3); order 2 runs as fast on the plane ring
   if (order == 2)
-    zmarch_launch<T>(ctx, d, OpLap<T, 2>{T(ih.x), T(ih.y), T(ih.z)}, in, nullptr, out);
+    zmarch_launch<T>(ctx, d, OpLap<T, 2>{T(ih.x), T(ih.y), T(ih.z)}, in, out);
   else
     axis_launch<T>(ctx, d, OpLap<T, 4>{T(ih.x), T(ih.y), T(ih.z)}, in, out);
   GC_LAUNCHED();
@@ -1176,7 +1194,7 @@ void first_deriv_launch(int which, gc_context* ctx, const gc_grid* g, const void
   switch (which) {
     case 0:
       // the plane ring: the register-queue engine measured 7 % slower for the gradient
-      zmarch_launch<T>(ctx, d, OpGrad<T, O>{x, y, z}, (const T*)in, nullptr, (T*)out);
+      zmarch_launch<T>(ctx, d, OpGrad<T, O>{x, y, z}, (const T*)in, (T*)out);
       break;
     case 1:
       vec_launch<T, O>(ctx, d, OpGradVec<T>{x, y, z}, (const T*)in, (T*)out);
@@ -1233,14 +1251,14 @@ void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
     hipLaunchKernelGGL((k_bih_zq<T, O>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
                        ctx->stream, op, (const T*)in, (T*)out, d, ntx, nty, kZLen);
   } else {
-    zmarch_launch<T>(ctx, d, op, (const T*)in, nullptr, (T*)out);
+    zmarch_launch<T>(ctx, d, op, (const T*)in, (T*)out);
   }
 }
 
 }  // namespace
 
 namespace gc {
-// The checked operator: k_cell_zf, then one 8-byte copy for the D > 0 verdict.
+// The checked operator: k_cell_zq, then one 8-byte copy for the D > 0 verdict.
 gc_status cell_laplacian_checked(gc_context* ctx, const gc_grid* g, gc_dtype dt, const void* psi,
                                  const void* D, void* out, int64_t* first_bad) {
   const Dims d = dims_of(g);
@@ -1250,18 +1268,23 @@ gc_status cell_laplacian_checked(gc_context* ctx, const gc_grid* g, gc_dtype dt,
   unsigned long long* bad = (unsigned long long*)scr;
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, ctx->stream, bad, ~0ull);
   GC_LAUNCHED();
-  const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + kZOY - 1) / kZOY;
-  const dim3 grid((unsigned)(ntx * nty * ((d.nz + kZLen - 1) / kZLen)));
-  if (dt == GC_F64)  // per-cell face means: at FP64 the shared-face kernel's smaller
-                     // occupancy costs more than its saved divisions (1.07 against 0.99 ms
-                     // on 512^3; FP32 0.55 against 0.63 ms the other way round)
-    zmarch_launch<double, OpCell<double>, true>(ctx, d, OpCell<double>{ih.x, ih.y, ih.z},
-                                                (const double*)psi, (const double*)D,
-                                                (double*)out, bad);
-  else
-    hipLaunchKernelGGL((k_cell_zf<float, kZOX, kZOY>), grid, dim3(256), 0, ctx->stream,
-                       float(ih.x), float(ih.y), float(ih.z), (const float*)psi,
-                       (const float*)D, (float*)out, bad, d, ntx, nty, kZLen);
+  // 64 x 16 tiles of 4-row columns (256 threads) for FP64, 64 x 32 of 4-row columns (512
+  // threads) for FP32: the fastest of 64 x {4, 8, 16, 32} tiles with 1, 2 or 4 rows per
+  // thread on 512^3 (FP64 0.92 ms, FP32 0.41 ms)
+  const int ntx = (d.nx + 63) / 64;
+  const int nzc = (d.nz + kZLen - 1) / kZLen;
+  if (dt == GC_F64) {
+    const int nty = (d.ny + 15) / 16;
+    hipLaunchKernelGGL((k_cell_zq<double, 16, 4, 1>), dim3((unsigned)(ntx * nty * nzc)),
+                       dim3(256), 0, ctx->stream, ih.x, ih.y, ih.z, (const double*)psi,
+                       (const double*)D, (double*)out, bad, d, ntx, nty, kZLen);
+  } else {
+    const int nty = (d.ny + 31) / 32;
+    hipLaunchKernelGGL((k_cell_zq<float, 32, 4, 1>), dim3((unsigned)(ntx * nty * nzc)),
+                       dim3(512), 0, ctx->stream, float(ih.x), float(ih.y), float(ih.z),
+                       (const float*)psi, (const float*)D, (float*)out, bad, d, ntx, nty,
+                       kZLen);
+  }
   GC_LAUNCHED();
   unsigned long long h = 0;
   GC_HIP(hipMemcpyAsync(ctx->host_result, bad, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

@@ -12,48 +12,41 @@ namespace {
 // ---- z-marching LDS engine for the scalar-input operators --------------------------------
 //
 // A block owns an OX x OY tile of xy-columns and a chunk of consecutive planes.  It keeps the
-// 2R+1 planes z-R..z+R of every input field (tile + R-site halo, periodic wrap applied while
+// 2R+1 planes z-R..z+R of the input field (tile + R-site halo, periodic wrap applied while
 // filling) in an LDS ring, so each input value is read from memory once per block plus the
 // halo; the next plane is fetched into registers while the current output plane is formed.
 // The operators read neighbours through an accessor and keep the reference's per-axis
 // accumulation order, so results equal the one-site-at-a-time evaluation bit for bit.
 
-template <int R, int NF, int OX, int OY>
+template <int R, int OX, int OY>
 struct ZTile {
   static constexpr int NT = 256;
   static constexpr int PW = OX + 2 * R, PH = OY + 2 * R, PN = PW * PH, NP = 2 * R + 1;
-  // Ring slots.  One field: one slot more than a site reads, so one barrier per plane
-  // suffices (k_zmarch).  Two fields: the extra slots would cost a block per CU of occupancy
-  // (measured slower), so the ring keeps 2R + 1 slots and two barriers per plane.
-  static constexpr int NS = NF == 1 ? NP + 1 : NP;
+  static constexpr int NS = NP + 1;  // ring slots: one more than a site reads (k_zmarch)
   static constexpr int PL = (PN + NT - 1) / NT;  // plane slots filled per thread
   static constexpr int ROWS = NT / OX;
   static constexpr int OR = OX * OY / NT;        // output sites per thread per plane
   static_assert(NT % OX == 0 && (OX * OY) % NT == 0, "tile must match the block");
 };
 
-// Neighbour access into the ring: F(field, dx, dy, dz) at the current output site.
-template <class T, int R, int PN, int NP, int PW>
+// Neighbour access into the ring: F(0, dx, dy, dz) at the current output site.
+template <class T, int R, int PN, int NS, int PW>
 struct RingAt {
-  const T* ring;     // [NF][NP][PN] (NP: slots of the ring)
+  const T* ring;     // [NS][PN]
   const int* slot;   // slot of plane z + dz at slot[dz + R]
   int base;          // (oy + R) * PW + (ox + R)
-  __device__ __forceinline__ T operator()(int f, int dx, int dy, int dz) const {
-    return ring[(f * NP + slot[dz + R]) * PN + base + dy * PW + dx];
+  __device__ __forceinline__ T operator()(int, int dx, int dy, int dz) const {
+    return ring[slot[dz + R] * PN + base + dy * PW + dx];
   }
 };
 
-// CHECK: every value of field 1 the block stages is tested (!(v > 0)) and the smallest
-// failing flat index goes to *bad by atomicMin (the fvm::cellLaplacian D > 0 precondition,
-// checked in the same pass).
-template <class T, int OX, int OY, class Op, bool CHECK = false>
-__global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
-                                                const T* __restrict__ f1, T* __restrict__ out,
-                                                Dims d, int ntx, int nty, int zlen,
-                                                unsigned long long* __restrict__ bad) {
-  constexpr int R = Op::R, NF = Op::NF;
-  using Z = ZTile<R, NF, OX, OY>;
-  __shared__ T ring[NF * Z::NS * Z::PN];
+template <class T, int OX, int OY, class Op>
+__global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f,
+                                                T* __restrict__ out, Dims d, int ntx, int nty,
+                                                int zlen) {
+  constexpr int R = Op::R;
+  using Z = ZTile<R, OX, OY>;
+  __shared__ T ring[Z::NS * Z::PN];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
@@ -73,35 +66,23 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
     poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
                      : 0u;
   }
-  T reg[NF][Z::PL];
-  unsigned long long first = ~0ull;
+  T reg[Z::PL];
   auto load = [&](int plane) {
     const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
 #pragma unroll
-    for (int s = 0; s < Z::PL; ++s) {
-      if (pin[s]) {
-        reg[0][s] = f0[zo + poff[s]];
-        if constexpr (NF > 1) {
-          reg[NF - 1][s] = f1[zo + poff[s]];
-          if constexpr (CHECK)
-            if (!(reg[NF - 1][s] > T(0))) first = min(first, (unsigned long long)(zo + poff[s]));
-        }
-      }
-    }
+    for (int s = 0; s < Z::PL; ++s)
+      if (pin[s]) reg[s] = f[zo + poff[s]];
   };
   auto store = [&](int slot) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int s = 0; s < Z::PL; ++s)
-        if (pin[s]) ring[(f * Z::NS + slot) * Z::PN + t + s * Z::NT] = reg[f][s];
+    for (int s = 0; s < Z::PL; ++s)
+      if (pin[s]) ring[slot * Z::PN + t + s * Z::NT] = reg[s];
   };
 
-  // Plane z0 - R + i lives in slot i % NS.  With NS = 2R + 2 the plane stored in iteration k
-  // goes to the slot that iteration k - 1 did not read, and one barrier per plane suffices: a
-  // thread stores in iteration k only after every thread has passed the barrier of iteration
-  // k - 1, hence finished iteration k - 2, the last reader of that slot.  With NS = 2R + 1 the
-  // slot was read in iteration k - 1, so a second barrier closes each iteration.
+  // Plane z0 - R + i lives in slot i % NS.  With NS = 2R + 2 slots the plane stored in
+  // iteration k goes to the slot that iteration k - 1 did not read, so one barrier per plane
+  // suffices: a thread stores in iteration k only after every thread has passed the barrier
+  // of iteration k - 1, hence finished iteration k - 2, the last reader of that slot.
   for (int i = 0; i < 2 * R; ++i) {
     load(z0 - R + i);
     store(i);
@@ -124,14 +105,6 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f0,
         op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
       }
     }
-    if constexpr (Z::NS == Z::NP) __syncthreads();
-  }
-  if constexpr (CHECK) {
-    for (int st = 32; st >= 1; st >>= 1) {
-      const unsigned long long o = __shfl_xor(first, st);
-      first = o < first ? o : first;
-    }
-    if ((t & 63) == 0 && first != ~0ull) atomicMin(bad, first);
   }
 }
 
@@ -149,14 +122,13 @@ struct ZRows<Op, std::void_t<decltype(Op::kTileRows)>> {
   static constexpr int OY = Op::kTileRows;
 };
 
-template <class T, class Op, bool CHECK = false>
-void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f0, const T* f1,
-                   T* out, unsigned long long* bad = nullptr) {
+template <class T, class Op>
+void zmarch_launch(gc_context* ctx, const Dims& d, const Op& op, const T* f, T* out) {
   constexpr int OY = ZRows<Op>::OY;
   const int ntx = (d.nx + kZOX - 1) / kZOX, nty = (d.ny + OY - 1) / OY;
   const int nzc = (d.nz + kZLen - 1) / kZLen;
-  hipLaunchKernelGGL((k_zmarch<T, kZOX, OY, Op, CHECK>), dim3((unsigned)(ntx * nty * nzc)),
-                     dim3(256), 0, ctx->stream, op, f0, f1, out, d, ntx, nty, kZLen, bad);
+  hipLaunchKernelGGL((k_zmarch<T, kZOX, OY, Op>), dim3((unsigned)(ntx * nty * nzc)), dim3(256),
+                     0, ctx->stream, op, f, out, d, ntx, nty, kZLen);
 }
 
 }  // namespace
