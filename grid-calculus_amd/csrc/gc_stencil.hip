@@ -399,162 +399,233 @@ void axis_launch(gc_context* ctx, const Dims& d, const Op& op, const T* in, T* o
                      dim3(256), 0, ctx->stream, op, in, out, d, ntx, nty, kZLen);
 }
 
-// ---- biharmonic: ring + rotating register queues (order 4) --------------------------------
-//
-// The biharmonic reads 21 axis values and 75 mixed values per site.  On the ring engine every
-// one is an LDS read; here the values along z stay in per-thread registers (one column per
-// thread): the centre column over z-2..z+2 and the columns (x+-a, y), (x, y+-a) that the xz
-// and yz mixed terms use over z-2..z+2, in queues of QP = 5 registers, plus the centre
-// column's values at z-3 and z+3 in two more.  The plane loop is unrolled by QP: at step k
-// the queue entry for plane z+m-2 sits in register (k+m) mod QP and the ring slot of plane
-// z+j is (k+j) mod QP -- compile-time constants inside the unrolled body -- so advancing a
-// plane costs two register moves (z-3 and z+2 of the centre column) and every ring address
-// is an immediate offset.  The ring holds the five planes z-1..z+3 (read: z in-plane, z+2
-// and z+3 for the queues).  One barrier per plane: a wave one step ahead stores plane z+4
-// into the slot of plane z-1, which no wave reads after step k-1.  The queues start from
-// global memory at the chunk's first plane.  Same summation order as OpBih.
+// Calls f(integral_constant<int, P>) for each P of the sequence, in order.
 template <int... Ps, class F>
 __device__ __forceinline__ void unroll_seq(std::integer_sequence<int, Ps...>, F&& f) {
   (f(std::integral_constant<int, Ps>{}), ...);
 }
 
-// Three waves per SIMD (at most 168 VGPRs; 164 without spills): with two, the kernel waits
-// on LDS and barrier latency (1.74 ms against 1.58 ms at 512^3).
-template <class T, int O>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_bih_zq(
-    OpBih<T, O> op, const T* __restrict__ f,
-                                                T* __restrict__ out, Dims d, int ntx, int nty,
-                                                int zlen) {
-  static_assert(O == 4, "the rotating-queue kernel is laid out for the order-4 stencils");
+// ---- FP64 biharmonic: z-accumulating sums with shared products -----------------------------
+//
+// Biharmonic.hpp:69-109 sums each of dz4, dxz and dyz with the z offset in the outer position,
+// so a site's terms arrive plane by plane in the order the z-march reads the planes.  Here
+// every input plane is staged in LDS once and its contributions are added to the running sums
+// of every site it belongs to: dz4 of the 2 R4 + 1 sites z - R4 .. z + R4, dxz and dyz of the
+// 2 R2 + 1 sites z - R2 .. z + R2 (in the reference's inner order), while dx4, dy4 and dxy of
+// the site in the plane itself are formed in one go.  A value times a weight is the same
+// rounded product whichever site it serves, so each distinct (weight, value) product is formed
+// once per thread and plane (the unrolled code names each product once per value and weight;
+// the symmetric tables make many of them equal): for order 4, 63 multiplications per site
+// instead of 96, for the same 96 additions, same operands, same order, same bits.  Weights
+// that are powers of two (every order-2 mixed weight, the order-2 -4 and the order-4 2) take
+// fused multiply-adds, which round the exact product once, like the reference's separate
+// product and sum (acc_w).  No value queues: each thread owns NY rows of one column and
+// keeps, per row, the running sums of the sites in flight (dz4: 2 R4, dxz and dyz: 2 R2) and,
+// for the R4 sites between their own plane and their last dz4 term, A = dx4 ihx4 + dy4 ihy4
+// and X = dxy ihx2 ihy2 (turned into T = 2 (X + dxz ihx2 ihz2 + dyz ihy2 ihz2) when dxz and
+// dyz complete); the plane loop is unrolled by the number of site slots L so these live in
+// fixed registers.  The output of site z - R4 is (A + dz4 ihz4) + T, the reference's
+// expression order.  PB planes per barrier: the planes of step q are staged in buffer pair
+// q & 1, whose previous readers (step q - 2) have passed the barrier of step q - 1; the next
+// PB planes are requested right after the barrier.  A chunk of nout output planes reads
+// nout + 2 R4 planes; the R4 planes at each end feed only sums of sites outside the chunk,
+// which are formed but never stored (the in-plane sums are skipped there).  32-wide tiles keep
+// a thread's whole 7 x 7 window within the immediate-offset reach of one LDS address.
+
+// acc + fl(w v) for a compile-time weight w.  When |w| is a power of two, w v is exact (short of
+// overflow, |v| > DBL_MAX / |w|), so the fused multiply-add rounds once to the same value;
+// w = +-1 is a plain addition or subtraction.
+__host__ __device__ constexpr bool pow2_weight(double a) {
+  a = a < 0 ? -a : a;
+  if (a == 0.0) return false;
+  while (a >= 2.0) a *= 0.5;
+  while (a < 1.0) a *= 2.0;
+  return a == 1.0;
+}
+template <class T, class W>
+__device__ __forceinline__ T acc_w(T acc, W wc, T v) {
+  constexpr double w = W::value();
+  if constexpr (w == 1.0) return acc + v;
+  else if constexpr (w == -1.0) return acc - v;
+  else if constexpr (pow2_weight(w)) return __builtin_fma(T(w), v, acc);
+  else return acc + T(w) * v;
+}
+template <class S, int M>
+struct W1 {  // a one-axis weight
+  static constexpr double value() { return S::w(M); }
+};
+template <class S, int MA, int MB>
+struct W2 {  // a mixed weight w(ma) * w(mb), formed in double as the reference does
+  static constexpr double value() { return S::w(MA) * S::w(MB); }
+};
+
+template <class T, int O, int OX, int NY, int PB, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_bih_za(
+    OpBih<T, O> op, const T* __restrict__ f, T* __restrict__ out, Dims d, int ntx, int nty,
+    int zlen) {
   using S2 = D2<O>;
   using S4 = D4<O>;
-  constexpr int R = S4::R, R2 = S2::R;  // 3, 2
-  constexpr int QP = 2 * R2 + 1;        // 5: queue length, ring slots, unroll period
-  static_assert(R == R2 + 1 && QP >= R + 2, "centre column = queue plus one plane each side");
-  using Z = ZTile<R, 64, 4>;
-  static_assert(Z::OR == 1, "one column per thread");
-  __shared__ T ring[QP * Z::PN];
+  constexpr int R4 = S4::R, R2 = S2::R, N2 = S2::N, N4 = S4::N;
+  static_assert(R4 > R2, "the mixed sums complete before dz4");
+  static_assert(PB == 1 || PB == 2, "planes per barrier");
+  // site slots = unroll period: 2 R4 + 1 sites are in flight; a multiple of 2 PB, so the LDS
+  // buffer pair alternates at compile time
+  constexpr int L = ((2 * R4 + 1 + 2 * PB - 1) / (2 * PB)) * (2 * PB);
+  constexpr int NT = 256, OY = (NT / OX) * NY;
+  constexpr int PW = OX + 2 * R4, PH = OY + 2 * R4, PN = PW * PH;
+  constexpr int PL = (PN + NT - 1) / NT;
+  __shared__ T pb[2 * PB * PN];
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(bid % (unsigned)ntx);
   const unsigned rr = bid / (unsigned)ntx;
   const int ty = (int)(rr % (unsigned)nty);
   const int zc = (int)(rr / (unsigned)nty);
-  const int x0 = tx * 64, y0 = ty * 4, z0 = zc * zlen;
+  const int x0 = tx * OX, y0 = ty * OY, z0 = zc * zlen;
   const int nout = min(zlen, d.nz - z0);
+  const int np = nout + 2 * R4;
 
-  unsigned poff[Z::PL];
-  bool pin[Z::PL];
+  unsigned poff[PL];
+  bool pin[PL];
 #pragma unroll
-  for (int s = 0; s < Z::PL; ++s) {
-    const int idx = t + s * Z::NT;
-    pin[s] = idx < Z::PN;
-    const int px = idx % Z::PW, py = idx / Z::PW;
-    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R, d.nx) + d.nx * wrap_any(y0 + py - R, d.ny))
+  for (int s = 0; s < PL; ++s) {
+    const int idx = t + s * NT;
+    pin[s] = idx < PN;
+    const int px = idx % PW, py = idx / PW;
+    poff[s] = pin[s] ? (unsigned)(wrap_any(x0 + px - R4, d.nx) +
+                                  d.nx * wrap_any(y0 + py - R4, d.ny))
                      : 0u;
   }
-  auto zoff = [&](int plane) { return (size_t)d.plane * (size_t)wrap_any(plane, d.nz); };
-  T reg[Z::PL];
-  auto load = [&](int plane) {
-    const size_t zo = zoff(plane);
+  T reg[PB][PL];
+  auto load = [&](T* r, int plane) {
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
 #pragma unroll
-    for (int s = 0; s < Z::PL; ++s)
-      if (pin[s]) reg[s] = f[zo + poff[s]];
+    for (int s = 0; s < PL; ++s)
+      if (pin[s]) r[s] = f[zo + poff[s]];
   };
-  auto store = [&](int slot) {
+  const int ox = t % OX, oyt = (t / OX) * NY;
+  // the thread's window origin (x - R4, y - R4): every LDS offset below is non-negative, so
+  // the reads take immediate offsets from one base register
+  const int base = oyt * PW + ox;
+  // that origin in each of the 2 PB staged planes, kept in registers as they are: folded into
+  // every address, the buffer offsets push the immediates out of the ds_read2 range and each
+  // read pair would get an address addition of its own
+  int wofs[2 * PB];
 #pragma unroll
-    for (int s = 0; s < Z::PL; ++s)
-      if (pin[s]) ring[slot * Z::PN + t + s * Z::NT] = reg[s];
-  };
-  const int ox = t % 64, oy = t / 64;
-  const int base = (oy + R) * Z::PW + (ox + R);
-  const bool valid = x0 + ox < d.nx && y0 + oy < d.ny;
-  const T* rb = ring + base;
-  auto side = [](int a) { return a < 0 ? a + R2 : a + R2 - 1; };  // offset -> queue index
-  // this thread's column and its x and y neighbours (wrapped), for the queue start
-  const int gx = wrap_any(x0 + ox, d.nx), gy = wrap_any(y0 + oy, d.ny);
-  auto col = [&](int dx, int dy) {
-    return (size_t)wrap_any(gx + dx, d.nx) + (size_t)d.nx * (size_t)wrap_any(gy + dy, d.ny);
-  };
+  for (int i = 0; i < 2 * PB; ++i) {
+    wofs[i] = base + i * PN;
+    asm volatile("" : "+v"(wofs[i]));
+  }
+  bool vxy[NY];
+  size_t ocol[NY];
+#pragma unroll
+  for (int j = 0; j < NY; ++j) {
+    const int gy = y0 + oyt + j;
+    vxy[j] = x0 + ox < d.nx && gy < d.ny;
+    ocol[j] = (size_t)(x0 + ox) + (size_t)d.nx * (size_t)gy;
+  }
+  // per row and site slot (site plane - (z0 - R4)) mod L: running sums and finished parts
+  T dz[NY][L], xz[NY][L], yz[NY][L], A[NY][L], X[NY][L];
+#pragma unroll
+  for (int j = 0; j < NY; ++j)
+#pragma unroll
+    for (int s = 0; s < L; ++s) dz[j][s] = xz[j][s] = yz[j][s] = A[j][s] = X[j][s] = T(0);
 
-  T qc[QP];          // centre column, planes z-2..z+2
-  T zm, zp;          // centre column at z-3 and z+3
-  T qx[2 * R2][QP];  // columns x-R2..x+R2 (without x), planes z-2..z+2
-  T qy[2 * R2][QP];  // columns y-R2..y+R2 (without y)
-  // ring: planes z0-1 .. z0+2 in slots (j mod QP); queues as after a step k = -1
-  for (int j = -1; j <= R - 1; ++j) {
-    load(z0 + j);
-    store((j + QP) % QP);
-  }
-  load(z0 + R);
-  {
-    const size_t c0 = col(0, 0);
 #pragma unroll
-    for (int m = 0; m < QP; ++m) qc[(QP - 1 + m) % QP] = f[zoff(z0 - R + m) + c0];  // z0-3..z0+1
-    zp = f[zoff(z0 + R2) + c0];                                                  // z0+2
+  for (int i = 0; i < PB; ++i)
+    if (i < np) load(reg[i], z0 - R4 + i);
+  // plane z0 - R4 + p from staged plane `st` (P = p mod L)
+  auto plane = [&](auto Pc, int p, int st) {
+      constexpr int P = decltype(Pc)::value;
+      const int w0 = wofs[st];
+      auto V = [&](int dx, int dy) { return pb[w0 + (dy + R4) * PW + (dx + R4)]; };
+      // slot of the site o planes below this one
+      auto sl = [](int o) { return ((P - o) % L + L) % L; };
+      const bool inplane = p >= R4 && p - R4 < nout;
 #pragma unroll
-    for (int a = -R2; a <= R2; ++a) {
-      if (a == 0) continue;
-      const size_t cx = col(a, 0), cy = col(0, a);
-#pragma unroll
-      for (int m = 0; m < QP - 1; ++m) {  // planes z0-2..z0+1 at registers 0..3
-        qx[side(a)][m] = f[zoff(z0 - R2 + m) + cx];
-        qy[side(a)][m] = f[zoff(z0 - R2 + m) + cy];
+      for (int j = 0; j < NY; ++j) {
+        const T c0 = V(0, j);
+        using MS4 = std::make_integer_sequence<int, N4>;
+        using MS2 = std::make_integer_sequence<int, N2>;
+        // dz4 of the sites z - R4 .. z + R4 (this plane at offset o = m - R4 from the site)
+        unroll_seq(MS4{}, [&](auto Mc) {
+          constexpr int m = decltype(Mc)::value, o = m - R4;
+          T& acc = dz[j][sl(o)];
+          acc = acc_w(o == -R4 ? T(0) : acc, W1<S4, m>{}, c0);
+        });
+        // dxz and dyz of the sites z - R2 .. z + R2 (z offset outer, x or y offset inner)
+        unroll_seq(MS2{}, [&](auto Bc) {
+          constexpr int mb = decltype(Bc)::value, o = mb - R2;
+          T& ax = xz[j][sl(o)];
+          T& ay = yz[j][sl(o)];
+          unroll_seq(MS2{}, [&](auto Ac) {
+            constexpr int ma = decltype(Ac)::value;
+            const bool first = o == -R2 && ma == 0;
+            ax = acc_w(first ? T(0) : ax, W2<S2, ma, mb>{}, V(ma - R2, j));
+          });
+          unroll_seq(MS2{}, [&](auto Ac) {
+            constexpr int ma = decltype(Ac)::value;
+            const bool first = o == -R2 && ma == 0;
+            ay = acc_w(first ? T(0) : ay, W2<S2, ma, mb>{}, V(0, j + ma - R2));
+          });
+        });
+        // the mixed sums of the site R2 planes below are complete
+        {
+          constexpr int s = (((P - R2) % L) + L) % L;
+          X[j][s] = T(2) * ((X[j][s] + xz[j][s] * op.ihx2 * op.ihz2) +
+                            yz[j][s] * op.ihy2 * op.ihz2);
+        }
+        if (inplane) {
+          T dx4 = 0, dy4 = 0;
+          unroll_seq(MS4{}, [&](auto Mc) {
+            constexpr int m = decltype(Mc)::value;
+            dx4 = acc_w(dx4, W1<S4, m>{}, V(m - R4, j));
+            dy4 = acc_w(dy4, W1<S4, m>{}, V(0, j + m - R4));
+          });
+          T dxy = 0;
+          unroll_seq(MS2{}, [&](auto Bc) {
+            constexpr int mb = decltype(Bc)::value;
+            unroll_seq(MS2{}, [&](auto Ac) {
+              constexpr int ma = decltype(Ac)::value;
+              dxy = acc_w(dxy, W2<S2, ma, mb>{}, V(ma - R2, j + mb - R2));
+            });
+          });
+          A[j][P] = dx4 * op.ihx4 + dy4 * op.ihy4;
+          X[j][P] = dxy * op.ihx2 * op.ihy2;
+        }
       }
-    }
-  }
-  size_t oidx = (size_t)d.plane * (size_t)z0 + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy);
-  for (int kb = 0; kb < nout; kb += QP) {
-    unroll_seq(std::make_integer_sequence<int, QP>{}, [&](auto Pc) {
-      constexpr int P = decltype(Pc)::value;  // step k = kb + P, k mod QP = P
-      const int k = kb + P;
-      if (k >= nout) return;
-      auto slot = [](int j) { return (P + j + QP) % QP; };  // ring slot of plane z+j
-      constexpr int qn = (P + QP - 1) % QP;                  // queue register of plane z+2
-      store(slot(R));
+      // the site R4 planes below is complete
+      const int so = p - 2 * R4;
+      if (so >= 0 && so < nout) {
+        constexpr int s = (((P - R4) % L) + L) % L;
+        const size_t zo = (size_t)d.plane * (size_t)(z0 + so);
+#pragma unroll
+        for (int j = 0; j < NY; ++j)
+          if (vxy[j]) out[zo + ocol[j]] = (A[j][s] + dz[j][s] * op.ihz4) + X[j][s];
+      }
+  };
+  for (int kb = 0; kb < np; kb += L) {
+    // PB planes per barrier: stage them, request the next PB, then form each
+    unroll_seq(std::make_integer_sequence<int, L / PB>{}, [&](auto Qc) {
+      constexpr int Q = decltype(Qc)::value;
+      const int p0 = kb + Q * PB;
+      if (p0 >= np) return;
+      T* b = pb + (Q & 1) * PB * PN;
+#pragma unroll
+      for (int i = 0; i < PB; ++i)
+        if (p0 + i < np) {
+#pragma unroll
+          for (int s = 0; s < PL; ++s)
+            if (pin[s]) b[i * PN + t + s * NT] = reg[i][s];
+        }
       __syncthreads();
-      if (k + 1 < nout) load(z0 + k + 1 + R);
-      zm = qc[qn];  // plane z-3 (the entry leaving the queue)
-      qc[qn] = zp;  // plane z+2 (z+3 of the previous step)
-      zp = rb[slot(R) * Z::PN];
 #pragma unroll
-      for (int a = -R2; a <= R2; ++a) {
-        if (a == 0) continue;
-        qx[side(a)][qn] = rb[slot(R2) * Z::PN + a];
-        qy[side(a)][qn] = rb[slot(R2) * Z::PN + a * Z::PW];
-      }
-      auto F = [&](int dx, int dy) { return rb[slot(0) * Z::PN + dy * Z::PW + dx]; };
-      auto q = [](int o) { return (P + R2 + o) % QP; };  // register of plane z+o, |o| <= 2
-      auto zc_at = [&](int o) { return o == -R ? zm : (o == R ? zp : qc[q(o)]); };
-      if (valid) {
-        T dx4 = 0, dy4 = 0, dz4 = 0;
-#pragma unroll
-        for (int m = 0; m < S4::N; ++m) {
-          const int o = m - S4::R;
-          const T w = T(S4::w(m));
-          dx4 += w * F(o, 0);
-          dy4 += w * F(0, o);
-          dz4 += w * zc_at(o);
-        }
-        T dxy = 0, dxz = 0, dyz = 0;
-#pragma unroll
-        for (int mb = 0; mb < S2::N; ++mb) {
-          const int ob = mb - S2::R;
-          const double wb = S2::w(mb);
-#pragma unroll
-          for (int ma = 0; ma < S2::N; ++ma) {
-            const int oa = ma - S2::R;
-            const T wab = T(S2::w(ma) * wb);
-            dxy += wab * F(oa, ob);
-            dxz += wab * (oa == 0 ? qc[q(ob)] : qx[side(oa)][q(ob)]);
-            dyz += wab * (oa == 0 ? qc[q(ob)] : qy[side(oa)][q(ob)]);
-          }
-        }
-        out[oidx] = dx4 * op.ihx4 + dy4 * op.ihy4 + dz4 * op.ihz4 +
-                    T(2) * (dxy * op.ihx2 * op.ihy2 + dxz * op.ihx2 * op.ihz2 +
-                            dyz * op.ihy2 * op.ihz2);
-      }
-      oidx += (size_t)d.plane;
+      for (int i = 0; i < PB; ++i)
+        if (p0 + PB + i < np) load(reg[i], z0 - R4 + p0 + PB + i);
+      unroll_seq(std::make_integer_sequence<int, PB>{}, [&](auto Ic) {
+        constexpr int i = decltype(Ic)::value;
+        if (p0 + i < np) plane(std::integral_constant<int, Q * PB + i>{}, p0 + i, (Q & 1) * PB + i);
+      });
     });
   }
 }
@@ -1243,15 +1314,19 @@ void bih_launch(gc_context* ctx, const gc_grid* g, const void* in, void* out) {
     const int ntx = (d.nx + 63) / 64, nty = (d.ny + 7) / 8, nzc = (d.nz + kZLen - 1) / kZLen;
     hipLaunchKernelGGL((k_bih_sep<O, 8>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
                        ctx->stream, c, (const float*)in, (float*)out, d, ntx, nty, kZLen);
-  } else if constexpr (O == 4) {
-    // the register-queue kernel pays where the plane ring is largest (9 planes of 8-byte
-    // values): 1.42 ms against 1.64 ms on 512^3; for the other three cases the plain z-march
-    // is as fast or faster
-    const int ntx = (d.nx + 63) / 64, nty = (d.ny + 3) / 4, nzc = (d.nz + kZLen - 1) / kZLen;
-    hipLaunchKernelGGL((k_bih_zq<T, O>), dim3((unsigned)(ntx * nty * nzc)), dim3(256), 0,
-                       ctx->stream, op, (const T*)in, (T*)out, d, ntx, nty, kZLen);
   } else {
-    zmarch_launch<T>(ctx, d, op, (const T*)in, (T*)out);
+    // FP64 (both orders): the z-accumulating kernel on 32 x 8 tiles, one row per thread; order
+    // 4 stages two planes per barrier (1.19 ms on 512^3 against 1.22 with one; the register-
+    // queue kernel it replaced took 1.59 ms), order 2 one (0.67 ms; the plane ring 0.80 ms)
+    constexpr int PB = O == 4 ? 2 : 1, WPE = O == 4 ? 3 : 1;
+    const int ntx = (d.nx + 31) / 32, nty = (d.ny + 7) / 8;
+    // 64-plane chunks (each chunk re-reads 2 R4 planes), shorter where the grid would not fill
+    // the GPU with workgroups
+    int zl = 64;
+    while (zl > 8 && (long)ntx * nty * ((d.nz + zl - 1) / zl) < 2048) zl /= 2;
+    const int nzc = (d.nz + zl - 1) / zl;
+    hipLaunchKernelGGL((k_bih_za<T, O, 32, 1, PB, WPE>), dim3((unsigned)(ntx * nty * nzc)),
+                       dim3(256), 0, ctx->stream, op, (const T*)in, (T*)out, d, ntx, nty, zl);
   }
 }
 

@@ -81,6 +81,18 @@ def test_operator_f32_close(gc, orc, ctx, op, order):
     assert rel < 2e-5, rel
 
 
+@pytest.mark.parametrize("order", [2, 4])
+def test_biharmonic_f64_bit_exact_long_chunks(gc, orc, ctx, order):
+    # large enough that the FP64 biharmonic marches 64-plane chunks (the small shapes above
+    # run 8- to 32-plane chunks); ragged x and y tiles, a ragged last chunk of 8 planes
+    shape = (250, 232, 520, (1.0 / 250, 1.0 / 232, 1.0 / 520))
+    g, og = _grids(gc, orc, shape)
+    f = _field(gc, shape[:3], seed=77 + order)
+    got = gc.biharmonic(ctx, g, gc.DeviceArray.from_numpy(ctx, f), order=order).numpy(ctx)
+    want = orc.biharmonic(og, f, order=order)
+    assert np.array_equal(got, want), f"max |diff| {np.abs(got - want).max()}"
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}")
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_cell_laplacian(gc, orc, ctx, shape, dtype):
