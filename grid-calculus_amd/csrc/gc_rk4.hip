@@ -929,7 +929,9 @@ gc_status launch_stage(gc_context* ctx, const gc_grid* g, const FvmRhs<T>& rhs, 
 //           out = psi + (dt/6 (k1 + k4) + dt/3 (k2 + k3))
 // Every value is formed with the single-stage kernels' arithmetic, so the results are the
 // same bits.  PAIR 1 must not write psi in place: neighbouring tiles and z-chunks still read
-// it as their halo.
+// it as their halo.  The FP32 register-queue kernels (k_ch_pairq3d) drop y3: PAIR 0 writes
+// k1 and k2 (3 W), PAIR 1 reads psi and k2 with the halo and forms y3 = psi + dt/2 k2 as it
+// loads them (4 W) -- 7 W per step.
 //
 // Iteration p of the z-march stores stencil-input plane z0-4+p, forms mu_A on plane z0-5+p
 // (tile+3), k_A and y' on plane z0-6+p (tile+2), mu_B on plane z0-7+p (tile+1) and the
@@ -1234,30 +1236,52 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   // compile-time constant and the queues rotate by renaming instead of register moves.
   T yq[3][L] = {}, mq[3][L] = {}, wq[3][L] = {}, bq[3][L] = {};
   T kq[3][PAIR == 1 ? L : 1] = {};
-  T yn[L] = {};  // the next plane of y, in flight
+  // FP32: S2 does not store y3 = psi + dt/2 k2; S3+S4 forms it from psi and k2 as it loads
+  // them (stage_axpy, the same bits), 7 words per site and step instead of 9.  FP64 keeps the
+  // stored y3: its S3+S4 kernel has no registers for the second load stream (1024-thread
+  // blocks cap it at 128 VGPRs) and lost what S1+S2 gained (106.5 + 181.5 us against
+  // 128.5 + 161.5 us at 256^3).
+  constexpr bool kY3 = sizeof(T) == 4;
+  T yn[L] = {};  // the next plane of y (kY3, PAIR 1: of psi), in flight
+  T kn[PAIR == 1 && kY3 ? L : 1] = {};  // kY3, PAIR 1: the next plane of k2, in flight
   T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
-  T pv[L] = {}, av[L] = {}, bv[L] = {};
-  // Phase Y's point operands are requested one iteration ahead (after that iteration's y
-  // plane, so waiting for them never waits for the newer y plane): they have a whole
-  // iteration to arrive instead of phase X only.  The S3+S4 kernels of FP64 and of tiles
-  // with more than three columns per thread have no registers to spare for them (they
-  // would spill) and request them in the iteration that uses them.
-  constexpr bool kAhead = PAIR == 0 || (sizeof(T) == 4 && L <= 3);
+  // the output site's point operands: PAIR 1 psi, k1, k2; PAIR 0 psi (for y3) unless kY3
+  constexpr bool kPV = PAIR == 1 || !kY3;
+  T pv[kPV ? L : 1] = {}, av[PAIR == 1 ? L : 1] = {}, bv[PAIR == 1 ? L : 1] = {};
+  // They are requested one iteration ahead (after that iteration's y plane, so waiting for
+  // them never waits for the newer y plane): they have a whole iteration to arrive instead of
+  // phase X only.  The S3+S4 kernels of FP64 and of tiles with more than three columns per
+  // thread have no registers to spare for them (they would spill) and request them in the
+  // iteration that uses them.
+  constexpr bool kAhead = kPV && (PAIR == 0 || (sizeof(T) == 4 && L <= 3));
   T pen[PAIR == 1 && kAhead ? L : 1] = {};
   T pvn[kAhead ? L : 1] = {}, avn[kAhead && PAIR == 1 ? L : 1] = {},
     bvn[kAhead && PAIR == 1 ? L : 1] = {};
 
   auto load_y = [&](int plane) {
-    const T* yp = y + (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+    const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
+    if constexpr (PAIR == 1 && kY3) {
 #pragma unroll
-    for (int s = 0; s < L; ++s)
-      if (dist[s] >= 0) yn[s] = yp[goff[s]];
+      for (int s = 0; s < L; ++s)
+        if (dist[s] >= 0) {
+          yn[s] = psi[zo + goff[s]];
+          kn[s] = b[zo + goff[s]];
+        }
+    } else {
+      const T* yp = y + zo;
+#pragma unroll
+      for (int s = 0; s < L; ++s)
+        if (dist[s] >= 0) yn[s] = yp[goff[s]];
+    }
   };
   const int last = nout + 8;
   auto iter = [&](auto rc, int p) {
     constexpr int R0 = decltype(rc)::value, R1 = (R0 + 2) % 3, R2 = (R0 + 1) % 3;
 #pragma unroll
-    for (int s = 0; s < L; ++s) yq[R0][s] = yn[s];
+    for (int s = 0; s < L; ++s) {
+      if constexpr (PAIR == 1 && kY3) yq[R0][s] = stage_axpy(yn[s], co.half_dt, kn[s]);
+      else yq[R0][s] = yn[s];
+    }
     if (p < last) load_y(z0 - 3 + p);
     if constexpr (PAIR == 1 && kAhead) {
       if (p >= 4) {
@@ -1302,7 +1326,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           }
         }
       }
-    } else if (p >= 9) {
+    } else if (PAIR == 1 && p >= 9) {
       const size_t kb = (size_t)d.plane * (size_t)(z0 - 9 + p);
 #pragma unroll
       for (int s = 0; s < L; ++s) {
@@ -1341,7 +1365,8 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           const T kv = ch_k(cf, GC_NB_Q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s]));
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + goff[s]);
-            __builtin_nontemporal_store(stage_axpy(pv[s], co.half_dt, kv), out + kb + goff[s]);
+            if constexpr (!kY3)
+              __builtin_nontemporal_store(stage_axpy(pv[s], co.half_dt, kv), out + kb + goff[s]);
           } else {
             // k3 of P_{p-5}: phase Y of iteration p-3, slot R0 (read before it is refilled
             // below)
