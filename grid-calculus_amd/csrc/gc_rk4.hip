@@ -1216,7 +1216,11 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
                                                    T* __restrict__ b, Dims d, int ntx, int nty,
                                                    StageCoef<T> co) {
   using C = PairCfg<OX, OY, NT>;
-  constexpr int W = C::W4, H = C::H4, N = C::N4, L = C::L4;
+  constexpr int W = C::W4, H = C::H4, N = C::N4;
+  // A thread's columns: slots 0 .. OR-1 are its tile sites (the outputs), the rest halo sites
+  // (rows 0-3, rows H-4 .. H-1, then the four columns each side of the tile rows), so the
+  // output-only queues and operands need OR slots instead of L
+  constexpr int OR = C::OR, NH = N - OX * OY, L = OR + (NH + NT - 1) / NT;
   __shared__ T ys[N], ms[N], ws[N], bs[N];  // y, mu_A, y', mu_B: one plane each
   const int t = threadIdx.x;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -1229,18 +1233,40 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
 
   unsigned goff[L];  // the column's site in a plane (wrapped)
   int dist[L];       // distance from the region edge; -1: no column
+  int ro[L];         // the column's index in the region (LDS planes)
   bool oin[L];       // the column is a tile site inside the grid (an output)
 #pragma unroll
   for (int s = 0; s < L; ++s) {
-    const int idx = t + s * NT;
-    const int px = idx % W, py = idx / W;
-    if (idx < N) {
+    int px = 0, py = 0;
+    bool have = true;
+    if (s < OR) {
+      const int i = t + s * NT;
+      px = 4 + i % OX;
+      py = 4 + i / OX;
+    } else {
+      const int h = t + (s - OR) * NT;
+      have = h < NH;
+      if (h < 4 * W) {
+        px = h % W;
+        py = h / W;
+      } else if (h < 8 * W) {
+        px = (h - 4 * W) % W;
+        py = H - 4 + (h - 4 * W) / W;
+      } else {
+        const int c = (h - 8 * W) % 8;
+        px = c < 4 ? c : W - 8 + c;
+        py = 4 + (h - 8 * W) / 8;
+      }
+    }
+    if (have) {
       goff[s] = (unsigned)(wrap_any(x0 + px - 4, d.nx) + d.nx * wrap_any(y0 + py - 4, d.ny));
       dist[s] = min(min(px, py), min(W - 1 - px, H - 1 - py));
-      oin[s] = dist[s] >= 4 && x0 + px - 4 < d.nx && y0 + py - 4 < d.ny;
+      ro[s] = py * W + px;
+      oin[s] = s < OR && x0 + px - 4 < d.nx && y0 + py - 4 < d.ny;
     } else {
       goff[s] = 0u;
       dist[s] = -1;
+      ro[s] = 0;
       oin[s] = false;
     }
   }
@@ -1249,19 +1275,19 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   // p-1's and (R + 1) mod 3 iteration p-2's.  The loop is unrolled by three so that R is a
   // compile-time constant and the queues rotate by renaming instead of register moves.
   T yq[3][L] = {}, mq[3][L] = {}, wq[3][L] = {}, bq[3][L] = {};
-  T kq[3][PAIR == 1 ? L : 1] = {};
+  T kq[3][PAIR == 1 ? OR : 1] = {};  // PAIR 1: k3 of the tile sites
   // FP32: S2 does not store y3 = psi + dt/2 k2; S3+S4 forms it from psi and k2 as it loads
   // them (stage_axpy, the same bits), 7 words per site and step instead of 9.  FP64 keeps the
   // stored y3: its S3+S4 kernel has no registers for the second load stream (1024-thread
-  // blocks cap it at 128 VGPRs) and lost what S1+S2 gained (106.5 + 181.5 us against
-  // 128.5 + 161.5 us at 256^3).
+  // blocks cap it at 128 VGPRs; with it, 0.2943 ms per step at 256^3 against 0.2830 ms
+  // without, on one box).
   constexpr bool kY3 = sizeof(T) == 4;
   T yn[L] = {};  // the next plane of y (kY3, PAIR 1: of psi), in flight
   T kn[PAIR == 1 && kY3 ? L : 1] = {};  // kY3, PAIR 1: the next plane of k2, in flight
   T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
   // the output site's point operands: PAIR 1 psi, k1, k2; PAIR 0 psi (for y3) unless kY3
   constexpr bool kPV = PAIR == 1 || !kY3;
-  T pv[kPV ? L : 1] = {}, av[PAIR == 1 ? L : 1] = {}, bv[PAIR == 1 ? L : 1] = {};
+  T pv[kPV ? OR : 1] = {}, av[PAIR == 1 ? OR : 1] = {}, bv[PAIR == 1 ? OR : 1] = {};
   // They are requested one iteration ahead (after that iteration's y plane, so waiting for
   // them never waits for the newer y plane): they have a whole iteration to arrive instead of
   // phase X only.  The S3+S4 kernels of FP64 and of tiles with more than three columns per
@@ -1269,8 +1295,8 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   // iteration that uses them.
   constexpr bool kAhead = kPV && (PAIR == 0 || (sizeof(T) == 4 && L <= 3));
   T pen[PAIR == 1 && kAhead ? L : 1] = {};
-  T pvn[kAhead ? L : 1] = {}, avn[kAhead && PAIR == 1 ? L : 1] = {},
-    bvn[kAhead && PAIR == 1 ? L : 1] = {};
+  T pvn[kAhead ? OR : 1] = {}, avn[kAhead && PAIR == 1 ? OR : 1] = {},
+    bvn[kAhead && PAIR == 1 ? OR : 1] = {};
 
   auto load_y = [&](int plane) {
     const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
@@ -1319,7 +1345,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
     if constexpr (kAhead) {
       if (p >= 9) {
 #pragma unroll
-        for (int s = 0; s < L; ++s) {
+        for (int s = 0; s < OR; ++s) {
           pv[s] = pvn[s];
           if constexpr (PAIR == 1) {
             av[s] = avn[s];
@@ -1330,7 +1356,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
       if (p >= 8 && p < last) {  // the outputs of iteration p+1: plane z0-8+p
         const size_t kb = (size_t)d.plane * (size_t)(z0 - 8 + p);
 #pragma unroll
-        for (int s = 0; s < L; ++s) {
+        for (int s = 0; s < OR; ++s) {
           if (oin[s]) {
             pvn[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
             if constexpr (PAIR == 1) {
@@ -1343,7 +1369,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
     } else if (PAIR == 1 && p >= 9) {
       const size_t kb = (size_t)d.plane * (size_t)(z0 - 9 + p);
 #pragma unroll
-      for (int s = 0; s < L; ++s) {
+      for (int s = 0; s < OR; ++s) {
         if (oin[s]) {
           pv[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
           av[s] = __builtin_nontemporal_load(a + kb + goff[s]);
@@ -1355,7 +1381,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
     // phase X of iteration p-1)
 #pragma unroll
     for (int s = 0; s < L; ++s) {
-      const int o = t + s * NT;
+      const int o = ro[s];
       if (p >= 3 && dist[s] >= 1) ms[o] = mq[R1][s];
       if (p >= 8 && dist[s] >= 3) bs[o] = bq[R1][s];
       if (p >= 2 && dist[s] >= 1)
@@ -1372,10 +1398,10 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
       const bool own = zA >= z0 && zA < z0 + nout;
 #pragma unroll
       for (int s = 0; s < L; ++s) {
-        const int o = t + s * NT;
+        const int o = ro[s];
         if (dist[s] >= 0) ys[o] = yq[R0][s];
         if (p >= 5 && dist[s] >= 2) ws[o] = wq[R1][s];
-        if (p >= 9 && oin[s]) {  // mu_B on P_{p-4}, P_{p-5}, P_{p-6}: slots R0, R1, R2
+        if (s < OR && p >= 9 && oin[s]) {  // mu_B on P_{p-4}, P_{p-5}, P_{p-6}: slots R0, R1, R2
           const T kv = ch_k(cf, GC_NB_Q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s]));
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + goff[s]);
@@ -1397,7 +1423,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
               __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + goff[s]);
           } else {
             wq[R0][s] = stage_axpy(pe[s], co.dt, kv);
-            kq[R0][s] = kv;
+            if (s < OR) kq[R0][s] = kv;
           }
         }
       }
