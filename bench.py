@@ -160,8 +160,12 @@ def ch3d(a):
             elif not a.no_tune:
                 gc.ch_tune(ctx, g, dtype)
             cfg = gc.ch_get_config(g, dtype)
-            tiles[name] = gc.ch_config_shape(g, cfg)
+            tile = list(gc.ch_config_shape(g, cfg))
             kinds[name] = gc.ch_config_kind(g, cfg)
+            if name == "f64" and kinds[name] in (1, 2):
+                # the shape is the FP32 one; wider two-stage tiles run 64 x 16 in FP64
+                tile = [min(tile[0], 64), min(tile[1], 16), tile[2]]
+            tiles[name] = tile
         extra["tile_ox_oy_zc"] = tiles
         extra["two_stage_kernels"] = {k: ["none", "plane rings", "register queues"][v]
                                       for k, v in kinds.items()}
