@@ -1285,6 +1285,11 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   T yn[L] = {};  // the next plane of y (kY3, PAIR 1: of psi), in flight
   T kn[PAIR == 1 && kY3 ? L : 1] = {};  // kY3, PAIR 1: the next plane of k2, in flight
   T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
+  // FP64 PAIR 1: pe of the tile slots over the last three iterations; iteration p's output
+  // plane P_{p-5} is iteration p-3's P_{p-2}, so its psi operand comes from here, not from
+  // HBM (the FP32 tiles hold two output sites per thread, and the queue would spill)
+  constexpr bool kPQ = PAIR == 1 && sizeof(T) == 8;
+  T pq[3][kPQ ? OR : 1] = {};
   // the output site's point operands: PAIR 1 psi, k1, k2; PAIR 0 psi (for y3) unless kY3
   constexpr bool kPV = PAIR == 1 || !kY3;
   T pv[kPV ? OR : 1] = {}, av[PAIR == 1 ? OR : 1] = {}, bv[PAIR == 1 ? OR : 1] = {};
@@ -1346,7 +1351,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
       if (p >= 9) {
 #pragma unroll
         for (int s = 0; s < OR; ++s) {
-          pv[s] = pvn[s];
+          if constexpr (!kPQ) pv[s] = pvn[s];
           if constexpr (PAIR == 1) {
             av[s] = avn[s];
             bv[s] = bvn[s];
@@ -1358,7 +1363,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
 #pragma unroll
         for (int s = 0; s < OR; ++s) {
           if (oin[s]) {
-            pvn[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
+            if constexpr (!kPQ) pvn[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
             if constexpr (PAIR == 1) {
               avn[s] = __builtin_nontemporal_load(a + kb + goff[s]);
               bvn[s] = __builtin_nontemporal_load(b + kb + goff[s]);
@@ -1371,7 +1376,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
 #pragma unroll
       for (int s = 0; s < OR; ++s) {
         if (oin[s]) {
-          pv[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
+          if constexpr (!kPQ) pv[s] = __builtin_nontemporal_load(psi + kb + goff[s]);
           av[s] = __builtin_nontemporal_load(a + kb + goff[s]);
           bv[s] = __builtin_nontemporal_load(b + kb + goff[s]);
         }
@@ -1411,8 +1416,9 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
             // k3 of P_{p-5}: phase Y of iteration p-3, slot R0 (read before it is refilled
             // below)
             const T bsum = bv[s] + kq[R0][s];
-            __builtin_nontemporal_store(rk4_final(pv[s], av[s], bsum, kv, co.sixth_dt, co.third_dt),
-                                        out + kb + goff[s]);
+            __builtin_nontemporal_store(
+                rk4_final(kPQ ? pq[R0][s] : pv[s], av[s], bsum, kv, co.sixth_dt, co.third_dt),
+                out + kb + goff[s]);
           }
         }
         if (p >= 4 && dist[s] >= 2) {
@@ -1426,6 +1432,10 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
             if (s < OR) kq[R0][s] = kv;
           }
         }
+      }
+      if constexpr (kPQ) {  // this iteration's P_{p-2}, the output plane 3 iterations on
+#pragma unroll
+        for (int s = 0; s < OR; ++s) pq[R0][s] = pe[s];
       }
     }
     __syncthreads();
