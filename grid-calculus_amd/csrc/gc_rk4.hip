@@ -1285,10 +1285,12 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   T yn[L] = {};  // the next plane of y (kY3, PAIR 1: of psi), in flight
   T kn[PAIR == 1 && kY3 ? L : 1] = {};  // kY3, PAIR 1: the next plane of k2, in flight
   T pe[PAIR == 1 ? L : 1] = {};  // PAIR 1: psi on P_{p-2} (base of y')
-  // FP64 PAIR 1: pe of the tile slots over the last three iterations; iteration p's output
-  // plane P_{p-5} is iteration p-3's P_{p-2}, so its psi operand comes from here, not from
-  // HBM (the FP32 tiles hold two output sites per thread, and the queue would spill)
-  constexpr bool kPQ = PAIR == 1 && sizeof(T) == 8;
+  // FP64: the output's psi operand (plane P_{p-5}) from registers instead of HBM.  PAIR 1:
+  // pe of the tile slots over the last three iterations (iteration p's output plane is
+  // iteration p-3's P_{p-2}).  PAIR 0: the tile slots' y (= psi) planes P_{p-3} .. P_{p-5},
+  // taken from the y queue as they leave it.  (The FP32 tiles hold two output sites per
+  // thread, and the queue would spill.)
+  constexpr bool kPQ = sizeof(T) == 8 && (PAIR == 1 || !kY3);
   T pq[3][kPQ ? OR : 1] = {};
   // the output site's point operands: PAIR 1 psi, k1, k2; PAIR 0 psi (for y3) unless kY3
   constexpr bool kPV = PAIR == 1 || !kY3;
@@ -1322,6 +1324,10 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
   const int last = nout + 8;
   auto iter = [&](auto rc, int p) {
     constexpr int R0 = decltype(rc)::value, R1 = (R0 + 2) % 3, R2 = (R0 + 1) % 3;
+    if constexpr (kPQ && PAIR == 0) {  // P_{p-3}, leaving the y queue
+#pragma unroll
+      for (int s = 0; s < OR; ++s) pq[R0][s] = yq[R0][s];
+    }
 #pragma unroll
     for (int s = 0; s < L; ++s) {
       if constexpr (PAIR == 1 && kY3) yq[R0][s] = stage_axpy(yn[s], co.half_dt, kn[s]);
@@ -1410,8 +1416,9 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           const T kv = ch_k(cf, GC_NB_Q(bs, o, W, bq[R1][s], bq[R2][s], bq[R0][s]));
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + goff[s]);
-            if constexpr (!kY3)
-              __builtin_nontemporal_store(stage_axpy(pv[s], co.half_dt, kv), out + kb + goff[s]);
+            if constexpr (!kY3)  // psi on P_{p-5}: saved in iteration p-2, slot R2
+              __builtin_nontemporal_store(stage_axpy(kPQ ? pq[R2][s] : pv[s], co.half_dt, kv),
+                                          out + kb + goff[s]);
           } else {
             // k3 of P_{p-5}: phase Y of iteration p-3, slot R0 (read before it is refilled
             // below)
@@ -1433,7 +1440,7 @@ __global__ __launch_bounds__(NT) void k_ch_pairq3d(CHCoef<T> cf, const T* __rest
           }
         }
       }
-      if constexpr (kPQ) {  // this iteration's P_{p-2}, the output plane 3 iterations on
+      if constexpr (kPQ && PAIR == 1) {  // this iteration's P_{p-2}, the output plane 3 on
 #pragma unroll
         for (int s = 0; s < OR; ++s) pq[R0][s] = pe[s];
       }
