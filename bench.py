@@ -202,6 +202,7 @@ def ch3d(a):
         for d in arrays.values():
             d.free()
     H = legs["f64"]
+    traffic64 = profiled_traffic(PROFILE_CH3D, "traffic_bytes_per_step_f64")
     extra["diagnostics_fp64"] = [{"step": s, "mass": m, "F": f} for s, m, f in H["diags"]]
 
     if not a.no_learned:
@@ -242,11 +243,15 @@ def ch3d(a):
                    "grid": [n, n, n], "dt": dt, "kappa": kappa, "M": 1.0},
         "roofline": {"bound": "hbm", "achieved": H["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": H["achieved"] / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(PROFILE_CH3D, "traffic_bytes_per_step_f64"),
+                     "traffic": traffic64,
                      "kernel": _kernel_name(kinds["f64"], "double", tiles["f64"]),
                      "step_kernels_ms": H["step_kernels_ms"],
                      "kernel_timing_steps": KERNEL_TIMING_STEPS,
-                     "alg_bytes_per_step": H["bytes_step"]},
+                     "alg_bytes_per_step": H["bytes_step"],
+                     # the kernels move 9 of the 16 algorithmic words per site and step: the
+                     # HBM rate they run at, from the profiled traffic and this run's kernel time
+                     "traffic_GBs": (traffic64 / (H["step_kernels_ms"] / 1e3) / 1e9
+                                     if traffic64 else None)},
         "cpu_baseline": cpu,
         "extra": extra,
     }
