@@ -633,15 +633,70 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
   const int x0 = tx * OX, y0 = ty * OY, z0 = zc * ZC;
   const int nout = min(ZC, d.nz - z0);
 
-  unsigned yoff[C::L2];
-#pragma unroll
-  for (int s = 0; s < C::L2; ++s) {
-    const int idx = t + s * NT;
-    const int px = idx % C::W2, py = idx / C::W2;
-    yoff[s] = idx < C::N2 ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
-                          : 0u;
-  }
   const int ox = t % OX, oy0 = t / OX;
+  // Constant D (kQ): a thread's region sites are slots 0 .. OR-1 its tile sites (the output
+  // sites below), then halo sites -- the tile+1 ring in ring1_site order (so ring site t + s NT
+  // is this thread's slot OR + s, as below), then the outer ring; PAIR 1 then finds the psi
+  // and k2 of its own sites from earlier planes in registers.  Heterogeneous D: region site
+  // t + s NT in row-major order, and PAIR 1 reloads those values (from L2): the queues would
+  // take the FP32 kernels past 64 VGPRs, and with them from two workgroups per CU to one
+  // (256^3, one box: FP32 0.190 -> 0.241 ms per step, FP64 0.381 -> 0.396 ms).
+  constexpr bool kQ = !FVM;
+  constexpr int NH2 = C::N2 - OX * OY, NRO = NH2 - C::NR;  // halo sites, outer-ring sites
+  constexpr int LS = kQ ? C::OR + (NH2 + NT - 1) / NT : C::L2;  // slots
+  unsigned yoff[LS];
+  int ro[kQ ? LS : 1];     // kQ: region index (LDS planes)
+  bool have[kQ ? LS : 1];  // kQ: the slot holds a site
+  // region index of slot s and whether the slot holds a site
+  auto rix = [&](int s) -> int {
+    if constexpr (kQ) return ro[s];
+    else return t + s * NT;
+  };
+  auto has = [&](int s) -> bool {
+    if constexpr (kQ) return have[s];
+    else return t + s * NT < C::N2;
+  };
+#pragma unroll
+  for (int s = 0; s < LS; ++s) {
+    int px = 0, py = 0;
+    bool hs = true;
+    if (!kQ) {
+      const int idx = t + s * NT;
+      hs = idx < C::N2;
+      px = idx % C::W2;
+      py = idx / C::W2;
+    } else if (s < C::OR) {
+      px = ox + 2;
+      py = oy0 + s * C::ROWS + 2;
+    } else {
+      const int h = t + (s - C::OR) * NT;
+      hs = h < NH2;
+      if (h < C::NR) {
+        int mx = 0, my = 0;
+        ring1_site<C, OY>(h, mx, my);
+        px = mx + 1;
+        py = my + 1;
+      } else {
+        const int g = h - C::NR;  // rows 0 and H2-1, then columns 0 and W2-1 of rows 1..H2-2
+        if (g < 2 * C::W2) {
+          px = g % C::W2;
+          py = g < C::W2 ? 0 : C::H2 - 1;
+        } else {
+          const int r = g - 2 * C::W2;
+          py = 1 + (r >> 1);
+          px = (r & 1) ? C::W2 - 1 : 0;
+        }
+      }
+    }
+    if constexpr (kQ) {
+      ro[s] = py * C::W2 + px;
+      have[s] = hs;
+    }
+    yoff[s] = hs ? (unsigned)(wrap_any(x0 + px - 2, d.nx) + d.nx * wrap_any(y0 + py - 2, d.ny))
+                      : 0u;
+  }
+  static_assert(NRO == 2 * C::W2 + 2 * (C::H2 - 2), "outer ring");
+  static_assert(!kQ || C::LR == 1 || C::OR + C::LR <= LS, "ring slots");
   unsigned ooff[C::OR];  // tile sites, wrapped; oin: the site is an output
   bool oin[C::OR];
 #pragma unroll
@@ -652,7 +707,8 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
   }
   // the 1-site ring around the tile: E1 coordinates
   int rmx[C::LR], rmy[C::LR];
-  unsigned roff[PAIR == 1 ? C::LR : 1];  // PAIR 1: psi at the ring sites of stage A
+  constexpr bool kReload = PAIR == 1 && !kQ;
+  unsigned roff[kReload ? C::LR : 1];  // reload path: psi at the ring sites of stage A
 #pragma unroll
   for (int s = 0; s < C::LR; ++s) {
     const int rr = t + s * NT;
@@ -660,19 +716,19 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
     if (rr < C::NR) ring1_site<C, OY>(rr, mx, my);
     rmx[s] = rr < C::NR ? mx : -1;
     rmy[s] = my;
-    if constexpr (PAIR == 1)
+    if constexpr (kReload)
       roff[s] = rr < C::NR ? (unsigned)(wrap_any(x0 + mx - 1, d.nx) + d.nx * wrap_any(y0 + my - 1, d.ny))
                            : 0u;
   }
 
   // S1+S2 does not store y3 = psi + dt/2 k2; S3+S4 forms it from psi and k2 as it loads them
   // (stage_axpy, the same bits) -- 7 words per site and step instead of 9 (+ D)
-  T yreg[C::L2], dreg[FVM ? C::L2 : 1], kreg[PAIR == 1 ? C::L2 : 1];
+  T yreg[LS], dreg[FVM ? LS : 1], kreg[PAIR == 1 ? LS : 1];
   auto load = [&](int plane) {
     const size_t zo = (size_t)d.plane * (size_t)wrap_any(plane, d.nz);
 #pragma unroll
-    for (int s = 0; s < C::L2; ++s)
-      if (t + s * NT < C::N2) {
+    for (int s = 0; s < LS; ++s)
+      if (has(s)) {
         if constexpr (PAIR == 1) {
           yreg[s] = psi[zo + yoff[s]];
           kreg[s] = b[zo + yoff[s]];
@@ -682,10 +738,14 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
         if constexpr (FVM) dreg[s] = Dfield[zo + yoff[s]];
       }
   };
-  T pe[PAIR == 1 ? C::LR : 1];  // PAIR 1: psi at the ring sites, plane of k_A
-  T pt[PAIR == 1 ? C::OR : 1];  // PAIR 1: psi at the tile sites, plane of k_A
+  // PAIR 1: psi and k2 of the stored planes as loaded (newest first): tile slots keep three
+  // planes (k_A's plane is the last but one, the output plane the one before), ring slots one
+  // (the reload path: pB, rB hold psi of k_A's plane, pC and kC2 psi and k2 of the output plane)
+  constexpr int NQ = PAIR == 1 ? C::OR : 1, NRQ = PAIR == 1 ? C::LR : 1;
+  T pA[NQ] = {}, pB[NQ] = {}, pC[NQ] = {}, kA2[NQ] = {}, kB2[NQ] = {}, kC2[NQ] = {};
+  T rA[NRQ] = {}, rB[NRQ] = {};
   T k0r[PAIR == 1 ? C::OR : 1] = {}, k1r[PAIR == 1 ? C::OR : 1] = {};  // k_A, newest first
-  T pv[C::OR] = {}, av[C::OR] = {}, bv[C::OR] = {};
+  T av[PAIR == 1 ? C::OR : 1] = {};  // PAIR 1: k1 of the output site
   // FVM: z-face means along this thread's columns: tile sites keep the last three planes'
   // (newest first), ring sites the last one
   T fzt0[FVM ? C::OR : 1] = {}, fzt1[FVM ? C::OR : 1] = {}, fzt2[FVM ? C::OR : 1] = {};
@@ -696,33 +756,53 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
   for (int p = 0; p <= last; ++p) {
     const int s0 = p % 3, s1 = (p + 2) % 3, s2 = (p + 1) % 3;  // slots of p, p-1, p-2
     const int d0 = p % ND, d1 = (p + ND - 1) % ND;               // D slots of p, p-1
+    if constexpr (PAIR == 1 && kQ) {  // the plane stored now becomes the newest
 #pragma unroll
-    for (int s = 0; s < C::L2; ++s)
-      if (t + s * NT < C::N2) {
-        if constexpr (PAIR == 1) ys[s0][t + s * NT] = stage_axpy(yreg[s], co.half_dt, kreg[s]);
-        else ys[s0][t + s * NT] = yreg[s];
-        if constexpr (FVM) ds[d0][t + s * NT] = dreg[s];
+      for (int q = 0; q < C::OR; ++q) {
+        pC[q] = pB[q];
+        pB[q] = pA[q];
+        pA[q] = yreg[q];
+        kC2[q] = kB2[q];
+        kB2[q] = kA2[q];
+        kA2[q] = kreg[q];
+      }
+#pragma unroll
+      for (int s = 0; s < C::LR; ++s) {
+        rB[s] = rA[s];
+        rA[s] = yreg[C::OR + s];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < LS; ++s)
+      if (has(s)) {
+        if constexpr (PAIR == 1) ys[s0][rix(s)] = stage_axpy(yreg[s], co.half_dt, kreg[s]);
+        else ys[s0][rix(s)] = yreg[s];
+        if constexpr (FVM) ds[d0][rix(s)] = dreg[s];
       }
     if (p < last) load(z0 - 1 + p);
     if constexpr (PAIR == 1) {
-      if (p >= 2) {
-        const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 3 + p, d.nz);
+      // psi of k_A's plane z0-3+p (tile: pB, ring: rB) and psi, k2 of the output plane
+      // z0-4+p (pC, kC2) are in registers (kQ) or reloaded; k1 of the output plane comes
+      // from HBM
+      if constexpr (kReload) {
+        if (p >= 2) {
+          const T* pp = psi + (size_t)d.plane * (size_t)wrap_any(z0 - 3 + p, d.nz);
 #pragma unroll
-        for (int s = 0; s < C::LR; ++s)
-          if (t + s * NT < C::NR) pe[s] = pp[roff[s]];
+          for (int s = 0; s < C::LR; ++s)
+            if (t + s * NT < C::NR) rB[s] = pp[roff[s]];
 #pragma unroll
-        for (int q = 0; q < C::OR; ++q) pt[q] = pp[ooff[q]];
+          for (int q = 0; q < C::OR; ++q) pB[q] = pp[ooff[q]];
+        }
       }
       if (p >= 4) {
         const size_t kb = (size_t)d.plane * (size_t)(z0 - 4 + p);
 #pragma unroll
-        for (int q = 0; q < C::OR; ++q) {
+        for (int q = 0; q < C::OR; ++q)
           if (oin[q]) {
-            pv[q] = __builtin_nontemporal_load(psi + kb + ooff[q]);
+            if constexpr (kReload) pC[q] = __builtin_nontemporal_load(psi + kb + ooff[q]);
             av[q] = __builtin_nontemporal_load(a + kb + ooff[q]);
-            bv[q] = __builtin_nontemporal_load(b + kb + ooff[q]);
+            if constexpr (kReload) kC2[q] = __builtin_nontemporal_load(b + kb + ooff[q]);
           }
-        }
       }
     }
     __syncthreads();
@@ -731,9 +811,9 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
       // between a site and its +x (+y) neighbour, shared by both cells and both stages
       const T* dp = ds[d0];
 #pragma unroll
-      for (int s = 0; s < C::L2; ++s) {
-        const int idx = t + s * NT;
-        if (idx < C::N2) {
+      for (int s = 0; s < LS; ++s) {
+        const int idx = rix(s);
+        if (has(s)) {
           const int mx = idx % C::W2, my = idx / C::W2;
           if (mx < C::W2 - 1) fx[s0][idx] = face_coef(dp[idx], dp[idx + 1], lc.ihx2);
           if (my < C::H2 - 1) fy[s0][idx] = face_coef(dp[idx], dp[idx + C::W2], lc.ihy2);
@@ -786,7 +866,7 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           if (own && oin[q])
             __builtin_nontemporal_store(kv, a + (size_t)d.plane * (size_t)zA + ooff[q]);
         } else {
-          y2s[s0][e] = stage_axpy(pt[q], co.dt, kv);
+          y2s[s0][e] = stage_axpy(pB[q], co.dt, kv);
           k1r[q] = k0r[q];
           k0r[q] = kv;
         }
@@ -805,14 +885,7 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           const T kv = kA(mx, my, fzp, fzm);
           const int e = my * C::W1 + mx;
           if constexpr (PAIR == 0) y2s[s0][e] = stage_axpy(ys[s1][(my + 1) * C::W2 + (mx + 1)], co.half_dt, kv);
-          else y2s[s0][e] = stage_axpy(pe[s], co.dt, kv);
-        }
-      }
-      if constexpr (PAIR == 0) {  // base psi of the output plane z0-4+p (input plane p-2)
-        if (p >= 4) {
-#pragma unroll
-          for (int q = 0; q < C::OR; ++q)
-            pv[q] = ys[s2][(oy0 + q * C::ROWS + 2) * C::W2 + (ox + 2)];
+          else y2s[s0][e] = stage_axpy(rB[s], co.dt, kv);
         }
       }
     }
@@ -836,8 +909,8 @@ __global__ __launch_bounds__(NT) void k_diff_pair3d(const T* __restrict__ Dfield
           if constexpr (PAIR == 0) {
             __builtin_nontemporal_store(kv, b + kb + ooff[q]);
           } else {
-            const T bsum = bv[q] + k1r[q];  // k3 of this plane (k_A pass of iteration p-1)
-            __builtin_nontemporal_store(rk4_final(pv[q], av[q], bsum, kv, co.sixth_dt, co.third_dt),
+            const T bsum = kC2[q] + k1r[q];  // k3 of this plane (k_A pass of iteration p-1)
+            __builtin_nontemporal_store(rk4_final(pC[q], av[q], bsum, kv, co.sixth_dt, co.third_dt),
                                         out + kb + ooff[q]);
           }
         }
