@@ -40,6 +40,16 @@ struct RingAt {
   }
 };
 
+// Values per site of an operator that declares NOUT (and provides values(F, v)), else 0.
+template <class Op, class = void>
+struct ZOut {
+  static constexpr int N = 0;
+};
+template <class Op>
+struct ZOut<Op, std::void_t<decltype(Op::NOUT)>> {
+  static constexpr int N = Op::NOUT;
+};
+
 template <class T, int OX, int OY, class Op>
 __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f,
                                                 T* __restrict__ out, Dims d, int ntx, int nty,
@@ -89,6 +99,18 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f,
   }
   load(z0 + R);
   const int ox = t % OX, oy0 = t / OX;
+  // More than 16 bytes of output per site: a wave is one tile row (OX = 64 = the wave size) for
+  // each q, so it stages the row's values in LDS and writes them 64 consecutive words per store
+  // instruction instead of one word of each site at the site stride.  Only the wave itself
+  // touches its row buffer: wave-level ordering suffices.
+  constexpr int NO = ZOut<Op>::N;
+  constexpr bool kRow = NO > 1 && NO * sizeof(T) > 16 && OX == 64;
+  __shared__ T orow[kRow ? Z::ROWS : 1][kRow ? OX * NO : 1];
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
   for (int k = 0; k < nout; ++k) {
     store((k + 2 * R) % Z::NS);
     __syncthreads();
@@ -100,7 +122,25 @@ __global__ __launch_bounds__(256) void k_zmarch(Op op, const T* __restrict__ f,
 #pragma unroll
     for (int q = 0; q < Z::OR; ++q) {
       const int oy = oy0 + q * Z::ROWS;
-      if (x0 + ox < d.nx && y0 + oy < d.ny) {
+      if constexpr (kRow) {
+        // every lane reads its ring position (in range for the whole tile), the row's sites
+        // inside the grid are written
+        const RingAt<T, R, Z::PN, Z::NS, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
+        T v[NO];
+        op.values(F, v);
+        T* rw = orow[oy0];
+#pragma unroll
+        for (int j = 0; j < NO; ++j) rw[ox * NO + j] = v[j];
+        wave_sync();
+        const int rowlen = y0 + oy < d.ny ? min(OX, d.nx - x0) * NO : 0;
+        T* gp = out + (size_t)NO * (zo + (size_t)d.nx * (size_t)(y0 + oy) + (size_t)x0);
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+          const int e = ox + OX * j;
+          if (e < rowlen) gp[e] = rw[e];
+        }
+        wave_sync();
+      } else if (x0 + ox < d.nx && y0 + oy < d.ny) {
         const RingAt<T, R, Z::PN, Z::NS, Z::PW> F{ring, slot, (oy + R) * Z::PW + (ox + R)};
         op(F, out, zo + (size_t)(x0 + ox) + (size_t)d.nx * (size_t)(y0 + oy));
       }
